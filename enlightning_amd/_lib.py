@@ -1,0 +1,92 @@
+"""ctypes binding of libenlightning_hip.so (declared in include/enlightning_hip.h).
+
+The library is the product path: there is no CPU fallback.  Loading fails
+with an ImportError-like RuntimeError when the shared object is missing.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libenlightning_hip.so")
+
+VARIANTS = [
+    "hybrid_high_llf", "hybrid_high", "hybrid_low_llf", "hybrid_low",
+    "weno_high_llf", "weno_high", "weno_low_llf", "weno_low", "drp",
+]
+VARIANT_ID = {n: i for i, n in enumerate(VARIANTS)}
+KERNEL_KINDS = ["rhs", "runge_kutta", "update_state", "physical_bc", "copy_interior"]
+
+# Every exported function and its prototype (argtypes, restype).
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_RHS_ARGS = [_P] * 27  # all by reference / pointer (enlightningfort.h:23-49)
+
+
+class EnlGas(ctypes.Structure):
+    """Mirror of enl_gas_t."""
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "weno_alpha", "c", "gamma1", "p0", "T0", "c_lr", "rho0", "R", "c_p", "T_star_n", "T_star_o",
+        "mu", "muB", "kappa", "c_v", "R_tilde", "M", "tau_n", "tau_o")] + [
+        ("tag_tolerance", ctypes.c_double * 3)]
+
+
+PROTOTYPES = {
+    "rhs_": (_RHS_ARGS, None),
+    "rhs_hybrid_high_llf_": (_RHS_ARGS, None),
+    "rhs_hybrid_high_": (_RHS_ARGS, None),
+    "rhs_hybrid_low_llf_": (_RHS_ARGS, None),
+    "rhs_hybrid_low_": (_RHS_ARGS, None),
+    "rhs_weno_high_llf_": (_RHS_ARGS, None),
+    "rhs_weno_high_": (_RHS_ARGS, None),
+    "rhs_weno_low_llf_": (_RHS_ARGS, None),
+    "rhs_weno_low_": (_RHS_ARGS, None),
+    "rhs_drp_": (_RHS_ARGS, None),
+    "enl_rhs_variant": ([_I] + _RHS_ARGS + [_P], _I),
+    "runge_kutta_": ([_P] * 14, None),
+    "update_state_": ([_P] * 23, None),
+    "tag_cells_": ([_P] * 12, None),
+    "enl_set_stream": ([_P], _I),
+    "enl_sync": ([], _I),
+    "enl_last_error": ([ctypes.c_char_p, ctypes.c_size_t], _I),
+    "enl_version": ([], ctypes.c_char_p),
+    "enl_fill_physical_bc": ([_I] * 6 + [_P] * 6, _I),
+    "enl_profile_start": ([_I], _I),
+    "enl_profile_stop": ([_P, _P, _I], _I),
+    "enl_advance_patch": ([_I] * 6 + [_P, _P, ctypes.POINTER(EnlGas), _P, _I, _P, _P, _P, _D, _I, _I]
+                          + [_P] * 6, _I),
+}
+
+_lib = None
+
+
+def load():
+    """Load the HIP library (once).  Raises RuntimeError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            "libenlightning_hip.so is not built (expected at %s); run "
+            "`python -c 'import __graft_entry__ as g; g.build()'`" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (argtypes, restype) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = restype
+    _lib = lib
+    return lib
+
+
+def last_error():
+    lib = load()
+    buf = ctypes.create_string_buffer(512)
+    code = lib.enl_last_error(buf, 512)
+    return code, buf.value.decode()
+
+
+def check(rc=None):
+    """Raise if the library recorded an error (rc < 0 or a pending message)."""
+    code, msg = last_error()
+    if (rc is not None and rc != 0) or code != 0:
+        raise RuntimeError("enlightning_hip error %s: %s" % (code if code else rc, msg))

@@ -1,0 +1,197 @@
+// enl_driver.hip -- host-side patch loop around the kernels for a patch that
+// covers the whole domain: the per-patch sequence of enlightning::singleStep
+// (enlightning.C:459-583) inside the SAMRAI MethodOfLinesIntegrator stage loop
+// (copy current -> scratch, then per stage: ghost fill incl. the physical
+// boundary of enlightning.C:790-1013, rhs, runge_kutta, update_state).
+// Everything stays on the device; the host only enqueues launches.
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "enl_device.h"
+#include "enl_kernels.h"
+#include "enlightning_hip.h"
+
+#pragma clang fp contract(off)
+
+using namespace enl;
+
+namespace {
+
+// Per-kernel-class HIP event timing of the patch loop (enl_profile_start /
+// enl_profile_stop).  Events are recorded on the launch stream around each
+// launch; elapsed times are summed when the profile is stopped.
+struct KernelTimer {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<int> kind;  // kind of the launch bracketed by events 2i, 2i+1
+  bool overflow = false;
+
+  void start(int capacity) {
+    while ((int)pool.size() < 2 * capacity) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) break;
+      pool.push_back(e);
+    }
+    used = 0;
+    kind.clear();
+    overflow = false;
+    on = true;
+  }
+  // Returns the event pair index or -1.
+  long begin(int k, hipStream_t s) {
+    if (!on) return -1;
+    if (used + 2 > pool.size()) {
+      overflow = true;
+      return -1;
+    }
+    long idx = (long)used / 2;
+    (void)hipEventRecord(pool[used], s);
+    used += 2;
+    kind.push_back(k);
+    return idx;
+  }
+  void end(long idx, hipStream_t s) {
+    if (idx >= 0) (void)hipEventRecord(pool[2 * idx + 1], s);
+  }
+};
+KernelTimer g_timer;
+
+}  // namespace
+
+extern "C" int enl_profile_start(int capacity) {
+  if (capacity < 1) return set_error(-2, "profile capacity must be positive");
+  g_timer.start(capacity);
+  return 0;
+}
+
+extern "C" int enl_profile_stop(double* ms, int* count, int nkinds) {
+  g_timer.on = false;
+  for (int k = 0; k < nkinds; k++) {
+    ms[k] = 0;
+    count[k] = 0;
+  }
+  for (size_t i = 0; i < g_timer.kind.size(); i++) {
+    hipEvent_t a = g_timer.pool[2 * i], b = g_timer.pool[2 * i + 1];
+    if (hipEventSynchronize(b) != hipSuccess) return set_error(-5, "hipEventSynchronize");
+    float t = 0;
+    if (hipEventElapsedTime(&t, a, b) != hipSuccess) return set_error(-5, "hipEventElapsedTime");
+    int k = g_timer.kind[i];
+    if (k < nkinds) {
+      ms[k] += t;
+      count[k] += 1;
+    }
+  }
+  return g_timer.overflow ? set_error(-7, "profile capacity exceeded; later launches were not timed") : 0;
+}
+
+extern "C" int enl_advance_patch(int variant, int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw,
+                                 const double* xlo, const double* dx, const enl_gas_t* gas, const int* reflect,
+                                 int order, const double* alpha_1, const double* alpha_2, const double* beta,
+                                 double dt, int nsteps, int level, double* w_n, double* pres, double* temp,
+                                 double* source, double* ww, double* KK) {
+  if (variant < 0 || variant >= kNumVariants) return set_error(-1, "unknown rhs variant");
+  int need = (variant == kWenoLowLlf || variant == kWenoLow) ? 2 : 3;
+  if (gcw < need) return set_error(-2, "ghost width below the stencil reach");
+  if (ilast0 < ifirst0 || ilast1 < ifirst1) return set_error(-2, "empty or invalid patch box");
+  if (order < 1 || order > 8) return set_error(-2, "RK order out of range");
+  const double* arrays[6] = {w_n, pres, temp, source, ww, KK};
+  for (int i = 0; i < 6; i++)
+    if (!is_device_ptr(arrays[i])) return set_error(-4, "enl_advance_patch needs device arrays");
+  hipStream_t s = lib_stream();
+  Geom G = make_geom(ifirst0, ilast0, ifirst1, ilast1, gcw, gcw);
+
+  RhsArgs ra;
+  ra.w = w_n;
+  ra.p = pres;
+  ra.T = temp;
+  // rhs reads the source through the interior context (enlightning.C:475-477):
+  // point it at interior cell (0,0) of the ghosted source array.
+  ra.H0 = source + G.g0 + (long)G.g1 * G.sy;
+  ra.h0_sy = G.sy;
+  ra.KK = KK;
+  ra.switch_map = nullptr;
+  ra.geom = G;
+  ra.dx = dx[0];
+  ra.dy = dx[1];
+  ra.ylo = xlo[1];
+  ra.alpha = gas->weno_alpha;
+  ra.kappa = gas->kappa;
+  ra.mu = gas->mu;
+  ra.muB = gas->muB;
+  ra.R = gas->R;
+  ra.T_star_n = gas->T_star_n;
+  ra.tau_n = gas->tau_n;
+  ra.c = gas->c;
+  ra.c_lr = gas->c_lr;
+  ra.ep = sensor_eps(gas->tag_tolerance[2]);
+  ra.level = level;
+
+
+  RkArgs rk;
+  rk.w = w_n;
+  rk.ww = ww;
+  rk.KK = KK;
+  rk.geom = G;
+  rk.dt = dt;
+
+  StateArgs st;
+  st.w = w_n;
+  st.p = pres;
+  st.T = temp;
+  st.geom = G;
+  st.dy = dx[1];
+  st.ylo = xlo[1];
+  st.rho0 = gas->rho0;
+  st.c = gas->c;
+  st.gamma1 = gas->gamma1;
+  st.c_p = gas->c_p;
+  st.p0 = gas->p0;
+  st.T0 = gas->T0;
+  st.c_lr = gas->c_lr;
+  st.c_v = gas->c_v;
+  st.R = gas->R;
+
+  BcArgs bc;
+  bc.w = w_n;
+  bc.f[0] = pres;
+  bc.f[1] = temp;
+  bc.f[2] = source;
+  bc.geom = G;
+  for (int k = 0; k < 4; k++) {
+    bc.touch[k] = 1;
+    bc.reflect[k] = reflect[k];
+  }
+
+  int rc = 0;
+  long t;
+  hipError_t e = hipSuccess;
+  for (int step = 0; step < nsteps && e == hipSuccess; step++) {
+    t = g_timer.begin(ENL_KERNEL_COPY, s);
+    e = launch_copy_interior(ww, w_n, G, s);
+    g_timer.end(t, s);
+    for (int stage = 0; stage < order && e == hipSuccess; stage++) {
+      rk.alpha_1 = alpha_1[stage];
+      rk.alpha_2 = alpha_2[stage];
+      rk.beta = beta[stage];
+      t = g_timer.begin(ENL_KERNEL_BC, s);
+      e = launch_physical_bc(bc, s);
+      g_timer.end(t, s);
+      if (e != hipSuccess) break;
+      t = g_timer.begin(ENL_KERNEL_RHS, s);
+      e = launch_rhs(variant, ra, s);
+      g_timer.end(t, s);
+      if (e != hipSuccess) break;
+      t = g_timer.begin(ENL_KERNEL_RK, s);
+      e = launch_runge_kutta(rk, s);
+      g_timer.end(t, s);
+      if (e != hipSuccess) break;
+      t = g_timer.begin(ENL_KERNEL_STATE, s);
+      e = launch_update_state(st, s);
+      g_timer.end(t, s);
+    }
+  }
+  if (e != hipSuccess) rc = check_hip(e, "patch loop launch");
+  return rc;
+}

@@ -1,0 +1,92 @@
+// enl_kernels.h -- launch interface of the HIP kernels of the per-patch
+// update path (internal C++ API; the exported C ABI sits in enl_abi.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace enl {
+
+// Geometry of one patch in the SAMRAI CellData layout: interior nx x ny,
+// ghost widths g0, g1; ghosted arrays have row stride sy = nx + 2*g0 and
+// component stride sk = sy * (ny + 2*g1).  Interior-only arrays use
+// row stride nx and component stride nx*ny.
+struct Geom {
+  int nx, ny, g0, g1;
+  long sy, sk;
+};
+
+inline Geom make_geom(int ifirst0, int ilast0, int ifirst1, int ilast1, int g0, int g1) {
+  Geom g;
+  g.nx = ilast0 - ifirst0 + 1;
+  g.ny = ilast1 - ifirst1 + 1;
+  g.g0 = g0;
+  g.g1 = g1;
+  g.sy = g.nx + 2L * g0;
+  g.sk = g.sy * (g.ny + 2L * g1);
+  return g;
+}
+
+struct RhsArgs {
+  const double* w;   // ghosted, 5 components
+  const double* p;   // ghosted
+  const double* T;   // ghosted
+  const double* H0;  // source at interior cell (0,0); row stride h0_sy
+  long h0_sy;        // nx for an interior array, sy for a ghosted one
+  double* KK;        // interior, 5 components
+  int* switch_map;   // optional: interior int map of the DRP(0)/WENO(1) choice
+  Geom geom;
+  double dx, dy, ylo;
+  double alpha, kappa, mu, muB, R, T_star_n, tau_n, c, c_lr;
+  double ep;  // sensor epsilon from tag_tolerance(2)
+  int level;
+};
+
+struct RkArgs {
+  double* w;          // ghosted, updated on the interior
+  const double* ww;   // interior
+  const double* KK;   // interior
+  Geom geom;
+  double alpha_1, alpha_2, beta, dt;
+};
+
+struct StateArgs {
+  const double* w;  // ghosted
+  double* p;        // ghosted, interior written
+  double* T;        // ghosted, interior written
+  Geom geom;
+  double dy, ylo;
+  double rho0, c, gamma1, c_p, p0, T0, c_lr, c_v, R;
+};
+
+struct TagArgs {
+  int* tags;          // interior
+  const double* w;    // ghosted
+  const double* H0;   // ghosted
+  Geom geom;
+  int refine_tag_val;
+  double ep0, ep1;    // from tag_tolerance(0), tag_tolerance(1)
+};
+
+struct BcArgs {
+  double* w;       // ghosted, 5 components
+  double* f[3];    // ghosted scalars (pres, temp, source); entries may be null
+  Geom geom;
+  int touch[4];    // side on the physical boundary: left, right, bottom, top
+  int reflect[4];  // REFLECT (1) or ABSORB (0) per side
+};
+
+hipError_t launch_rhs(int variant, const RhsArgs& a, hipStream_t s);
+hipError_t launch_runge_kutta(const RkArgs& a, hipStream_t s);
+hipError_t launch_update_state(const StateArgs& a, hipStream_t s);
+hipError_t launch_tag_cells(const TagArgs& a, hipStream_t s);
+hipError_t launch_physical_bc(const BcArgs& a, hipStream_t s);
+// ww(interior) <- w(interior), 5 components
+hipError_t launch_copy_interior(double* ww, const double* w, const Geom& g, hipStream_t s);
+
+// Library runtime state shared by the exported entry points (enl_abi.hip).
+hipStream_t lib_stream();
+int set_error(int code, const char* fmt, ...);
+int check_hip(hipError_t e, const char* what);
+bool is_device_ptr(const void* p);
+
+}  // namespace enl

@@ -1,0 +1,102 @@
+"""Synthetic inputs of the measured workloads.
+
+A workload is one square patch of n x n interior cells with g ghost cells on
+[0, L]^2, reflecting walls on all sides, a superposition of Gaussian
+overpressure pulses dp(x) = A*p0*exp(-|x-x_c|^2 / (2 sigma^2)) and optional
+uniform noise of amplitude noise*p0.  The reference has no overpressure
+initial condition (it only injects a mass source, enlightning.C:2176-2192),
+so the pulse is realised as a density perturbation at zero entropy: from the
+second-order EOS of update_state.m4:90-94 with s = 0,
+    dp = c^2 (rho' + a rho'^2),  a = (gamma-1)/(2 rho0)
+    => rho' = (-1 + sqrt(1 + 4 a dp / c^2)) / (2 a),
+with rho*u = rho*v = rho*s = 0 and rho*T_n = rho*T0 (enlightning.C:418-424).
+
+Arrays use the SAMRAI CellData layout: numpy C-order shape (5, Ny, Nx) for
+w_n (component slowest, i fastest) and (Ny, Nx) for scalars.
+"""
+from dataclasses import dataclass
+
+import numpy as np
+
+
+@dataclass
+class Workload:
+    name: str
+    n: int              # interior cells per side
+    g: int = 4          # ghost width
+    L: float = 1.0      # domain side, m
+    steps: int = 100
+    cfl: float = 0.5
+    variant: str = "hybrid_high_llf"
+    seed: int = 0
+    npulses: int = 1
+    noise: float = 0.0  # uniform noise amplitude / p0
+    # single-pulse parameters (npulses == 1 and fixed == True)
+    fixed: bool = False
+    amp: float = 0.5
+    sigma: float = 0.02
+    center: tuple = (0.5, 0.5)
+    amp_range: tuple = (0.05, 0.8)
+    sigma_range: tuple = (0.005, 0.03)
+
+    @property
+    def dx(self):
+        return self.L / self.n
+
+    def dt(self, gas):
+        # enlightning.C:446-447: dt = cfl * min(dx, dy) / c
+        return self.cfl * (self.dx / gas.c)
+
+    def pulses(self):
+        """(centers [m], amplitudes [dp/p0], sigmas [m]) of the pulses."""
+        if self.fixed:
+            return (np.array([self.center], float), np.array([self.amp]), np.array([self.sigma]))
+        rng = np.random.default_rng(self.seed)
+        centers = rng.uniform(0.0, self.L, size=(self.npulses, 2))
+        amps = rng.uniform(*self.amp_range, size=self.npulses)
+        sigmas = rng.uniform(*self.sigma_range, size=self.npulses) * self.L
+        return centers, amps, sigmas
+
+
+# The plain-method workloads: a 256^2 CPU-runnable case and the 4096^2 case
+# the single-GPU throughput is quoted on.
+SMALL_256 = Workload("pulse_256_single", n=256, steps=100, seed=0, fixed=True, amp=0.5, sigma=0.02,
+                     center=(0.5, 0.5), noise=1e-6)
+PULSES_4096 = Workload("pulses_4096_8", n=4096, steps=500, seed=1, npulses=8)
+
+
+def overpressure(wl):
+    """dp [Pa / p0 units] on the ghosted cell centres, shape (Ny, Nx)."""
+    n, g, dx = wl.n, wl.g, wl.dx
+    xc = (np.arange(-g, n + g) + 0.5) * dx
+    X = xc[None, :]
+    Y = xc[:, None]
+    dp = np.zeros((n + 2 * g, n + 2 * g))
+    centers, amps, sigmas = wl.pulses()
+    for (cx, cy), a, s in zip(centers, amps, sigmas):
+        # separable evaluation keeps this cheap at 4096^2
+        ex = np.exp(-((X - cx) ** 2) / (2 * s * s))
+        ey = np.exp(-((Y - cy) ** 2) / (2 * s * s))
+        dp += a * (ey * ex)
+    if wl.noise:
+        rng = np.random.default_rng(wl.seed)
+        dp[g:g + n, g:g + n] += rng.uniform(-1.0, 1.0, size=(n, n)) * wl.noise
+    return dp
+
+
+def initial_fields(wl, gas):
+    """Host arrays (w, p, T, source) of the initial state.  p and T hold the
+    ambient values; the caller derives them from w with update_state and
+    fills the ghosts with the physical boundary routine."""
+    dp = overpressure(wl) * gas.p0
+    a = (gas.gamma1 - 1.0) / (2.0 * gas.rho0)
+    rho_p = (-1.0 + np.sqrt(1.0 + 4.0 * a * dp / gas.c ** 2)) / (2.0 * a)
+    rho = gas.rho0 + rho_p
+    N = wl.n + 2 * wl.g
+    w = np.zeros((5, N, N))
+    w[0] = rho
+    w[4] = rho * gas.T0
+    p = np.full((N, N), gas.p0)
+    T = np.full((N, N), gas.T0)
+    src = np.zeros((N, N))
+    return w, p, T, src

@@ -1,0 +1,177 @@
+/*
+ * enlightning_hip.h -- C ABI of libenlightning_hip.so, the MI355X (gfx950)
+ * implementation of Enlightning's per-patch update path.
+ *
+ * 1. Drop-in kernels.  Same symbols, argument order and by-reference
+ *    argument passing as the reference's Fortran subroutines declared in
+ *    enlightningfort.h:22-101 (SAMRAI_F77_FUNC(name,NAME) mangles to name_
+ *    with gfortran/flang), so enlightning::singleStep (enlightning.C:507-578)
+ *    and enlightning::tagGradientDetectorCells (enlightning.C:615-626) call
+ *    them unchanged.
+ *
+ *    Memory: every array argument may be a device pointer (hipMalloc'd) or
+ *    a host pointer.  Device arrays are used in place and the work is queued
+ *    on the library stream (enl_set_stream; default: the null stream) --
+ *    results are visible in stream order; call enl_sync() before reading
+ *    them from the host.  Host arrays are staged through a device mirror and
+ *    copied back before the call returns (the reference's synchronous
+ *    contract; PCIe-bound).  Scalars and the small xlo/dx/tag_tolerance
+ *    arrays are always host memory, as in the reference.
+ *
+ *    Errors: the reference subroutines have no error path.  These keep the
+ *    void signature and report through enl_last_error(); with the
+ *    environment variable ENL_ABORT_ON_ERROR=1 they abort instead
+ *    (TBOX_ERROR semantics).
+ *
+ * 2. Patch-loop entry points (enl_*): the host-side shape of the reference's
+ *    patch strategy around the kernels (physical boundary fill, the
+ *    method-of-lines RK step), operating on device-resident patch data.
+ */
+#ifndef ENLIGHTNING_HIP_H
+#define ENLIGHTNING_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* rhs variants: one per fortran/rhs_*.m4 file (makefile:6 picks one per
+ * reference build). */
+enum {
+  ENL_RHS_HYBRID_HIGH_LLF = 0, /* fortran/rhs_hybrid_high_llf.m4 (default) */
+  ENL_RHS_HYBRID_HIGH = 1,     /* fortran/rhs_hybrid_high.m4 */
+  ENL_RHS_HYBRID_LOW_LLF = 2,  /* fortran/rhs_hybrid_low_llf.m4 */
+  ENL_RHS_HYBRID_LOW = 3,      /* fortran/rhs_hybrid_low.m4 */
+  ENL_RHS_WENO_HIGH_LLF = 4,   /* fortran/rhs_weno_high_llf.m4 */
+  ENL_RHS_WENO_HIGH = 5,       /* fortran/rhs_weno_high.m4 */
+  ENL_RHS_WENO_LOW_LLF = 6,    /* fortran/rhs_weno_low_llf.m4 */
+  ENL_RHS_WENO_LOW = 7,        /* fortran/rhs_weno_low.m4 */
+  ENL_RHS_DRP = 8              /* fortran/rhs_drp.m4 */
+};
+
+/* ---- 1. drop-in kernels ------------------------------------------------ */
+
+#define ENL_RHS_ARGS                                                                          \
+  const int *ifirst0, const int *ilast0, const int *ifirst1, const int *ilast1,               \
+      const int *gcw0, const int *gcw1, const double *xlo, const double *dx,                  \
+      const double *w_n, const double *source, double *rhs, const double *pres,               \
+      const double *temp, const double *weno_alpha, const double *kappa, const double *mu,    \
+      const double *muB, const double *R, const double *T_star_n, const double *T_star_o,     \
+      const double *tau_n, const double *tau_o, const double *tag_tolerance, const int *level, \
+      const double *c, const double *c_lr, const int *nequ
+
+/* rhs: replaces SAMRAI_F77_FUNC(rhs,RHS), enlightningfort.h:23-49, called at
+ * enlightning.C:507-533.  Writes all of rhs (nx*ny*nequ, interior).  w_n,
+ * pres, temp are ghosted (gcw), source is interior.  rhs_ runs the variant
+ * named by ENL_RHS_VARIANT (name such as "hybrid_high_llf" or its number),
+ * default hybrid_high_llf (makefile:6). */
+void rhs_(ENL_RHS_ARGS);
+/* Per-variant aliases with the identical signature. */
+void rhs_hybrid_high_llf_(ENL_RHS_ARGS); /* fortran/rhs_hybrid_high_llf.m4:28-51 */
+void rhs_hybrid_high_(ENL_RHS_ARGS);     /* fortran/rhs_hybrid_high.m4:28-51 */
+void rhs_hybrid_low_llf_(ENL_RHS_ARGS);  /* fortran/rhs_hybrid_low_llf.m4:28-51 */
+void rhs_hybrid_low_(ENL_RHS_ARGS);      /* fortran/rhs_hybrid_low.m4:28-51 */
+void rhs_weno_high_llf_(ENL_RHS_ARGS);   /* fortran/rhs_weno_high_llf.m4:27-50 */
+void rhs_weno_high_(ENL_RHS_ARGS);       /* fortran/rhs_weno_high.m4:27-50 */
+void rhs_weno_low_llf_(ENL_RHS_ARGS);    /* fortran/rhs_weno_low_llf.m4:27-50 */
+void rhs_weno_low_(ENL_RHS_ARGS);        /* fortran/rhs_weno_low.m4:27-50 */
+void rhs_drp_(ENL_RHS_ARGS);             /* fortran/rhs_drp.m4:26-49 */
+
+/* rhs of any variant, plus an optional interior int map (nx*ny, may be
+ * NULL) of the scheme each cell took: 0 = DRP, 1 = WENO
+ * (rhs_hybrid_high_llf.m4:239-258).  Returns 0 or a negative error code. */
+int enl_rhs_variant(int variant, ENL_RHS_ARGS, int *switch_map);
+
+/* runge_kutta: replaces SAMRAI_F77_FUNC(runge_kutta,RUNGE_KUTTA),
+ * enlightningfort.h:51-64, called at enlightning.C:538-551.
+ * w_n(interior) = alpha_1*w + alpha_2*w_n + beta*dt*rhs; ghosts untouched. */
+void runge_kutta_(const int *ifirst0, const int *ilast0, const int *ifirst1, const int *ilast1,
+                  const int *gcw0, const int *gcw1, const double *dt, const double *alpha_1,
+                  const double *alpha_2, const double *beta, double *w_n, const double *w,
+                  const double *rhs, const int *nequ);
+
+/* update_state: replaces SAMRAI_F77_FUNC(update_state,UPDATE_STATE),
+ * enlightningfort.h:66-88, called at enlightning.C:556-578.  Note the
+ * argument order starts with dx, xlo.  Writes the interior of pres, temp. */
+void update_state_(const double *dx, const double *xlo, const int *ifirst0, const int *ilast0,
+                   const int *ifirst1, const int *ilast1, const int *gcw0, const int *gcw1,
+                   const double *w_n, double *pres, double *temp, const double *rho0, const double *c,
+                   const double *gamma1, const double *c_p, const double *p0, const double *T0,
+                   const double *c_lr, const double *c_v, const double *R_tilde, const double *M,
+                   const double *R, const int *nequ);
+
+/* tag_cells: replaces SAMRAI_F77_FUNC(tag_cells,TAG_CELLS),
+ * enlightningfort.h:90-101, called at enlightning.C:615-626.  Writes all of
+ * tags (nx*ny int, interior); w_n and source are ghosted (gcw >= 1). */
+void tag_cells_(const int *ifirst0, const int *ilast0, const int *ifirst1, const int *ilast1,
+                const int *gcw0, const int *gcw1, int *tags, const double *w_n, const double *source,
+                const int *refine_tag_val, const double *tag_tolerance, const int *nequ);
+
+/* ---- library state ------------------------------------------------------ */
+
+/* Stream for all subsequent launches (a hipStream_t; NULL = null stream). */
+int enl_set_stream(void *hip_stream);
+/* Waits for the library stream.  Returns 0 or a negative error code. */
+int enl_sync(void);
+/* Last error of this thread: returns its code (0 = none) and copies the
+ * message into buf (NUL-terminated, truncated to n).  Clears it. */
+int enl_last_error(char *buf, size_t n);
+/* Library version string. */
+const char *enl_version(void);
+
+/* ---- 2. patch-loop entry points (device pointers only) ----------------- */
+
+/* The Enlightning{constants} block (input.txt:40-59), the derived tau_n,
+ * tau_o (enlightning.C:218-234) and tag_tolerance (input.txt:4). */
+typedef struct enl_gas {
+  double weno_alpha, c, gamma1, p0, T0, c_lr, rho0, R, c_p, T_star_n, T_star_o;
+  double mu, muB, kappa, c_v, R_tilde, M, tau_n, tau_o;
+  double tag_tolerance[3];
+} enl_gas_t;
+
+/* Physical boundary fill of one patch's ghost strips: replaces
+ * enlightning::setPhysicalBoundaryConditions, enlightning.C:790-1013.
+ * touch[4]: side (left, right, bottom, top) lies on the domain boundary;
+ * reflect[4]: REFLECT (1) or ABSORB (0).  pres, temp, source may be NULL. */
+int enl_fill_physical_bc(int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw0, int gcw1,
+                         const int *touch, const int *reflect, double *w_n, double *pres,
+                         double *temp, double *source);
+
+/* nsteps method-of-lines RK steps of one patch that covers the whole domain
+ * (all four sides physical): per step ww <- w_n(interior), then per stage
+ * the physical fill, rhs, runge_kutta and update_state, i.e. the per-patch
+ * sequence of enlightning::singleStep (enlightning.C:459-583) inside the
+ * SAMRAI MethodOfLinesIntegrator stage loop.  alpha_1/alpha_2/beta hold
+ * `order` coefficients (input.txt:129-138).  ww and KK are device scratch of
+ * nx*ny*5 doubles.  Returns 0 or a negative error code. */
+int enl_advance_patch(int variant, int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw,
+                      const double *xlo, const double *dx, const enl_gas_t *gas, const int *reflect,
+                      int order, const double *alpha_1, const double *alpha_2, const double *beta,
+                      double dt, int nsteps, int level, double *w_n, double *pres, double *temp,
+                      double *source, double *ww, double *KK);
+
+/* ---- measurement hooks (no reference counterpart) ---------------------- */
+
+/* Kernel classes timed by the patch-loop entry points. */
+enum {
+  ENL_KERNEL_RHS = 0,
+  ENL_KERNEL_RK = 1,
+  ENL_KERNEL_STATE = 2,
+  ENL_KERNEL_BC = 3,
+  ENL_KERNEL_COPY = 4,
+  ENL_NUM_KERNEL_KINDS = 5
+};
+
+/* Starts bracketing every launch of enl_advance_patch with HIP events on the
+ * launch stream (the timing equivalent of the reference's t_rhs, t_rk,
+ * t_state timers, enlightning.C:177-197).  capacity = launches to time. */
+int enl_profile_start(int capacity);
+/* Stops timing, waits for the events and returns, per kernel class, the
+ * summed device time in ms and the number of launches. */
+int enl_profile_stop(double *ms, int *count, int nkinds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ENLIGHTNING_HIP_H */
