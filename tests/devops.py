@@ -1,0 +1,77 @@
+"""Device operations used by the GPU tests.
+
+Normally this is the HIP library through enlightning_amd.patch with torch
+device memory.  With ENL_TEST_DRY_RUN=1 (pytest --dry-gpu) the same tests run
+on the CPU with the oracle standing in for the device, which checks the
+tests' own logic and inputs without a GPU; it proves nothing about the
+kernels.
+"""
+import os
+
+import numpy as np
+
+DRY = os.environ.get("ENL_TEST_DRY_RUN") == "1"
+
+if not DRY:
+    import torch
+
+    from enlightning_amd import patch as P
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+    def host(t):
+        torch.cuda.synchronize()
+        return t.cpu().numpy()
+
+    def zeros(shape, dtype=np.float64):
+        return torch.zeros(shape, dtype=torch.int32 if dtype == np.int32 else torch.float64, device="cuda")
+
+    def full(shape, value, dtype=np.float64):
+        return torch.full(shape, value, dtype=torch.int32 if dtype == np.int32 else torch.float64, device="cuda")
+
+    def sync():
+        torch.cuda.synchronize()
+
+    def interior_copy(t, g):
+        return t[g:-g, g:-g].contiguous()
+
+    def all_finite(t):
+        return bool(torch.isfinite(t).all().item())
+
+    DevicePatch = P.DevicePatch
+    use_stream = P.use_torch_stream
+else:
+    import oracle as P  # noqa: N812  (the oracle stands in for the device)
+    from enlightning_amd.gas import RK3
+
+    def dev(a):
+        return np.array(a, copy=True, order="C")
+
+    def host(t):
+        return t
+
+    def zeros(shape, dtype=np.float64):
+        return np.zeros(shape, dtype)
+
+    def full(shape, value, dtype=np.float64):
+        return np.full(shape, value, dtype)
+
+    def sync():
+        pass
+
+    def interior_copy(t, g):
+        return np.ascontiguousarray(t[g:-g, g:-g])
+
+    def all_finite(t):
+        return bool(np.isfinite(t).all())
+
+    class DevicePatch(P.HostPatch):
+        def advance(self, nsteps, dt, variant=None, level=0, rk=RK3):
+            super().advance(nsteps, dt, rk, variant=variant, level=level, tile=64, nthreads=8)
+
+        def to_host(self):
+            return self.w.copy(), self.p.copy(), self.T.copy(), self.src.copy()
+
+    def use_stream():
+        pass
