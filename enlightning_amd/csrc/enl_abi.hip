@@ -60,6 +60,39 @@ bool is_device_ptr(const void* p) {
 }
 
 namespace {
+// Per-device WENO-list workspace of the rhs kernels.
+struct RhsWorkspace {
+  int* list = nullptr;
+  unsigned* count = nullptr;
+  long cap = 0;
+};
+std::mutex g_ws_mu;
+RhsWorkspace g_ws[64];
+}  // namespace
+
+int rhs_workspace(long cells, int** list, unsigned** count) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return set_error(-5, "hipGetDevice");
+  std::lock_guard<std::mutex> lock(g_ws_mu);
+  RhsWorkspace& w = g_ws[dev];
+  if (!w.count) {
+    hipError_t e = hipMalloc((void**)&w.count, sizeof(unsigned));
+    if (e != hipSuccess) return check_hip(e, "hipMalloc(rhs counter)");
+  }
+  if (w.cap < cells) {
+    if (w.list) (void)hipFree(w.list);
+    w.list = nullptr;
+    w.cap = 0;
+    hipError_t e = hipMalloc((void**)&w.list, sizeof(int) * (size_t)cells);
+    if (e != hipSuccess) return check_hip(e, "hipMalloc(rhs list)");
+    w.cap = cells;
+  }
+  *list = w.list;
+  *count = w.count;
+  return 0;
+}
+
+namespace {
 
 // Device mirrors for host-pointer calls.  Grow-only buffers, one per argument
 // slot, guarded by one mutex (host-pointer calls are serialized; device-pointer
@@ -200,6 +233,9 @@ int do_rhs(int variant, const int* ifirst0, const int* ilast0, const int* ifirst
   a.c_lr = *c_lr;
   a.ep = sensor_eps(tag_tolerance[2]);
   a.level = *level;
+  a.list = nullptr;
+  a.count = nullptr;
+  if (rhs_needs_list(variant, a.level) && (rc = rhs_workspace((long)G.nx * G.ny, &a.list, &a.count))) return rc;
   if ((rc = check_hip(launch_rhs(variant, a, g_stream), "rhs launch"))) return rc;
   return finish(args, switch_map ? 6 : 5);
 }

@@ -127,6 +127,12 @@ extern "C" int enl_advance_patch(int variant, int ifirst0, int ilast0, int ifirs
   ra.c_lr = gas->c_lr;
   ra.ep = sensor_eps(gas->tag_tolerance[2]);
   ra.level = level;
+  ra.list = nullptr;
+  ra.count = nullptr;
+  if (rhs_needs_list(variant, level)) {
+    int rc = rhs_workspace((long)G.nx * G.ny, &ra.list, &ra.count);
+    if (rc) return rc;
+  }
 
 
   RkArgs rk;

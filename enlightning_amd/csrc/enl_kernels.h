@@ -39,6 +39,12 @@ struct RhsArgs {
   double alpha, kappa, mu, muB, R, T_star_n, tau_n, c, c_lr;
   double ep;  // sensor epsilon from tag_tolerance(2)
   int level;
+  // WENO-cell list of the split hybrid path (nx*ny ints) and its length;
+  // needed when rhs_needs_list(variant, level)
+  int* list;
+  unsigned* count;
+  // reciprocals of the grid constants (filled by launch_rhs)
+  double inv_dx2, inv_dy2, inv_4dxdy, inv_2dx, inv_2dy, inv_mdx, inv_mdy, inv_dx, inv_dy;
 };
 
 struct RkArgs {
@@ -76,6 +82,7 @@ struct BcArgs {
 };
 
 hipError_t launch_rhs(int variant, const RhsArgs& a, hipStream_t s);
+bool rhs_needs_list(int variant, int level);
 hipError_t launch_runge_kutta(const RkArgs& a, hipStream_t s);
 hipError_t launch_update_state(const StateArgs& a, hipStream_t s);
 hipError_t launch_tag_cells(const TagArgs& a, hipStream_t s);
@@ -88,5 +95,8 @@ hipStream_t lib_stream();
 int set_error(int code, const char* fmt, ...);
 int check_hip(hipError_t e, const char* what);
 bool is_device_ptr(const void* p);
+// Device workspace of the current device (grow-only, owned by the library):
+// the WENO-cell list for `cells` cells and its counter.
+int rhs_workspace(long cells, int** list, unsigned** count);
 
 }  // namespace enl

@@ -1,0 +1,532 @@
+// enl_rhs.hip -- gfx950 kernels of the reference's rhs subroutine
+// (fortran/rhs_*.m4, nine variants; one template instance per variant).
+//
+// Three kernels, chosen per call by variant and level:
+//
+//   rhs_split_kernel    hybrid variants on level 0, and rhs_drp.  A 256-lane
+//                       workgroup owns a 32 x 8 tile; tile + 3-cell halo of
+//                       rho, p, T and the four quotients u=w1/w0, v=w2/w0,
+//                       s=w3/w0, Tn=w4/w0 is staged in LDS (29.8 KB).  Each
+//                       lane computes the physical source H and the sensor;
+//                       DRP cells finish here.  WENO cells store H in their
+//                       rhs slots and are appended (wave ballot + one atomic
+//                       per wave) to a compact list, so no wave runs the ~10x
+//                       more expensive WENO path for a few of its lanes.
+//   rhs_weno_list_kernel  one lane per listed WENO cell, stencil read from
+//                       global memory (the flagged cells cluster along shock
+//                       fronts, so the reads hit L2), direction by direction.
+//   rhs_dense_kernel    every cell WENO (rhs_weno_* variants, and hybrid
+//                       variants on levels > 0 where the sensor is off):
+//                       tile staging as above plus the raw w_k, all lanes on
+//                       the WENO path.
+//
+// All three keep the reference operation order (rhs_hybrid_high_llf.m4),
+// compile with -ffp-contract=off and use IEEE division, so each result is
+// the reference's value.  Two exact rewrites speed them up:
+//   * when dx and dy are powers of two, x/c for the grid constants c (dx^2,
+//     4 dx dy, 2 dx, -dx, ...) equals x*(1/c) exactly, since 1/c is exact;
+//   * the sensor only needs the sign of r - 0.9999f: a single-precision
+//     estimate decides it unless it lies within 1e-6 of the threshold, where
+//     the exact double quotient is computed (same decision either way).
+#include <cmath>
+
+#include "enl_device.h"
+#include "enl_kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace enl {
+namespace {
+
+constexpr int kBX = 32;
+constexpr int kBY = 8;
+constexpr int kHalo = 3;
+constexpr int kTW = kBX + 2 * kHalo;
+constexpr int kTH = kBY + 2 * kHalo;
+constexpr int kTN = kTW * kTH;
+constexpr int kThreads = kBX * kBY;
+
+// a / c for a grid constant c; a * (1/c) when 1/c is exact (POW2).
+template <bool POW2>
+__device__ inline double divc(double a, double c, double inv_c) {
+  if constexpr (POW2)
+    return a * inv_c;
+  else
+    return a / c;
+}
+
+// Pointers into the LDS tile; offsets are relative to a cell centre, row
+// stride kTW.  w[k] is only staged by the dense kernel.
+struct Tile {
+  const double* w0;
+  const double* w[5];
+  const double* p;
+  const double* T;
+  const double* u;
+  const double* v;
+  const double* s;
+  const double* tn;
+};
+
+// Inviscid fluxes F (x) and G (y), rhs_hybrid_high_llf.m4:125-156.
+__device__ inline double flux_f(double w0, double u, double v, double s, double tn, double p, int k) {
+  switch (k) {
+    case 0: return w0 * u;
+    case 1: return w0 * sq(u) + p;
+    case 2: return w0 * u * v;
+    case 3: return w0 * u * s;
+    default: return w0 * u * tn;
+  }
+}
+__device__ inline double flux_g(double w0, double u, double v, double s, double tn, double p, int k) {
+  switch (k) {
+    case 0: return w0 * v;
+    case 1: return w0 * v * u;
+    case 2: return w0 * sq(v) + p;
+    case 3: return w0 * v * s;
+    default: return w0 * v * tn;
+  }
+}
+__device__ inline double tflux_f(const Tile& t, int e, int k) {
+  return flux_f(t.w0[e], t.u[e], t.v[e], t.s[e], t.tn[e], t.p[e], k);
+}
+__device__ inline double tflux_g(const Tile& t, int e, int k) {
+  return flux_g(t.w0[e], t.u[e], t.v[e], t.s[e], t.tn[e], t.p[e], k);
+}
+
+// Physical source H, rhs_hybrid_high_llf.m4:159-237.
+template <bool P>
+__device__ inline void cell_source(const Tile& t, int e, double H0, const RhsArgs& a, double H[5]) {
+  constexpr int S = kTW;
+  const double dx = a.dx, dy = a.dy;
+  const double* U = t.u;
+  const double* V = t.v;
+  const double* TT = t.T;
+  const double dx2 = sq(dx), dy2 = sq(dy), dxy4 = 4 * dx * dy, dx_2 = 2 * dx, dy_2 = 2 * dy;
+  H[0] = H0;
+
+  double u_dprime_x = divc<P>(U[e - 1] - 2 * U[e] + U[e + 1], dx2, a.inv_dx2);
+  double u_dprime_y = divc<P>(U[e - S] - 2 * U[e] + U[e + S], dy2, a.inv_dy2);
+  double v_dprime_xy = divc<P>(V[e + 1 + S] - V[e + 1 - S] - V[e - 1 + S] + V[e - 1 - S], dxy4, a.inv_4dxdy);
+  double phi_xx_prime_x = kC4_3 * u_dprime_x - kC2_3 * v_dprime_xy;
+  double phi_xy_prime_y = u_dprime_y + v_dprime_xy;
+  H[1] = a.muB * (u_dprime_x + v_dprime_xy) + a.mu * (phi_xx_prime_x + phi_xy_prime_y);
+
+  double v_dprime_y = divc<P>(V[e - S] - 2 * V[e] + V[e + S], dy2, a.inv_dy2);
+  double v_dprime_x = divc<P>(V[e - 1] - 2 * V[e] + V[e + 1], dx2, a.inv_dx2);
+  double u_dprime_yx = divc<P>(U[e + 1 + S] - U[e + 1 - S] - U[e - 1 + S] + U[e - 1 - S], dxy4, a.inv_4dxdy);
+  double phi_yx_prime_x = u_dprime_yx + v_dprime_x;
+  double phi_yy_prime_y = kC4_3 * v_dprime_y - kC2_3 * u_dprime_yx;
+  H[2] = a.muB * (v_dprime_y + u_dprime_yx) + a.mu * (phi_yx_prime_x + phi_yy_prime_y);
+
+  double u_prime_x = divc<P>(U[e + 1] - U[e - 1], dx_2, a.inv_2dx);
+  double u_prime_y = divc<P>(U[e + S] - U[e - S], dy_2, a.inv_2dy);
+  double v_prime_y = divc<P>(V[e + S] - V[e - S], dy_2, a.inv_2dy);
+  double v_prime_x = divc<P>(V[e + 1] - V[e - 1], dx_2, a.inv_2dx);
+  double t_prime_x = divc<P>(TT[e + 1] - TT[e - 1], dx_2, a.inv_2dx);
+  double t_prime_y = divc<P>(TT[e + S] - TT[e - S], dy_2, a.inv_2dy);
+  double t_dprime_x = divc<P>(TT[e - 1] - 2 * TT[e] + TT[e + 1], dx2, a.inv_dx2);
+  double t_dprime_y = divc<P>(TT[e - S] - 2 * TT[e] + TT[e + S], dy2, a.inv_dy2);
+  double w0 = t.w0[e];
+  double temp = TT[e];
+  double T_n = t.tn[e];
+  double Mn = (temp - T_n) / a.tau_n;
+  double tmp1 = a.T_star_n / (T_n);
+  double c_vn = kC078 * a.R * sq(tmp1) * exp(-tmp1);
+  double A_n = ((temp / T_n) - 1) * c_vn;
+  double phi_xx = kC4_3 * u_prime_x - kC2_3 * v_prime_y;
+  double phi_xy_yx = u_prime_y + v_prime_x;
+  double phi_yy = kC4_3 * v_prime_y - kC2_3 * u_prime_x;
+  double sigma = (a.muB / temp) * sq(u_prime_x + v_prime_y) +
+                 (a.mu / (2 * temp)) * (sq(phi_xx) + 2 * sq(phi_xy_yx) + sq(phi_yy)) +
+                 (a.kappa / sq(temp)) * (sq(t_prime_x) + sq(t_prime_y)) + (w0 / temp) * (A_n * Mn);
+  H[3] = sigma - (w0 / T_n) * c_vn * Mn +
+         (-((a.kappa * sq(t_prime_x)) / sq(temp)) + ((a.kappa * t_dprime_x) / temp) -
+          ((a.kappa * sq(t_prime_y)) / sq(temp)) + ((a.kappa * t_dprime_y) / temp));
+  H[4] = (w0 / a.tau_n) * (temp - T_n);
+}
+
+// r = (2|t1 t2| + ep) / (t1^2 + t2^2 + ep) < float(0.9999)?  The numerator
+// and denominator are formed exactly as in rhs_hybrid_high_llf.m4:246; the
+// comparison of their IEEE quotient with the threshold is decided from a
+// single-precision estimate (relative error < 3e-7) whenever that estimate
+// is more than 1e-6 away from the threshold, else from the exact quotient.
+__device__ inline bool ratio_below(double tmp1, double tmp2, double ep) {
+  double num = 2 * fabs(tmp1 * tmp2) + ep;
+  double den = sq(tmp1) + sq(tmp2) + ep;
+  if (num > 1e-30 && num < 1e30 && den > 1e-30 && den < 1e30) {
+    float q = (float)num * __builtin_amdgcn_rcpf((float)den);
+    if (q < (float)(kThresh * (1 - 1e-6))) return true;
+    if (q > (float)(kThresh * (1 + 1e-6))) return false;
+  }
+  return num / den < kThresh;
+}
+
+// Hybrid density sensor, rhs_hybrid_high_llf.m4:239-256.  1 = WENO.
+__device__ inline int cell_sensor(const double* r, int e, double ep, int level) {
+  constexpr int S = kTW;
+  int tag = 1;
+  if (level == 0) {
+    tag = 0;
+#pragma unroll
+    for (int q = -1; q <= 1; q++) {
+      if (ratio_below(r[e + q] - r[e - 1 + q], r[e + 1 + q] - r[e + q], ep)) tag = 1;
+      if (ratio_below(r[e + q * S] - r[e + (q - 1) * S], r[e + (q + 1) * S] - r[e + q * S], ep)) tag = 1;
+    }
+  }
+  return tag;
+}
+
+// DRP branch, rhs_hybrid_high_llf.m4:258-268.
+template <bool P>
+__device__ inline void cell_drp(const Tile& t, int e, const RhsArgs& a, const double H[5], double out[5]) {
+  constexpr int S = kTW;
+  const double mdx = -a.dx, mdy = -a.dy;
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    double fx = -kA3 * tflux_f(t, e - 3, k) - kA2 * tflux_f(t, e - 2, k) - kA1 * tflux_f(t, e - 1, k) +
+                kA1 * tflux_f(t, e + 1, k) + kA2 * tflux_f(t, e + 2, k) + kA3 * tflux_f(t, e + 3, k);
+    double gy = -kA3 * tflux_g(t, e - 3 * S, k) - kA2 * tflux_g(t, e - 2 * S, k) - kA1 * tflux_g(t, e - S, k) +
+                kA1 * tflux_g(t, e + S, k) + kA2 * tflux_g(t, e + 2 * S, k) + kA3 * tflux_g(t, e + 3 * S, k);
+    out[k] = divc<P>(fx, mdx, a.inv_mdx) + divc<P>(gy, mdy, a.inv_mdy) + H[k];
+  }
+}
+
+// One direction of the WENO branch for component k: split fluxes at the
+// 2R+1 points and the four reconstructions pf1, mf1, pf2, mf2
+// (rhs_hybrid_high_llf.m4:286-354); returns pf1 + mf1 - pf2 - mf2.
+template <int ORDER>
+__device__ inline double weno_dir(const double* F, const double* W, double alpha) {
+  double fp[7], fm[7];
+  constexpr int R = ORDER == 5 ? 3 : 2;
+#pragma unroll
+  for (int m = -R; m <= R; m++) {
+    fp[m + 3] = 0.5 * (F[m + 3] + alpha * W[m + 3]);
+    fm[m + 3] = 0.5 * (F[m + 3] - alpha * W[m + 3]);
+  }
+  double p1, m1, p2, m2;
+  if constexpr (ORDER == 5) {
+    p1 = wenop5(fp[0], fp[1], fp[2], fp[3], fp[4]);
+    m1 = wenom5(fm[1], fm[2], fm[3], fm[4], fm[5]);
+    p2 = wenop5(fp[1], fp[2], fp[3], fp[4], fp[5]);
+    m2 = wenom5(fm[2], fm[3], fm[4], fm[5], fm[6]);
+  } else {
+    p1 = wenop3(fp[1], fp[2], fp[3]);
+    m1 = wenom3(fm[2], fm[3], fm[4]);
+    p2 = wenop3(fp[2], fp[3], fp[4]);
+    m2 = wenom3(fm[3], fm[4], fm[5]);
+  }
+  return p1 + m1 - p2 - m2;
+}
+
+// Per-cell LLF wave speeds, rhs_hybrid_high_llf.m4:270-284: alpha_f over
+// |u| in the x window at this row's height, alpha_g over |v| in the y
+// window at each point's height.  `u` and `v` hold the window (index m+3).
+template <int REACH>
+__device__ inline void llf_alpha(const double* u, const double* v, int j, const RhsArgs& a, double& af,
+                                 double& ag) {
+  af = 0;
+  ag = 0;
+#pragma unroll
+  for (int q = -REACH; q <= REACH; q++) {
+    double height = a.ylo + a.dy * ((double)(j) + 0.5);
+    double vf = fabs(u[q + 3]) + a.c + a.c_lr * height;
+    if (vf > af) af = vf;
+    height = a.ylo + a.dy * ((double)(j + q) + 0.5);
+    double vg = fabs(v[q + 3]) + a.c + a.c_lr * height;
+    if (vg > ag) ag = vg;
+  }
+}
+
+// WENO branch of one cell out of an LDS tile that holds the raw w_k.
+template <int V, bool P>
+__device__ inline void cell_weno_tile(const Tile& t, int e, int j, const RhsArgs& a, const double H[5],
+                                      double out[5]) {
+  using Sc = Scheme<V>;
+  constexpr int S = kTW;
+  constexpr int R = Sc::order == 5 ? 3 : 2;
+  double af, ag;
+  if constexpr (Sc::llf) {
+    double uu[7], vv[7];
+#pragma unroll
+    for (int q = -R; q <= R; q++) {
+      uu[q + 3] = t.u[e + q];
+      vv[q + 3] = t.v[e + q * S];
+    }
+    llf_alpha<R>(uu, vv, j, a, af, ag);
+  } else {
+    af = a.alpha;
+    ag = a.alpha;
+  }
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    double F[7], Wx[7], G[7], Wy[7];
+#pragma unroll
+    for (int m = -R; m <= R; m++) {
+      F[m + 3] = tflux_f(t, e + m, k);
+      Wx[m + 3] = t.w[k][e + m];
+      G[m + 3] = tflux_g(t, e + m * S, k);
+      Wy[m + 3] = t.w[k][e + m * S];
+    }
+    double x = weno_dir<Sc::order>(F, Wx, af);
+    double y = weno_dir<Sc::order>(G, Wy, ag);
+    out[k] = divc<P>(x, a.dx, a.inv_dx) + divc<P>(y, a.dy, a.inv_dy) + H[k];
+  }
+}
+
+// Stage rho, p, T and the quotients (and the raw w_k if RAW) of the tile +
+// halo.  Cells outside the ghosted array are skipped; no interior cell of
+// the patch reads them.
+template <bool RAW>
+__device__ inline void stage_tile(const RhsArgs& a, int i0, int j0, double* s0, double* sp, double* sT,
+                                  double* su, double* sv, double* ss, double* stn, double (*sw)[kTN]) {
+  const Geom& G = a.geom;
+  const int tid = threadIdx.x + threadIdx.y * kBX;
+  for (int e = tid; e < kTN; e += kThreads) {
+    int gi = i0 + (e % kTW) - kHalo, gj = j0 + (e / kTW) - kHalo;
+    if (gi >= -G.g0 && gi < G.nx + G.g0 && gj >= -G.g1 && gj < G.ny + G.g1) {
+      long gidx = (long)(gi + G.g0) + (long)(gj + G.g1) * G.sy;
+      double w0 = a.w[gidx], w1 = a.w[gidx + G.sk], w2 = a.w[gidx + 2 * G.sk];
+      double w3 = a.w[gidx + 3 * G.sk], w4 = a.w[gidx + 4 * G.sk];
+      s0[e] = w0;
+      sp[e] = a.p[gidx];
+      sT[e] = a.T[gidx];
+      su[e] = w1 / w0;
+      sv[e] = w2 / w0;
+      ss[e] = w3 / w0;
+      stn[e] = w4 / w0;
+      if constexpr (RAW) {
+        sw[1][e] = w1;
+        sw[2][e] = w2;
+        sw[3][e] = w3;
+        sw[4][e] = w4;
+      }
+    }
+  }
+}
+
+template <int V, bool P>
+__global__ void __launch_bounds__(kThreads) rhs_split_kernel(RhsArgs a) {
+  __shared__ double s0[kTN], sp[kTN], sT[kTN], su[kTN], sv[kTN], ss[kTN], stn[kTN];
+  using Sc = Scheme<V>;
+  const Geom& G = a.geom;
+  const int i0 = blockIdx.x * kBX, j0 = blockIdx.y * kBY;
+  stage_tile<false>(a, i0, j0, s0, sp, sT, su, sv, ss, stn, nullptr);
+  __syncthreads();
+
+  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+  const bool valid = i < G.nx && j < G.ny;
+  int tag = 0;
+  if (valid) {
+    Tile t;
+    t.w0 = s0;
+    t.p = sp;
+    t.T = sT;
+    t.u = su;
+    t.v = sv;
+    t.s = ss;
+    t.tn = stn;
+    const int e = (threadIdx.y + kHalo) * kTW + threadIdx.x + kHalo;
+    const long icell = (long)i + (long)j * G.nx;
+    const long ik = (long)G.nx * G.ny;
+    double H[5];
+    cell_source<P>(t, e, a.H0[(long)i + (long)j * a.h0_sy], a, H);
+    if constexpr (Sc::hybrid) tag = cell_sensor(s0, e, a.ep, 0);
+    double out[5];
+    if (tag == 0) {
+      cell_drp<P>(t, e, a, H, out);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 5; k++) out[k] = H[k];  // finished by rhs_weno_list_kernel
+    }
+#pragma unroll
+    for (int k = 0; k < 5; k++) a.KK[icell + k * ik] = out[k];
+    if (a.switch_map) a.switch_map[icell] = tag;
+  }
+  if constexpr (Sc::hybrid) {
+    // append this wave's WENO cells to the list: one atomic per wave
+    unsigned long long mask = __ballot(tag);
+    if (mask) {
+      int lane = __lane_id();
+      int leader = __ffsll((unsigned long long)mask) - 1;
+      unsigned base = 0;
+      if (lane == leader) base = atomicAdd(a.count, (unsigned)__popcll(mask));
+      base = __shfl(base, leader);
+      if (tag) a.list[base + __popcll(mask & __lanemask_lt())] = i + j * G.nx;
+    }
+  }
+}
+
+// WENO cells of the list, one lane each, stencil from global memory.
+template <int V, bool P>
+__global__ void __launch_bounds__(256) rhs_weno_list_kernel(RhsArgs a) {
+  using Sc = Scheme<V>;
+  constexpr int R = Sc::order == 5 ? 3 : 2;
+  const Geom& G = a.geom;
+  const unsigned n = *a.count;
+  const long ik = (long)G.nx * G.ny;
+  for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const int cell = a.list[q];
+    const int i = cell % G.nx, j = cell / G.nx;
+    const long g = (long)(i + G.g0) + (long)(j + G.g1) * G.sy;
+    double X[5], Y[5];
+#pragma unroll
+    for (int dir = 0; dir < 2; dir++) {
+      const long step = dir == 0 ? 1 : G.sy;
+      double w0[7], u[7], v[7], s[7], tn[7], p[7];
+#pragma unroll
+      for (int m = -R; m <= R; m++) {
+        long gm = g + m * step;
+        double r = a.w[gm];
+        w0[m + 3] = r;
+        u[m + 3] = a.w[gm + G.sk] / r;
+        v[m + 3] = a.w[gm + 2 * G.sk] / r;
+        s[m + 3] = a.w[gm + 3 * G.sk] / r;
+        tn[m + 3] = a.w[gm + 4 * G.sk] / r;
+        p[m + 3] = a.p[gm];
+      }
+      double alpha = a.alpha;
+      if constexpr (Sc::llf) {
+        // the x window gives alpha_f (height of row j), the y window alpha_g
+        alpha = 0;
+#pragma unroll
+        for (int m = -R; m <= R; m++) {
+          double height = a.ylo + a.dy * ((double)(dir == 0 ? j : j + m) + 0.5);
+          double vv = fabs(dir == 0 ? u[m + 3] : v[m + 3]) + a.c + a.c_lr * height;
+          if (vv > alpha) alpha = vv;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
+        double F[7], W[7];
+#pragma unroll
+        for (int m = -R; m <= R; m++) {
+          F[m + 3] = dir == 0 ? flux_f(w0[m + 3], u[m + 3], v[m + 3], s[m + 3], tn[m + 3], p[m + 3], k)
+                              : flux_g(w0[m + 3], u[m + 3], v[m + 3], s[m + 3], tn[m + 3], p[m + 3], k);
+          W[m + 3] = a.w[g + m * step + k * G.sk];
+        }
+        double d = weno_dir<Sc::order>(F, W, alpha);
+        if (dir == 0)
+          X[k] = divc<P>(d, a.dx, a.inv_dx);
+        else
+          Y[k] = divc<P>(d, a.dy, a.inv_dy);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      long o = (long)cell + k * ik;
+      a.KK[o] = X[k] + Y[k] + a.KK[o];  // the slot holds H(k)
+    }
+  }
+}
+
+// Every cell on the WENO path.
+template <int V, bool P>
+__global__ void __launch_bounds__(kThreads) rhs_dense_kernel(RhsArgs a) {
+  __shared__ double sw[5][kTN];
+  __shared__ double sp[kTN], sT[kTN], su[kTN], sv[kTN], ss[kTN], stn[kTN];
+  const Geom& G = a.geom;
+  const int i0 = blockIdx.x * kBX, j0 = blockIdx.y * kBY;
+  stage_tile<true>(a, i0, j0, sw[0], sp, sT, su, sv, ss, stn, sw);
+  __syncthreads();
+  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+  if (i >= G.nx || j >= G.ny) return;
+  Tile t;
+  t.w0 = sw[0];
+  for (int k = 0; k < 5; k++) t.w[k] = sw[k];
+  t.p = sp;
+  t.T = sT;
+  t.u = su;
+  t.v = sv;
+  t.s = ss;
+  t.tn = stn;
+  const int e = (threadIdx.y + kHalo) * kTW + threadIdx.x + kHalo;
+  const long icell = (long)i + (long)j * G.nx;
+  const long ik = (long)G.nx * G.ny;
+  double H[5], out[5];
+  cell_source<P>(t, e, a.H0[(long)i + (long)j * a.h0_sy], a, H);
+  cell_weno_tile<V, P>(t, e, j, a, H, out);
+#pragma unroll
+  for (int k = 0; k < 5; k++) a.KK[icell + k * ik] = out[k];
+  if (a.switch_map) a.switch_map[icell] = 1;
+}
+
+__global__ void reset_count_kernel(unsigned* count) { *count = 0; }
+
+bool is_pow2(double x) {
+  if (!(x > 0) || !std::isfinite(x)) return false;
+  int e;
+  return std::frexp(x, &e) == 0.5;
+}
+
+template <int V, bool P>
+hipError_t launch_t(const RhsArgs& a, hipStream_t s) {
+  using Sc = Scheme<V>;
+  const Geom& G = a.geom;
+  dim3 block(kBX, kBY);
+  dim3 grid((G.nx + kBX - 1) / kBX, (G.ny + kBY - 1) / kBY);
+  if constexpr (!Sc::drp) {
+    // rhs_weno_*: every cell on the WENO path
+    hipLaunchKernelGGL((rhs_dense_kernel<V, P>), grid, block, 0, s, a);
+  } else if constexpr (!Sc::hybrid) {
+    // rhs_drp: every cell on the DRP path
+    hipLaunchKernelGGL((rhs_split_kernel<V, P>), grid, block, 0, s, a);
+  } else if (a.level != 0) {
+    // hybrid above level 0: the sensor is off, every cell WENO
+    hipLaunchKernelGGL((rhs_dense_kernel<V, P>), grid, block, 0, s, a);
+  } else {
+    if (!a.list || !a.count) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(reset_count_kernel, dim3(1), dim3(1), 0, s, a.count);
+    hipLaunchKernelGGL((rhs_split_kernel<V, P>), grid, block, 0, s, a);
+    // a grid that covers the machine; lanes stride over the list
+    long cells = (long)G.nx * G.ny;
+    long blocks = (cells + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL((rhs_weno_list_kernel<V, P>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+template <int V>
+hipError_t launch_v(const RhsArgs& a, hipStream_t s) {
+  if (is_pow2(a.dx) && is_pow2(a.dy)) return launch_t<V, true>(a, s);
+  return launch_t<V, false>(a, s);
+}
+
+}  // namespace
+
+void fill_rhs_constants(RhsArgs& a) {
+  a.inv_dx2 = 1.0 / (a.dx * a.dx);
+  a.inv_dy2 = 1.0 / (a.dy * a.dy);
+  a.inv_4dxdy = 1.0 / (4 * a.dx * a.dy);
+  a.inv_2dx = 1.0 / (2 * a.dx);
+  a.inv_2dy = 1.0 / (2 * a.dy);
+  a.inv_mdx = 1.0 / (-a.dx);
+  a.inv_mdy = 1.0 / (-a.dy);
+  a.inv_dx = 1.0 / a.dx;
+  a.inv_dy = 1.0 / a.dy;
+}
+
+bool rhs_needs_list(int variant, int level) {
+  return variant <= kHybridLow && level == 0;
+}
+
+hipError_t launch_rhs(int variant, const RhsArgs& a0, hipStream_t s) {
+  if (a0.geom.nx <= 0 || a0.geom.ny <= 0) return hipSuccess;
+  RhsArgs a = a0;
+  fill_rhs_constants(a);
+  switch (variant) {
+    case kHybridHighLlf: return launch_v<kHybridHighLlf>(a, s);
+    case kHybridHigh: return launch_v<kHybridHigh>(a, s);
+    case kHybridLowLlf: return launch_v<kHybridLowLlf>(a, s);
+    case kHybridLow: return launch_v<kHybridLow>(a, s);
+    case kWenoHighLlf: return launch_v<kWenoHighLlf>(a, s);
+    case kWenoHigh: return launch_v<kWenoHigh>(a, s);
+    case kWenoLowLlf: return launch_v<kWenoLowLlf>(a, s);
+    case kWenoLow: return launch_v<kWenoLow>(a, s);
+    case kDrp: return launch_v<kDrp>(a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace enl
