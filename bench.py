@@ -8,12 +8,18 @@ Workload (one step = one SSP-RK3 step of the whole patch):
   CFL 0.5, FP64: the plain drop-in (rhs -> runge_kutta -> update_state per
   stage, SAMRAI-layout arrays resident in HBM).
 
-Prints one JSON line.  value = interior cell-updates per second over the
-timed steps (inputs resident in HBM).  roofline: the rhs kernel's algorithmic
-HBM bytes (104 B per interior cell: w_n, p, T, source read once, rhs written
-once) over its average launch duration, timed with HIP events on its launch
-stream during the timed region.  cpu_baseline: the CPU restatement of the
-same kernels (oracle/) on this host's cores, over a bounded sample.
+Both forms run from the same initial state for the same steps: the plain
+drop-in (per stage rhs_, runge_kutta_, update_state_ plus SAMRAI's copy and
+boundary fill) and the tuned fused form (one kernel per stage, bit-identical
+results, checked at the end of the run).
+
+Prints one JSON line.  value = interior cell-updates per second of the tuned
+form over the timed steps (inputs resident in HBM); plain_fp64_dropin holds
+the plain form's figures.  roofline: the dominant kernel's algorithmic HBM
+bytes over its average launch duration, timed with HIP events on its launch
+stream during the timed region (fused stage kernel: 146.7 B per cell on
+average; plain rhs: 104 B per cell).  cpu_baseline: the CPU restatement of
+the same kernels (oracle/) on this host's cores, over a bounded sample.
 """
 import argparse
 import ctypes
@@ -29,6 +35,7 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 RHS_BYTES_PER_CELL = 104.0  # 13 doubles: w_n(5), p, T, source read; rhs(5) written
 STEP_BYTES_PER_CELL = 888.0  # 3 stages x (rhs 104 + runge_kutta 160 + update_state 32)
+FUSED_STAGE_BYTES_PER_CELL = (120.0 + 160.0 + 160.0) / 3
 
 
 def parse():
@@ -42,10 +49,13 @@ def parse():
     return ap.parse_args()
 
 
-def read_traffic():
-    """HBM bytes per rhs launch from the committed rocprofv3 counter summary
-    (the line `traffic_bytes_per_launch: <bytes>`), or None."""
-    path = os.path.join(ROOT, "profiles", "rhs_hybrid_high_llf_4096_traffic.txt")
+TRAFFIC_FILES = {"rhs": "rhs_hybrid_high_llf_4096_traffic.txt", "stage_fused": "stage_fused_4096_traffic.txt"}
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (the line `traffic_bytes_per_launch: <bytes>`), or None."""
+    path = os.path.join(ROOT, "profiles", TRAFFIC_FILES[kernel])
     try:
         with open(path) as f:
             for line in f:
@@ -101,74 +111,100 @@ def main():
     wl = PULSES_4096 if args.n == 4096 else Workload("pulses_%d_8" % args.n, n=args.n, seed=1, npulses=8)
     dt = wl.dt(gas)
     w, p, T, s = initial_fields(wl, gas)
-    d = P.DevicePatch(wl, gas)
-
-    # warmup on the initial state, then restart from it for the timed steps
-    d.init_state(w, p, T, s)
-    if args.warmup:
-        d.advance(args.warmup, dt)
-    d.init_state(w, p, T, s)
-    del w, p, T, s
-    torch.cuda.synchronize()
-
+    plain = P.DevicePatch(wl, gas)
+    fused = P.DevicePatch(wl, gas)
     lib = _lib.load()
     nk = len(_lib.KERNEL_KINDS)
-    rc = lib.enl_profile_start(args.steps * 16)
-    _lib.check(rc)
+
+    def timed(patch, run):
+        """Restart from the initial state, then time `steps` steps; per-kernel
+        device times come from HIP events on the launch stream."""
+        patch.init_state(w, p, T, s)
+        torch.cuda.synchronize()
+        _lib.check(lib.enl_profile_start(args.steps * 16))
+        t0 = time.perf_counter()
+        run(args.steps, dt)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        ms = (ctypes.c_double * nk)()
+        cnt = (ctypes.c_int * nk)()
+        _lib.check(lib.enl_profile_stop(ms, cnt, nk))
+        per = {k: (ms[i] / cnt[i] if cnt[i] else None) for i, k in enumerate(_lib.KERNEL_KINDS)}
+        return elapsed, per
+
+    # warmup of both paths on the initial state
+    for patch, run in ((plain, plain.advance), (fused, fused.advance_fused)):
+        patch.init_state(w, p, T, s)
+        if args.warmup:
+            run(args.warmup, dt)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    d.advance(args.steps, dt)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms = (ctypes.c_double * nk)()
-    cnt = (ctypes.c_int * nk)()
-    rc = lib.enl_profile_stop(ms, cnt, nk)
-    _lib.check(rc)
-    kernel_ms = {k: (ms[i] / cnt[i] if cnt[i] else None) for i, k in enumerate(_lib.KERNEL_KINDS)}
+
+    t_plain, k_plain = timed(plain, plain.advance)
+    t_fused, k_fused = timed(fused, fused.advance_fused)
+    del w, p, T, s
+
+    n, g = wl.n, wl.g
+    same = all(torch.equal(a[..., g:-g, g:-g], b[..., g:-g, g:-g])
+               for a, b in ((plain.w, fused.w), (plain.p, fused.p), (plain.T, fused.T)))
 
     # fraction of WENO cells in the final state (level-0 sensor)
-    n, g = wl.n, wl.g
     sm = torch.empty((n, n), dtype=torch.int32, device="cuda")
     KK = torch.empty((5, n, n), dtype=torch.float64, device="cuda")
-    P.fill_physical_bc(d.box, g, (1, 1, 1, 1), d.reflect, d.w, d.p, d.T, d.src)
-    P.rhs("hybrid_high_llf", d.box, g, d.xlo, d.dx, d.w, d.src[g:-g, g:-g].contiguous(), KK, d.p, d.T, gas, 0, sm)
+    P.fill_physical_bc(plain.box, g, (1, 1, 1, 1), plain.reflect, plain.w, plain.p, plain.T, plain.src)
+    P.rhs("hybrid_high_llf", plain.box, g, plain.xlo, plain.dx, plain.w, plain.src[g:-g, g:-g].contiguous(), KK,
+          plain.p, plain.T, gas, 0, sm)
     weno_frac = float(sm.double().mean().item())
-    finite = bool(torch.isfinite(d.w).all().item())
+    finite = bool(torch.isfinite(plain.w).all().item())
     del KK, sm
 
     cells = wl.n * wl.n
-    value = cells * args.steps / elapsed
-    rhs_ms = kernel_ms["rhs"]
+    # plain drop-in: rhs is the dominant kernel
     rhs_bytes = RHS_BYTES_PER_CELL * cells
-    achieved = rhs_bytes / (rhs_ms * 1e-3) / 1e9
+    rhs_achieved = rhs_bytes / (k_plain["rhs"] * 1e-3) / 1e9
+    # fused: one stage kernel per stage; per cell it reads w_n(5), p, T,
+    # source (+ ww(5) in stages 2 and 3, which is w_n itself in stage 1) and
+    # writes w_n(5), p, T: 120 / 160 / 160 B, mean 146.7 B per launch-cell
+    stage_bytes = FUSED_STAGE_BYTES_PER_CELL * cells
+    stage_achieved = stage_bytes / (k_fused["stage_fused"] * 1e-3) / 1e9
+    value = cells * args.steps / t_fused
     out = {
-        "metric": "RK3 cell-updates/s on 1 MI355X, plain FP64 drop-in (hybrid WENO5/DRP LLF, 4096^2)",
+        "metric": "RK3 cell-updates/s on 1 MI355X, tuned (fused FP64) vs plain FP64 drop-in; %HBM peak",
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed * 1e3 / args.steps,
+        "ms_per_step": t_fused * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": wl.name, "grid": [wl.n, wl.n], "ghosts": wl.g, "rhs": "hybrid_high_llf",
-                   "time_integrator": "ssp-rk3", "cfl": wl.cfl, "form": "plain_fp64_dropin",
+                   "time_integrator": "ssp-rk3", "cfl": wl.cfl, "form": "fused_fp64_stage",
                    "walls": "reflecting"},
-        "roofline": {"bound": "hbm", "kernel": "rhs_hybrid_high_llf", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": read_traffic(),
-                     "algorithmic_bytes_per_launch": rhs_bytes},
-        "step_hbm_frac": STEP_BYTES_PER_CELL * cells / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS,
-        "kernel_ms_avg": kernel_ms,
+        "roofline": {"bound": "hbm", "kernel": "fused stage (rhs+runge_kutta+update_state, hybrid_high_llf)",
+                     "achieved": stage_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": stage_achieved / HBM_PEAK_GBS, "traffic": read_traffic("stage_fused"),
+                     "algorithmic_bytes_per_launch": stage_bytes},
+        "speedup_vs_plain": t_plain / t_fused,
+        "fused_bit_identical_to_plain": same,
+        "kernel_ms_avg": k_fused,
+        "plain_fp64_dropin": {
+            "value": cells * args.steps / t_plain,
+            "ms_per_step": t_plain * 1e3 / args.steps,
+            "step_hbm_frac": STEP_BYTES_PER_CELL * cells / (t_plain / args.steps) / 1e9 / HBM_PEAK_GBS,
+            "roofline": {"bound": "hbm", "kernel": "rhs_hybrid_high_llf", "achieved": rhs_achieved,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": rhs_achieved / HBM_PEAK_GBS,
+                         "traffic": read_traffic("rhs"), "algorithmic_bytes_per_launch": rhs_bytes},
+            "kernel_ms_avg": k_plain,
+        },
         "weno_fraction_final": weno_frac,
         "state_finite": finite,
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        del d
+        del plain, fused
         torch.cuda.empty_cache()
         out["cpu_baseline"] = cpu_baseline(wl, gas, args.cpu_steps)
     print(json.dumps(out), flush=True)

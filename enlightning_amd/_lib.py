@@ -14,7 +14,7 @@ VARIANTS = [
     "weno_high_llf", "weno_high", "weno_low_llf", "weno_low", "drp",
 ]
 VARIANT_ID = {n: i for i, n in enumerate(VARIANTS)}
-KERNEL_KINDS = ["rhs", "runge_kutta", "update_state", "physical_bc", "copy_interior"]
+KERNEL_KINDS = ["rhs", "runge_kutta", "update_state", "physical_bc", "copy_interior", "stage_fused"]
 
 # Every exported function and its prototype (argtypes, restype).
 _P = ctypes.c_void_p
@@ -53,6 +53,9 @@ PROTOTYPES = {
     "enl_fill_physical_bc": ([_I] * 6 + [_P] * 6, _I),
     "enl_profile_start": ([_I], _I),
     "enl_profile_stop": ([_P, _P, _I], _I),
+    "enl_fused_scratch_bytes": ([_I] * 5, ctypes.c_size_t),
+    "enl_advance_patch_fused": ([_I] * 6 + [_P, _P, ctypes.POINTER(EnlGas), _P, _I, _P, _P, _P, _D, _I, _I]
+                                + [_P] * 5 + [ctypes.c_size_t], _I),
     "enl_advance_patch": ([_I] * 6 + [_P, _P, ctypes.POINTER(EnlGas), _P, _I, _P, _P, _P, _D, _I, _I]
                           + [_P] * 6, _I),
 }

@@ -129,4 +129,12 @@ __host__ __device__ inline double sensor_eps(double tol) {
   return kC09 * tol * sq(xi) / (1 - kC09 * sq(tol));
 }
 
+// Pressure and temperature of one cell, update_state.m4:88-105 (xc1 is the
+// cell-centre height xylo(1)+dxy(1)*(j-ifirst1+0.5)).
+__device__ inline void eos_p_T(double c, double c_lr, double rho0, double gamma1, double c_p, double p0, double T0,
+                               double c_v, double R, double xc1, double w0, double w3, double& p, double& T) {
+  p = sq(c + c_lr * xc1) * ((w0 - rho0) + ((gamma1 - 1) / (2 * rho0)) * sq(w0 - rho0) + (w0 / c_p) * (w3 / w0)) + p0;
+  T = T0 * exp((w3 / w0 - R * log(rho0 / w0)) / c_v);
+}
+
 }  // namespace enl

@@ -201,3 +201,121 @@ extern "C" int enl_advance_patch(int variant, int ifirst0, int ilast0, int ifirs
   if (e != hipSuccess) rc = check_hip(e, "patch loop launch");
   return rc;
 }
+
+extern "C" size_t enl_fused_scratch_bytes(int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw) {
+  Geom G = make_geom(ifirst0, ilast0, ifirst1, ilast1, gcw, gcw);
+  return sizeof(double) * (size_t)(2 * 7 * G.sk + 5L * G.nx * G.ny);
+}
+
+extern "C" int enl_advance_patch_fused(int variant, int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw,
+                                       const double* xlo, const double* dx, const enl_gas_t* gas, const int* reflect,
+                                       int order, const double* alpha_1, const double* alpha_2, const double* beta,
+                                       double dt, int nsteps, int level, double* w_n, double* pres, double* temp,
+                                       double* source, void* scratch, size_t scratch_bytes) {
+  if (variant < 0 || variant >= kNumVariants) return set_error(-1, "unknown rhs variant");
+  int need = (variant == kWenoLowLlf || variant == kWenoLow) ? 2 : 3;
+  if (gcw < need) return set_error(-2, "ghost width below the stencil reach");
+  if (ilast0 < ifirst0 || ilast1 < ifirst1) return set_error(-2, "empty or invalid patch box");
+  if (order < 1 || order > 8) return set_error(-2, "RK order out of range");
+  if (scratch_bytes < enl_fused_scratch_bytes(ifirst0, ilast0, ifirst1, ilast1, gcw))
+    return set_error(-2, "fused scratch too small");
+  const void* arrays[5] = {w_n, pres, temp, source, scratch};
+  for (int i = 0; i < 5; i++)
+    if (!is_device_ptr(arrays[i])) return set_error(-4, "enl_advance_patch_fused needs device arrays");
+  hipStream_t s = lib_stream();
+  Geom G = make_geom(ifirst0, ilast0, ifirst1, ilast1, gcw, gcw);
+
+  // states: A = caller's arrays, B and C in the scratch, then the rhs slots
+  struct State {
+    double *w, *p, *T;
+  };
+  double* sc = (double*)scratch;
+  State A{w_n, pres, temp};
+  State B{sc, sc + 5 * G.sk, sc + 6 * G.sk};
+  State C{sc + 7 * G.sk, sc + 12 * G.sk, sc + 13 * G.sk};
+  double* KK = sc + 14 * G.sk;
+
+  RhsArgs ra = {};
+  ra.H0 = source + G.g0 + (long)G.g1 * G.sy;
+  ra.h0_sy = G.sy;
+  ra.KK = KK;
+  ra.switch_map = nullptr;
+  ra.geom = G;
+  ra.dx = dx[0];
+  ra.dy = dx[1];
+  ra.ylo = xlo[1];
+  ra.alpha = gas->weno_alpha;
+  ra.kappa = gas->kappa;
+  ra.mu = gas->mu;
+  ra.muB = gas->muB;
+  ra.R = gas->R;
+  ra.T_star_n = gas->T_star_n;
+  ra.tau_n = gas->tau_n;
+  ra.c = gas->c;
+  ra.c_lr = gas->c_lr;
+  ra.ep = sensor_eps(gas->tag_tolerance[2]);
+  ra.level = level;
+  ra.list = nullptr;
+  ra.count = nullptr;
+  if (rhs_needs_list(variant, level)) {
+    int rc = rhs_workspace((long)G.nx * G.ny, &ra.list, &ra.count);
+    if (rc) return rc;
+  }
+  ra.ww = A.w;  // the step's start state is A's interior
+  ra.ww_off = G.g0 + (long)G.g1 * G.sy;
+  ra.ww_sy = G.sy;
+  ra.ww_sk = G.sk;
+  ra.rho0 = gas->rho0;
+  ra.gamma1 = gas->gamma1;
+  ra.c_p = gas->c_p;
+  ra.p0 = gas->p0;
+  ra.T0 = gas->T0;
+  ra.c_v = gas->c_v;
+
+  BcArgs bc;
+  bc.f[2] = source;
+  bc.geom = G;
+  for (int k = 0; k < 4; k++) {
+    bc.touch[k] = 1;
+    bc.reflect[k] = reflect[k];
+  }
+
+  hipError_t e = hipSuccess;
+  long t;
+  for (int step = 0; step < nsteps && e == hipSuccess; step++) {
+    State in = A;
+    for (int stage = 0; stage < order && e == hipSuccess; stage++) {
+      // the last stage writes A (in is B or C then), the others alternate;
+      // a one-stage scheme goes through B and is copied back
+      State out = (stage == order - 1 && order > 1) ? A : ((stage % 2 == 0) ? B : C);
+      bc.w = in.w;
+      bc.f[0] = in.p;
+      bc.f[1] = in.T;
+      t = g_timer.begin(ENL_KERNEL_BC, s);
+      e = launch_physical_bc(bc, s);
+      g_timer.end(t, s);
+      bc.f[2] = nullptr;  // the source is constant: its ghosts are filled once
+      if (e != hipSuccess) break;
+      ra.w = in.w;
+      ra.p = in.p;
+      ra.T = in.T;
+      ra.w_out = out.w;
+      ra.p_out = out.p;
+      ra.T_out = out.T;
+      ra.alpha_1 = alpha_1[stage];
+      ra.alpha_2 = alpha_2[stage];
+      ra.bdt = beta[stage] * dt;
+      t = g_timer.begin(ENL_KERNEL_STAGE, s);
+      e = launch_stage(variant, ra, s);
+      g_timer.end(t, s);
+      in = out;
+    }
+    if (order == 1 && e == hipSuccess) {
+      // copy B back into A (w_n, pres, temp are separate arrays)
+      e = hipMemcpyAsync(A.w, B.w, sizeof(double) * 5 * G.sk, hipMemcpyDeviceToDevice, s);
+      if (e == hipSuccess) e = hipMemcpyAsync(A.p, B.p, sizeof(double) * G.sk, hipMemcpyDeviceToDevice, s);
+      if (e == hipSuccess) e = hipMemcpyAsync(A.T, B.T, sizeof(double) * G.sk, hipMemcpyDeviceToDevice, s);
+    }
+  }
+  return e == hipSuccess ? 0 : check_hip(e, "fused patch loop launch");
+}

@@ -43,8 +43,20 @@ struct RhsArgs {
   // needed when rhs_needs_list(variant, level)
   int* list;
   unsigned* count;
-  // reciprocals of the grid constants (filled by launch_rhs)
+  // reciprocals of the grid constants (filled by launch_rhs / launch_stage)
   double inv_dx2, inv_dy2, inv_4dxdy, inv_2dx, inv_2dy, inv_mdx, inv_mdy, inv_dx, inv_dy;
+  // launch_stage only: instead of storing rhs, each cell applies the
+  // runge_kutta update (runge_kutta.m4:60-63) and the state update
+  // (update_state.m4:87-107) and stores the new state into w_out, p_out,
+  // T_out (ghosted arrays, interior written; must not alias w, p, T).
+  const double* ww;  // state at the start of the step: element (i,j,k) at
+                     // ww[ww_off + i + j*ww_sy + k*ww_sk]
+  long ww_off, ww_sy, ww_sk;
+  double* w_out;
+  double* p_out;
+  double* T_out;
+  double alpha_1, alpha_2, bdt;  // bdt = beta*dt
+  double rho0, gamma1, c_p, p0, T0, c_v;
 };
 
 struct RkArgs {
@@ -82,6 +94,8 @@ struct BcArgs {
 };
 
 hipError_t launch_rhs(int variant, const RhsArgs& a, hipStream_t s);
+// One fused method-of-lines stage: rhs + runge_kutta + update_state.
+hipError_t launch_stage(int variant, const RhsArgs& a, hipStream_t s);
 bool rhs_needs_list(int variant, int level);
 hipError_t launch_runge_kutta(const RkArgs& a, hipStream_t s);
 hipError_t launch_update_state(const StateArgs& a, hipStream_t s);

@@ -6,6 +6,8 @@
 //   physical BC    enlightning.C:790-1013
 //
 // The rhs kernels live in enl_rhs.hip.
+#include <stdint.h>
+
 #include "enl_device.h"
 #include "enl_kernels.h"
 
@@ -28,6 +30,66 @@ __global__ void __launch_bounds__(256) rk_kernel(RkArgs a) {
   }
 }
 
+// 16-byte vector forms of the streaming kernels: each lane owns two
+// consecutive cells of a row (one double2 per array and component).  Valid
+// when nx and g0 are even and every base pointer is 16-byte aligned, so the
+// interior of each ghosted row starts on a 16-byte boundary.
+__global__ void __launch_bounds__(256) rk_kernel_v(RkArgs a) {
+  const Geom& G = a.geom;
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= G.nx / 2) return;
+  const double bdt = a.beta * a.dt;  // beta*dt*KK groups as (beta*dt)*KK
+  const long ik = (long)G.nx * G.ny;
+  for (int row = blockIdx.y; row < G.ny * 5; row += gridDim.y) {
+    const int j = row % G.ny, k = row / G.ny;
+    const long g = (long)(2 * ip + G.g0) + (long)(j + G.g1) * G.sy + k * G.sk;
+    const long c = (long)(2 * ip) + (long)j * G.nx + k * ik;
+    double2 w = *(const double2*)(a.w + g);
+    double2 v = *(const double2*)(a.ww + c);
+    double2 K = *(const double2*)(a.KK + c);
+    double2 o;
+    o.x = a.alpha_1 * v.x + a.alpha_2 * w.x + bdt * K.x;
+    o.y = a.alpha_1 * v.y + a.alpha_2 * w.y + bdt * K.y;
+    *(double2*)(a.w + g) = o;
+  }
+}
+
+__device__ inline void state_cell(const StateArgs& a, double xc1, double w0, double w3, double& p, double& T) {
+  eos_p_T(a.c, a.c_lr, a.rho0, a.gamma1, a.c_p, a.p0, a.T0, a.c_v, a.R, xc1, w0, w3, p, T);
+}
+
+__global__ void __launch_bounds__(256) state_kernel_v(StateArgs a) {
+  const Geom& G = a.geom;
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= G.nx / 2) return;
+  for (int j = blockIdx.y; j < G.ny; j += gridDim.y) {
+    const double xc1 = a.ylo + a.dy * ((double)(j) + 0.5);
+    const long g = (long)(2 * ip + G.g0) + (long)(j + G.g1) * G.sy;
+    double2 w0 = *(const double2*)(a.w + g);
+    double2 w3 = *(const double2*)(a.w + g + 3 * G.sk);
+    double2 p, T;
+    state_cell(a, xc1, w0.x, w3.x, p.x, T.x);
+    state_cell(a, xc1, w0.y, w3.y, p.y, T.y);
+    *(double2*)(a.p + g) = p;
+    *(double2*)(a.T + g) = T;
+  }
+}
+
+__global__ void __launch_bounds__(256) copy_interior_kernel_v(double* ww, const double* w, Geom G) {
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= G.nx / 2) return;
+  const long ik = (long)G.nx * G.ny;
+  for (int j = blockIdx.y; j < G.ny; j += gridDim.y) {
+    const long g = (long)(2 * ip + G.g0) + (long)(j + G.g1) * G.sy;
+    const long c = (long)(2 * ip) + (long)j * G.nx;
+#pragma unroll
+    for (int k = 0; k < 5; k++) *(double2*)(ww + c + k * ik) = *(const double2*)(w + g + k * G.sk);
+  }
+}
+
+bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+bool vec_ok(const Geom& G) { return (G.nx % 2) == 0 && (G.g0 % 2) == 0; }
+
 // update_state.m4:87-107.
 __global__ void __launch_bounds__(256) state_kernel(StateArgs a) {
   const Geom& G = a.geom;
@@ -36,11 +98,7 @@ __global__ void __launch_bounds__(256) state_kernel(StateArgs a) {
   for (int j = blockIdx.y; j < G.ny; j += gridDim.y) {
     double xc1 = a.ylo + a.dy * ((double)(j) + 0.5);
     long gidx = (long)(i + G.g0) + (long)(j + G.g1) * G.sy;
-    double w0 = a.w[gidx], w3 = a.w[gidx + 3 * G.sk];
-    a.p[gidx] = sq(a.c + a.c_lr * xc1) *
-                    ((w0 - a.rho0) + ((a.gamma1 - 1) / (2 * a.rho0)) * sq(w0 - a.rho0) + (w0 / a.c_p) * (w3 / w0)) +
-                a.p0;
-    a.T[gidx] = a.T0 * exp((w3 / w0 - a.R * log(a.rho0 / w0)) / a.c_v);
+    state_cell(a, xc1, a.w[gidx], a.w[gidx + 3 * G.sk], a.p[gidx], a.T[gidx]);
   }
 }
 
@@ -142,6 +200,11 @@ inline unsigned rows_grid(int rows) { return (unsigned)(rows < 16384 ? rows : 16
 
 hipError_t launch_runge_kutta(const RkArgs& a, hipStream_t s) {
   if (a.geom.nx <= 0 || a.geom.ny <= 0) return hipSuccess;
+  if (vec_ok(a.geom) && aligned16(a.w) && aligned16(a.ww) && aligned16(a.KK)) {
+    dim3 grid((a.geom.nx / 2 + 255) / 256, rows_grid(a.geom.ny * 5));
+    hipLaunchKernelGGL(rk_kernel_v, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+  }
   dim3 grid((a.geom.nx + 255) / 256, rows_grid(a.geom.ny * 5));
   hipLaunchKernelGGL(rk_kernel, grid, dim3(256), 0, s, a);
   return hipGetLastError();
@@ -149,6 +212,11 @@ hipError_t launch_runge_kutta(const RkArgs& a, hipStream_t s) {
 
 hipError_t launch_update_state(const StateArgs& a, hipStream_t s) {
   if (a.geom.nx <= 0 || a.geom.ny <= 0) return hipSuccess;
+  if (vec_ok(a.geom) && aligned16(a.w) && aligned16(a.p) && aligned16(a.T)) {
+    dim3 grid((a.geom.nx / 2 + 255) / 256, rows_grid(a.geom.ny));
+    hipLaunchKernelGGL(state_kernel_v, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+  }
   dim3 grid((a.geom.nx + 255) / 256, rows_grid(a.geom.ny));
   hipLaunchKernelGGL(state_kernel, grid, dim3(256), 0, s, a);
   return hipGetLastError();
@@ -178,6 +246,11 @@ hipError_t launch_physical_bc(const BcArgs& a, hipStream_t s) {
 
 hipError_t launch_copy_interior(double* ww, const double* w, const Geom& g, hipStream_t s) {
   if (g.nx <= 0 || g.ny <= 0) return hipSuccess;
+  if (vec_ok(g) && aligned16(ww) && aligned16(w)) {
+    dim3 grid((g.nx / 2 + 255) / 256, rows_grid(g.ny));
+    hipLaunchKernelGGL(copy_interior_kernel_v, grid, dim3(256), 0, s, ww, w, g);
+    return hipGetLastError();
+  }
   dim3 grid((g.nx + 255) / 256, rows_grid(g.ny * 5));
   hipLaunchKernelGGL(copy_interior_kernel, grid, dim3(256), 0, s, ww, w, g);
   return hipGetLastError();

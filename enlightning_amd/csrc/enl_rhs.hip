@@ -46,6 +46,18 @@ constexpr int kTH = kBY + 2 * kHalo;
 constexpr int kTN = kTW * kTH;
 constexpr int kThreads = kBX * kBY;
 
+void fill_rhs_constants(RhsArgs& a) {
+  a.inv_dx2 = 1.0 / (a.dx * a.dx);
+  a.inv_dy2 = 1.0 / (a.dy * a.dy);
+  a.inv_4dxdy = 1.0 / (4 * a.dx * a.dy);
+  a.inv_2dx = 1.0 / (2 * a.dx);
+  a.inv_2dy = 1.0 / (2 * a.dy);
+  a.inv_mdx = 1.0 / (-a.dx);
+  a.inv_mdy = 1.0 / (-a.dy);
+  a.inv_dx = 1.0 / a.dx;
+  a.inv_dy = 1.0 / a.dy;
+}
+
 // a / c for a grid constant c; a * (1/c) when 1/c is exact (POW2).
 template <bool POW2>
 __device__ inline double divc(double a, double c, double inv_c) {
@@ -92,6 +104,33 @@ __device__ inline double tflux_f(const Tile& t, int e, int k) {
 }
 __device__ inline double tflux_g(const Tile& t, int e, int k) {
   return flux_g(t.w0[e], t.u[e], t.v[e], t.s[e], t.tn[e], t.p[e], k);
+}
+
+// Result of one cell: store rhs (plain), or apply the runge_kutta update
+// and update_state and store the new state (fused stage).
+template <bool FUSED>
+__device__ inline void cell_output(const RhsArgs& a, int i, int j, const double K[5]) {
+  const Geom& G = a.geom;
+  const long c = (long)i + (long)j * G.nx;
+  const long ik = (long)G.nx * G.ny;
+  if constexpr (!FUSED) {
+#pragma unroll
+    for (int k = 0; k < 5; k++) a.KK[c + k * ik] = K[k];
+  } else {
+    const long g = (long)(i + G.g0) + (long)(j + G.g1) * G.sy;
+    double wn[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      wn[k] = a.alpha_1 * a.ww[a.ww_off + i + j * a.ww_sy + k * a.ww_sk] + a.alpha_2 * a.w[g + k * G.sk] +
+              a.bdt * K[k];
+      a.w_out[g + k * G.sk] = wn[k];
+    }
+    const double xc1 = a.ylo + a.dy * ((double)(j) + 0.5);
+    double p, T;
+    eos_p_T(a.c, a.c_lr, a.rho0, a.gamma1, a.c_p, a.p0, a.T0, a.c_v, a.R, xc1, wn[0], wn[3], p, T);
+    a.p_out[g] = p;
+    a.T_out[g] = T;
+  }
 }
 
 // Physical source H, rhs_hybrid_high_llf.m4:159-237.
@@ -305,8 +344,8 @@ __device__ inline void stage_tile(const RhsArgs& a, int i0, int j0, double* s0, 
   }
 }
 
-template <int V, bool P>
-__global__ void __launch_bounds__(kThreads) rhs_split_kernel(RhsArgs a) {
+template <int V, bool P, bool FUSED>
+__device__ inline void split_body(const RhsArgs& a) {
   __shared__ double s0[kTN], sp[kTN], sT[kTN], su[kTN], sv[kTN], ss[kTN], stn[kTN];
   using Sc = Scheme<V>;
   const Geom& G = a.geom;
@@ -332,15 +371,15 @@ __global__ void __launch_bounds__(kThreads) rhs_split_kernel(RhsArgs a) {
     double H[5];
     cell_source<P>(t, e, a.H0[(long)i + (long)j * a.h0_sy], a, H);
     if constexpr (Sc::hybrid) tag = cell_sensor(s0, e, a.ep, 0);
-    double out[5];
     if (tag == 0) {
+      double out[5];
       cell_drp<P>(t, e, a, H, out);
+      cell_output<FUSED>(a, i, j, out);
     } else {
+      // WENO cell: keep H in the rhs slots; rhs_weno_list_kernel finishes it
 #pragma unroll
-      for (int k = 0; k < 5; k++) out[k] = H[k];  // finished by rhs_weno_list_kernel
+      for (int k = 0; k < 5; k++) a.KK[icell + k * ik] = H[k];
     }
-#pragma unroll
-    for (int k = 0; k < 5; k++) a.KK[icell + k * ik] = out[k];
     if (a.switch_map) a.switch_map[icell] = tag;
   }
   if constexpr (Sc::hybrid) {
@@ -357,8 +396,20 @@ __global__ void __launch_bounds__(kThreads) rhs_split_kernel(RhsArgs a) {
   }
 }
 
+// The fused form carries the runge_kutta and update_state epilogue; capping
+// it at 128 VGPRs keeps 4 waves per SIMD without spilling.
+template <int V, bool P, bool FUSED>
+__global__ void __launch_bounds__(kThreads) rhs_split_kernel(RhsArgs a) {
+  split_body<V, P, FUSED>(a);
+}
+template <int V, bool P, bool FUSED>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4)))
+rhs_split_fused_kernel(RhsArgs a) {
+  split_body<V, P, FUSED>(a);
+}
+
 // WENO cells of the list, one lane each, stencil from global memory.
-template <int V, bool P>
+template <int V, bool P, bool FUSED>
 __global__ void __launch_bounds__(256) rhs_weno_list_kernel(RhsArgs a) {
   using Sc = Scheme<V>;
   constexpr int R = Sc::order == 5 ? 3 : 2;
@@ -412,16 +463,15 @@ __global__ void __launch_bounds__(256) rhs_weno_list_kernel(RhsArgs a) {
           Y[k] = divc<P>(d, a.dy, a.inv_dy);
       }
     }
+    double K[5];
 #pragma unroll
-    for (int k = 0; k < 5; k++) {
-      long o = (long)cell + k * ik;
-      a.KK[o] = X[k] + Y[k] + a.KK[o];  // the slot holds H(k)
-    }
+    for (int k = 0; k < 5; k++) K[k] = X[k] + Y[k] + a.KK[(long)cell + k * ik];  // the slot holds H(k)
+    cell_output<FUSED>(a, i, j, K);
   }
 }
 
 // Every cell on the WENO path.
-template <int V, bool P>
+template <int V, bool P, bool FUSED>
 __global__ void __launch_bounds__(kThreads) rhs_dense_kernel(RhsArgs a) {
   __shared__ double sw[5][kTN];
   __shared__ double sp[kTN], sT[kTN], su[kTN], sv[kTN], ss[kTN], stn[kTN];
@@ -441,14 +491,11 @@ __global__ void __launch_bounds__(kThreads) rhs_dense_kernel(RhsArgs a) {
   t.s = ss;
   t.tn = stn;
   const int e = (threadIdx.y + kHalo) * kTW + threadIdx.x + kHalo;
-  const long icell = (long)i + (long)j * G.nx;
-  const long ik = (long)G.nx * G.ny;
   double H[5], out[5];
   cell_source<P>(t, e, a.H0[(long)i + (long)j * a.h0_sy], a, H);
   cell_weno_tile<V, P>(t, e, j, a, H, out);
-#pragma unroll
-  for (int k = 0; k < 5; k++) a.KK[icell + k * ik] = out[k];
-  if (a.switch_map) a.switch_map[icell] = 1;
+  cell_output<FUSED>(a, i, j, out);
+  if (a.switch_map) a.switch_map[(long)i + (long)j * G.nx] = 1;
 }
 
 __global__ void reset_count_kernel(unsigned* count) { *count = 0; }
@@ -459,7 +506,7 @@ bool is_pow2(double x) {
   return std::frexp(x, &e) == 0.5;
 }
 
-template <int V, bool P>
+template <int V, bool P, bool F>
 hipError_t launch_t(const RhsArgs& a, hipStream_t s) {
   using Sc = Scheme<V>;
   const Geom& G = a.geom;
@@ -467,66 +514,68 @@ hipError_t launch_t(const RhsArgs& a, hipStream_t s) {
   dim3 grid((G.nx + kBX - 1) / kBX, (G.ny + kBY - 1) / kBY);
   if constexpr (!Sc::drp) {
     // rhs_weno_*: every cell on the WENO path
-    hipLaunchKernelGGL((rhs_dense_kernel<V, P>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((rhs_dense_kernel<V, P, F>), grid, block, 0, s, a);
   } else if constexpr (!Sc::hybrid) {
     // rhs_drp: every cell on the DRP path
-    hipLaunchKernelGGL((rhs_split_kernel<V, P>), grid, block, 0, s, a);
+    if constexpr (F)
+      hipLaunchKernelGGL((rhs_split_fused_kernel<V, P, F>), grid, block, 0, s, a);
+    else
+      hipLaunchKernelGGL((rhs_split_kernel<V, P, F>), grid, block, 0, s, a);
   } else if (a.level != 0) {
     // hybrid above level 0: the sensor is off, every cell WENO
-    hipLaunchKernelGGL((rhs_dense_kernel<V, P>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((rhs_dense_kernel<V, P, F>), grid, block, 0, s, a);
   } else {
     if (!a.list || !a.count) return hipErrorInvalidValue;
     hipLaunchKernelGGL(reset_count_kernel, dim3(1), dim3(1), 0, s, a.count);
-    hipLaunchKernelGGL((rhs_split_kernel<V, P>), grid, block, 0, s, a);
+    if constexpr (F)
+      hipLaunchKernelGGL((rhs_split_fused_kernel<V, P, F>), grid, block, 0, s, a);
+    else
+      hipLaunchKernelGGL((rhs_split_kernel<V, P, F>), grid, block, 0, s, a);
     // a grid that covers the machine; lanes stride over the list
     long cells = (long)G.nx * G.ny;
     long blocks = (cells + 255) / 256;
     if (blocks > 2048) blocks = 2048;
-    hipLaunchKernelGGL((rhs_weno_list_kernel<V, P>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((rhs_weno_list_kernel<V, P, F>), dim3((unsigned)blocks), dim3(256), 0, s, a);
   }
   return hipGetLastError();
 }
 
-template <int V>
+template <int V, bool F>
 hipError_t launch_v(const RhsArgs& a, hipStream_t s) {
-  if (is_pow2(a.dx) && is_pow2(a.dy)) return launch_t<V, true>(a, s);
-  return launch_t<V, false>(a, s);
+  if (is_pow2(a.dx) && is_pow2(a.dy)) return launch_t<V, true, F>(a, s);
+  return launch_t<V, false, F>(a, s);
+}
+
+template <bool F>
+hipError_t launch_any(int variant, const RhsArgs& a0, hipStream_t s) {
+  if (a0.geom.nx <= 0 || a0.geom.ny <= 0) return hipSuccess;
+  RhsArgs a = a0;
+  fill_rhs_constants(a);
+  switch (variant) {
+    case kHybridHighLlf: return launch_v<kHybridHighLlf, F>(a, s);
+    case kHybridHigh: return launch_v<kHybridHigh, F>(a, s);
+    case kHybridLowLlf: return launch_v<kHybridLowLlf, F>(a, s);
+    case kHybridLow: return launch_v<kHybridLow, F>(a, s);
+    case kWenoHighLlf: return launch_v<kWenoHighLlf, F>(a, s);
+    case kWenoHigh: return launch_v<kWenoHigh, F>(a, s);
+    case kWenoLowLlf: return launch_v<kWenoLowLlf, F>(a, s);
+    case kWenoLow: return launch_v<kWenoLow, F>(a, s);
+    case kDrp: return launch_v<kDrp, F>(a, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace
-
-void fill_rhs_constants(RhsArgs& a) {
-  a.inv_dx2 = 1.0 / (a.dx * a.dx);
-  a.inv_dy2 = 1.0 / (a.dy * a.dy);
-  a.inv_4dxdy = 1.0 / (4 * a.dx * a.dy);
-  a.inv_2dx = 1.0 / (2 * a.dx);
-  a.inv_2dy = 1.0 / (2 * a.dy);
-  a.inv_mdx = 1.0 / (-a.dx);
-  a.inv_mdy = 1.0 / (-a.dy);
-  a.inv_dx = 1.0 / a.dx;
-  a.inv_dy = 1.0 / a.dy;
-}
 
 bool rhs_needs_list(int variant, int level) {
   return variant <= kHybridLow && level == 0;
 }
 
-hipError_t launch_rhs(int variant, const RhsArgs& a0, hipStream_t s) {
-  if (a0.geom.nx <= 0 || a0.geom.ny <= 0) return hipSuccess;
-  RhsArgs a = a0;
-  fill_rhs_constants(a);
-  switch (variant) {
-    case kHybridHighLlf: return launch_v<kHybridHighLlf>(a, s);
-    case kHybridHigh: return launch_v<kHybridHigh>(a, s);
-    case kHybridLowLlf: return launch_v<kHybridLowLlf>(a, s);
-    case kHybridLow: return launch_v<kHybridLow>(a, s);
-    case kWenoHighLlf: return launch_v<kWenoHighLlf>(a, s);
-    case kWenoHigh: return launch_v<kWenoHigh>(a, s);
-    case kWenoLowLlf: return launch_v<kWenoLowLlf>(a, s);
-    case kWenoLow: return launch_v<kWenoLow>(a, s);
-    case kDrp: return launch_v<kDrp>(a, s);
-    default: return hipErrorInvalidValue;
-  }
+hipError_t launch_rhs(int variant, const RhsArgs& a, hipStream_t s) { return launch_any<false>(variant, a, s); }
+
+hipError_t launch_stage(int variant, const RhsArgs& a, hipStream_t s) {
+  if (!a.ww || !a.w_out || !a.p_out || !a.T_out || a.w_out == a.w) return hipErrorInvalidValue;
+  return launch_any<true>(variant, a, s);
 }
 
 }  // namespace enl

@@ -139,6 +139,25 @@ class DevicePatch:
             _ptr(self.T), _ptr(self.src), _ptr(self.ww), _ptr(self.KK))
         _lib.check(rc)
 
+    def advance_fused(self, nsteps, dt, variant=None, level=0, rk=RK3):
+        """Same steps as advance(), one fused kernel per stage (bit-identical
+        results; the ghosts of w, p, T are not refreshed on return)."""
+        lib = _lib.load()
+        use_torch_stream()
+        variant = self.wl.variant if variant is None else variant
+        vid = _lib.VARIANT_ID[variant] if isinstance(variant, str) else int(variant)
+        nbytes = lib.enl_fused_scratch_bytes(self.box[0], self.box[1], self.box[2], self.box[3], self.g)
+        if getattr(self, "_scratch", None) is None or self._scratch.numel() < nbytes:
+            self._scratch = torch.empty(nbytes, dtype=torch.uint8, device=self.w.device)
+        a1, a2, b = rk
+        gas_s = self.gas.to_struct()
+        rc = lib.enl_advance_patch_fused(
+            vid, self.box[0], self.box[1], self.box[2], self.box[3], self.g, _darr(self.xlo),
+            _darr(self.dx), ctypes.byref(gas_s), (_c_int * 4)(*self.reflect), len(a1), _darr(a1),
+            _darr(a2), _darr(b), float(dt), int(nsteps), int(level), _ptr(self.w), _ptr(self.p),
+            _ptr(self.T), _ptr(self.src), ctypes.c_void_p(self._scratch.data_ptr()), nbytes)
+        _lib.check(rc)
+
     def to_host(self):
         torch.cuda.synchronize()
         return (self.w.cpu().numpy(), self.p.cpu().numpy(), self.T.cpu().numpy(), self.src.cpu().numpy())

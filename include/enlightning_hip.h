@@ -151,6 +151,23 @@ int enl_advance_patch(int variant, int ifirst0, int ilast0, int ifirst1, int ila
                       double dt, int nsteps, int level, double *w_n, double *pres, double *temp,
                       double *source, double *ww, double *KK);
 
+/* Bytes of device scratch enl_advance_patch_fused needs for this patch:
+ * two more ghosted states (w_n, pres, temp) and an interior rhs. */
+size_t enl_fused_scratch_bytes(int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw);
+
+/* Same steps and same results as enl_advance_patch (bit for bit), computed
+ * with one fused kernel per stage: each cell's rhs, its runge_kutta update
+ * and its update_state are applied in one pass (rhs is never stored for DRP
+ * cells), and the stages ping-pong between w_n and two scratch states, so
+ * the current -> scratch copy disappears.  scratch: device memory of at
+ * least enl_fused_scratch_bytes().  On return w_n, pres and temp hold the
+ * state on the interior (their ghosts are not refreshed). */
+int enl_advance_patch_fused(int variant, int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw,
+                            const double *xlo, const double *dx, const enl_gas_t *gas, const int *reflect,
+                            int order, const double *alpha_1, const double *alpha_2, const double *beta,
+                            double dt, int nsteps, int level, double *w_n, double *pres, double *temp,
+                            double *source, void *scratch, size_t scratch_bytes);
+
 /* ---- measurement hooks (no reference counterpart) ---------------------- */
 
 /* Kernel classes timed by the patch-loop entry points. */
@@ -160,7 +177,8 @@ enum {
   ENL_KERNEL_STATE = 2,
   ENL_KERNEL_BC = 3,
   ENL_KERNEL_COPY = 4,
-  ENL_NUM_KERNEL_KINDS = 5
+  ENL_KERNEL_STAGE = 5, /* fused rhs + runge_kutta + update_state */
+  ENL_NUM_KERNEL_KINDS = 6
 };
 
 /* Starts bracketing every launch of enl_advance_patch with HIP events on the

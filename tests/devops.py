@@ -70,6 +70,9 @@ else:
         def advance(self, nsteps, dt, variant=None, level=0, rk=RK3):
             super().advance(nsteps, dt, rk, variant=variant, level=level, tile=64, nthreads=8)
 
+        def advance_fused(self, nsteps, dt, variant=None, level=0, rk=RK3):
+            self.advance(nsteps, dt, variant=variant, level=level, rk=rk)
+
         def to_host(self):
             return self.w.copy(), self.p.copy(), self.T.copy(), self.src.copy()
 

@@ -104,8 +104,9 @@ def test_rhs_host_pointer_staging_matches_device_path():
     assert np.array_equal(KK_h, Kg) and np.array_equal(sm_h, smg)
 
 
-def test_runge_kutta_bit_exact():
-    nx, ny, g = 70, 33, 4
+@pytest.mark.parametrize("nx,g", [(70, 4), (71, 3), (64, 2)])  # 16-byte and scalar paths
+def test_runge_kutta_bit_exact(nx, g):
+    ny = 33
     rng = np.random.default_rng(1)
     w = rng.normal(size=(5, ny + 2 * g, nx + 2 * g))
     ww = rng.normal(size=(5, ny, nx))
@@ -119,9 +120,10 @@ def test_runge_kutta_bit_exact():
         assert np.array_equal(host(wd), wc)
 
 
-def test_update_state_parity():
+@pytest.mark.parametrize("nx,g", [(90, 4), (91, 4), (90, 3)])  # 16-byte and scalar paths
+def test_update_state_parity(nx, g):
     gas = Gas(c_lr=-3.9e-3)
-    nx, ny, g = 90, 21, 4
+    ny = 21
     w, p, T, _ = random_state(nx, ny, g, seed=3)
     box, xlo, dx = (0, nx - 1, 4, 4 + ny - 1), (0.0, 50.0), (0.5, 2.0)
     pc, Tc = p.copy(), T.copy()
