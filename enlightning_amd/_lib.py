@@ -31,6 +31,12 @@ class EnlGas(ctypes.Structure):
         ("tag_tolerance", ctypes.c_double * 3)]
 
 
+class EnlPatchDesc(ctypes.Structure):
+    """Mirror of enl_patch_desc_t."""
+    _fields_ = [("level", ctypes.c_int), ("box", ctypes.c_int * 4), ("xlo", ctypes.c_double * 2),
+                ("dx", ctypes.c_double * 2)]
+
+
 PROTOTYPES = {
     "rhs_": (_RHS_ARGS, None),
     "rhs_hybrid_high_llf_": (_RHS_ARGS, None),
@@ -56,6 +62,17 @@ PROTOTYPES = {
     "enl_fused_scratch_bytes": ([_I] * 5, ctypes.c_size_t),
     "enl_advance_patch_fused": ([_I] * 6 + [_P, _P, ctypes.POINTER(EnlGas), _P, _I, _P, _P, _P, _D, _I, _I]
                                 + [_P] * 5 + [ctypes.c_size_t], _I),
+    "enl_batch_create": ([_I, ctypes.POINTER(EnlPatchDesc), _I, _I, ctypes.POINTER(ctypes.c_void_p)], _I),
+    "enl_batch_set_fill": ([_P, _I, _I] + [_P] * 4 + [_I] + [_P] * 6 + [_I] + [_P] * 4 + [_I] + [_P] * 4, _I),
+    "enl_batch_set_coarsen": ([_P, _I] + [_P] * 4, _I),
+    "enl_batch_set_tiles": ([_P, _I, _I], _I),
+    "enl_batch_patch_state": ([_P, _I] + [_P] * 4, _I),
+    "enl_batch_copy_state": ([_P, _I, _I] + [_P] * 4, _I),
+    "enl_batch_prepare": ([_P, _P], _I),
+    "enl_batch_fill": ([_P, _P], _I),
+    "enl_batch_update_state": ([_P, ctypes.POINTER(EnlGas)], _I),
+    "enl_batch_advance": ([_P, _I, _I, ctypes.POINTER(EnlGas), _P, _I, _P, _P, _P, _D, _I], _I),
+    "enl_batch_destroy": ([_P], _I),
     "enl_advance_patch": ([_I] * 6 + [_P, _P, ctypes.POINTER(EnlGas), _P, _I, _P, _P, _P, _D, _I, _I]
                           + [_P] * 6, _I),
 }

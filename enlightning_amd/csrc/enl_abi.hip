@@ -62,7 +62,7 @@ bool is_device_ptr(const void* p) {
 namespace {
 // Per-device WENO-list workspace of the rhs kernels.
 struct RhsWorkspace {
-  int* list = nullptr;
+  unsigned long long* list = nullptr;
   unsigned* count = nullptr;
   long cap = 0;
 };
@@ -70,7 +70,7 @@ std::mutex g_ws_mu;
 RhsWorkspace g_ws[64];
 }  // namespace
 
-int rhs_workspace(long cells, int** list, unsigned** count) {
+int rhs_workspace(long cells, unsigned long long** list, unsigned** count) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return set_error(-5, "hipGetDevice");
   std::lock_guard<std::mutex> lock(g_ws_mu);
@@ -83,7 +83,7 @@ int rhs_workspace(long cells, int** list, unsigned** count) {
     if (w.list) (void)hipFree(w.list);
     w.list = nullptr;
     w.cap = 0;
-    hipError_t e = hipMalloc((void**)&w.list, sizeof(int) * (size_t)cells);
+    hipError_t e = hipMalloc((void**)&w.list, sizeof(unsigned long long) * (size_t)cells);
     if (e != hipSuccess) return check_hip(e, "hipMalloc(rhs list)");
     w.cap = cells;
   }

@@ -1,0 +1,795 @@
+// enl_batch.hip -- all patches of an AMR hierarchy advanced together.
+//
+// The reference advances a hierarchy patch by patch: for every RK stage
+// SAMRAI fills the ghosts of each level (same-level copies, coarse-to-fine
+// LINEAR_REFINE, setPhysicalBoundaryConditions, enlightning.C:790-1013) and
+// calls enlightning::singleStep on each patch (enlightning.C:459-583); after
+// the step it averages fine data onto the coarse level (CONSERVATIVE_COARSEN).
+// Small patches (8..64 cells a side) make that launch-bound on a GPU.
+//
+// Here every patch lives in one device pool, described by a device patch
+// table.  Each stage runs level by level, as SAMRAI's method-of-lines loop
+// does (fill a level's ghosts, then step its patches, then the next level,
+// whose coarse-to-fine ghosts therefore see the coarse level already
+// advanced by this stage):
+//   * the ghost fill of the level from a precomputed schedule (copy /
+//     linear-refine / physical-BC passes: one launch each; the refine reads
+//     coarse cells through the coarse patches' ghosts, refreshed after the
+//     coarse level's stage, i.e. coarse interior data as SAMRAI's
+//     temporary coarse level provides);
+//   * one persistent launch of the fused stage kernel over the tile list of
+//     all patches of the level (rhs + runge_kutta + update_state per cell;
+//     WENO cells -- the sensor's picks on level 0, every cell above it -- go
+//     to a compacted list);
+//   * one persistent launch over that WENO list;
+// and the step ends with one coarsen launch.  The stage sequence is captured
+// once into a hipGraph and replayed.  A plain mode advances the same
+// hierarchy through the drop-in entry points, patch by patch, for comparison.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "enl_rhs_body.h"
+#include "enlightning_hip.h"
+
+#pragma clang fp contract(off)
+
+namespace enl {
+namespace {
+
+// One patch in device memory.  Offsets are in doubles from the pool base.
+struct PatchDev {
+  int nx, ny, g, level;
+  long sy, sk;      // ghosted row / component strides
+  long state[3];    // three ghosted state blocks [w_n(5), pres, temp], 7*sk each
+  long src;         // ghosted source
+  long src_int;     // interior copy of the source (the drop-in's layout)
+  long rhs;         // interior rhs slots, 5*nx*ny
+  double dx, dy, ylo;
+};
+
+struct BatchArgs {
+  double* pool;
+  const PatchDev* patches;
+  const int4* tiles;  // (patch, i0, j0, -) of the level being stepped
+  int ntiles;
+  int in, out, ww;  // state blocks read, written, and holding the step's start state
+  RhsArgs base;     // uniform part of the arguments (gas constants, RK coefficients, list)
+};
+
+__device__ inline void patch_args(const BatchArgs& b, int pid, RhsArgs& a) {
+  const PatchDev& pd = b.patches[pid];
+  a = b.base;
+  a.geom.nx = pd.nx;
+  a.geom.ny = pd.ny;
+  a.geom.g0 = pd.g;
+  a.geom.g1 = pd.g;
+  a.geom.sy = pd.sy;
+  a.geom.sk = pd.sk;
+  const double* in = b.pool + pd.state[b.in];
+  a.w = in;
+  a.p = in + 5 * pd.sk;
+  a.T = in + 6 * pd.sk;
+  a.H0 = b.pool + pd.src + pd.g + pd.g * pd.sy;
+  a.h0_sy = pd.sy;
+  a.KK = b.pool + pd.rhs;
+  a.dx = pd.dx;
+  a.dy = pd.dy;
+  a.ylo = pd.ylo;
+  a.level = pd.level;
+  fill_rhs_constants(a);
+  a.ww = b.pool + pd.state[b.ww];
+  a.ww_off = pd.g + pd.g * pd.sy;
+  a.ww_sy = pd.sy;
+  a.ww_sk = pd.sk;
+  double* out = b.pool + pd.state[b.out];
+  a.w_out = out;
+  a.p_out = out + 5 * pd.sk;
+  a.T_out = out + 6 * pd.sk;
+}
+
+#define ENL_BATCH_LDS(BX, BY, L)                                             \
+  using ShB = TileShape<BX, BY>;                                             \
+  __shared__ double lds_w0[ShB::TN], lds_p[ShB::TN], lds_T[ShB::TN];         \
+  __shared__ double lds_u[ShB::TN], lds_v[ShB::TN], lds_s[ShB::TN], lds_tn[ShB::TN]; \
+  TileLds<BX, BY> L;                                                         \
+  L.w[0] = lds_w0;                                                           \
+  L.p = lds_p;                                                               \
+  L.T = lds_T;                                                               \
+  L.u = lds_u;                                                               \
+  L.v = lds_v;                                                               \
+  L.s = lds_s;                                                               \
+  L.tn = lds_tn;                                                             \
+  L.w[1] = L.w[2] = L.w[3] = L.w[4] = nullptr;
+
+// Persistent over the tile list of all patches.
+template <int V, bool P, int BX, int BY>
+__global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(4)))
+batch_split_kernel(BatchArgs b) {
+  ENL_BATCH_LDS(BX, BY, L)
+  for (int t = blockIdx.x; t < b.ntiles; t += gridDim.x) {
+    const int4 tl = b.tiles[t];
+    RhsArgs a;
+    patch_args(b, tl.x, a);
+    split_body<V, P, true, BX, BY>(a, tl.y, tl.z, (unsigned)tl.x, L);
+    __syncthreads();  // the next tile reuses the LDS
+  }
+}
+
+template <int V, bool P>
+__global__ void __launch_bounds__(256) batch_list_kernel(BatchArgs b) {
+  const unsigned n = *b.base.count;
+  for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const unsigned long long e = b.base.list[q];
+    RhsArgs a;
+    patch_args(b, (int)(e >> 32), a);
+    list_cell<V, P, true>(a, (int)(e & 0xffffffffu));
+  }
+}
+
+__global__ void batch_reset_kernel(unsigned* count) { *count = 0; }
+
+// ---- ghost fill and coarsen -------------------------------------------------
+
+struct FillArgs {
+  double* pool;
+  const PatchDev* patches;
+  int buf;    // state block filled, or -1 for the source
+  int cbuf;   // state block the coarse-to-fine refine reads (-1: source)
+  int ncomp;  // 7 for a state block, 1 for the source
+  int reflect[4];
+};
+
+__device__ inline double* fill_base(const FillArgs& f, const PatchDev& p) {
+  return f.pool + (f.buf >= 0 ? p.state[f.buf] : p.src);
+}
+__device__ inline const double* coarse_base(const FillArgs& f, const PatchDev& p) {
+  return f.pool + (f.cbuf >= 0 ? p.state[f.cbuf] : p.src);
+}
+
+// same-level copy: every component of the cell
+__global__ void __launch_bounds__(256) fill_copy_kernel(FillArgs f, int n, const int* dp, const int* dl,
+                                                         const int* sp, const int* sl) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const PatchDev& D = f.patches[dp[q]];
+    const PatchDev& S = f.patches[sp[q]];
+    double* d = fill_base(f, D) + dl[q];
+    const double* s = fill_base(f, S) + sl[q];
+    for (int k = 0; k < f.ncomp; k++) d[k * D.sk] = s[k * S.sk];
+  }
+}
+
+// linear refinement from the coarse cell holding the fine ghost cell:
+// c + fx*(c[i+1]-c[i-1])/2 + fy*(c[j+1]-c[j-1])/2, (fx, fy) the fine cell
+// centre's offset from the coarse centre in coarse cell widths
+__global__ void __launch_bounds__(256) fill_interp_kernel(FillArgs f, int n, const int* dp, const int* dl,
+                                                           const int* sp, const int* sl, const double* fx,
+                                                           const double* fy) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const PatchDev& D = f.patches[dp[q]];
+    const PatchDev& C = f.patches[sp[q]];
+    double* d = fill_base(f, D) + dl[q];
+    const double* c = coarse_base(f, C) + sl[q];
+    const double ax = fx[q], ay = fy[q];
+    for (int k = 0; k < f.ncomp; k++) {
+      const double* ck = c + k * C.sk;
+      d[k * D.sk] = ck[0] + ax * (0.5 * (ck[1] - ck[-1])) + ay * (0.5 * (ck[C.sy] - ck[-C.sy]));
+    }
+  }
+}
+
+// physical boundary: copy within the patch; side 1..4 = left, right, bottom,
+// top selects the REFLECT flag and the momentum component it negates
+__global__ void __launch_bounds__(256) fill_bc_kernel(FillArgs f, int n, const int* pp, const int* dl,
+                                                       const int* sl, const int* side) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const PatchDev& P = f.patches[pp[q]];
+    double* base = fill_base(f, P);
+    const int sd = side[q];
+    const int negk = (f.ncomp > 2 && f.reflect[sd - 1]) ? (sd <= 2 ? 1 : 2) : -1;
+    for (int k = 0; k < f.ncomp; k++) {
+      double v = base[sl[q] + k * P.sk];
+      base[dl[q] + k * P.sk] = k == negk ? -v : v;
+    }
+  }
+}
+
+// fine-to-coarse average of the r x r fine cells over a coarse cell
+__global__ void __launch_bounds__(256) coarsen_kernel(double* pool, const PatchDev* patches, int buf, int r, int n,
+                                                       const int* cp, const int* cl, const int* fp, const int* fl) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const PatchDev& C = patches[cp[q]];
+    const PatchDev& F = patches[fp[q]];
+    double* c = pool + C.state[buf] + cl[q];
+    const double* fb = pool + F.state[buf] + fl[q];
+    const double inv = 1.0 / (double)(r * r);
+    for (int k = 0; k < 7; k++) {
+      double s = 0;
+      for (int fj = 0; fj < r; fj++)
+        for (int fi = 0; fi < r; fi++) s += fb[fi + fj * F.sy + k * F.sk];
+      c[k * C.sk] = s * inv;
+    }
+  }
+}
+
+__global__ void copy_interior_block_kernel(double* dst, const double* src, PatchDev p) {
+  // state A interior (w_n) -> the patch's interior ww slots (rhs layout)
+  const int n = p.nx * p.ny;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < 5 * n; q += gridDim.x * blockDim.x) {
+    int k = q / n, c = q % n, i = c % p.nx, j = c / p.nx;
+    dst[q] = src[(i + p.g) + (long)(j + p.g) * p.sy + (long)k * p.sk];
+  }
+}
+
+unsigned grid_for(long n, int block = 256, long cap = 4096) {
+  long b = (n + block - 1) / block;
+  if (b < 1) b = 1;
+  return (unsigned)(b > cap ? cap : b);
+}
+
+bool pow2(double x) {
+  if (!(x > 0) || !std::isfinite(x)) return false;
+  int e;
+  return std::frexp(x, &e) == 0.5;
+}
+
+// Device copies of one level's fill schedule.
+struct LevelFill {
+  int ncopy = 0, ninterp = 0, nedge = 0, ncorner = 0;
+  int *c_dp = nullptr, *c_dl = nullptr, *c_sp = nullptr, *c_sl = nullptr;
+  int *i_dp = nullptr, *i_dl = nullptr, *i_sp = nullptr, *i_sl = nullptr;
+  double *i_fx = nullptr, *i_fy = nullptr;
+  int *e_p = nullptr, *e_d = nullptr, *e_s = nullptr, *e_n = nullptr;
+  int *k_p = nullptr, *k_d = nullptr, *k_s = nullptr, *k_n = nullptr;
+};
+
+}  // namespace
+
+struct Batch {
+  int npatch = 0, g = 0, ratio = 0, nlevels = 0;
+  std::vector<PatchDev> hp;
+  std::vector<enl_patch_desc_t> desc;
+  double* pool = nullptr;
+  size_t pool_doubles = 0;
+  PatchDev* dp = nullptr;
+  int4* tiles = nullptr;
+  int ntiles = 0, tile_bx = 0, tile_by = 0;
+  int level_tile_begin[9] = {0};  // tiles sorted by level
+  bool all_pow2 = true;
+  LevelFill fill[8];
+  int ncoarsen = 0;
+  int *co_cp = nullptr, *co_cl = nullptr, *co_fp = nullptr, *co_fl = nullptr;
+  unsigned long long* list = nullptr;
+  unsigned* count = nullptr;
+  long cells = 0;
+  std::vector<void*> allocations;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<unsigned char> graph_key;
+};
+
+namespace {
+
+template <typename T>
+int upload(Batch* b, T** dst, const T* src, long n) {
+  *dst = nullptr;
+  if (n <= 0) return 0;
+  hipError_t e = hipMalloc((void**)dst, sizeof(T) * (size_t)n);
+  if (e != hipSuccess) return check_hip(e, "hipMalloc(batch table)");
+  b->allocations.push_back(*dst);
+  return check_hip(hipMemcpy(*dst, src, sizeof(T) * (size_t)n, hipMemcpyHostToDevice), "upload batch table");
+}
+
+int make_tiles(Batch* b, int bx, int by) {
+  std::vector<int4> t;
+  for (int lev = 0; lev < 8; lev++) {
+    b->level_tile_begin[lev] = (int)t.size();
+    for (int p = 0; p < b->npatch; p++)
+      if (b->hp[p].level == lev)
+        for (int j0 = 0; j0 < b->hp[p].ny; j0 += by)
+          for (int i0 = 0; i0 < b->hp[p].nx; i0 += bx) t.push_back(make_int4(p, i0, j0, 0));
+  }
+  b->level_tile_begin[8] = (int)t.size();
+  if (b->tiles) {
+    (void)hipFree(b->tiles);
+    for (auto& a : b->allocations)
+      if (a == b->tiles) a = nullptr;
+  }
+  b->ntiles = (int)t.size();
+  b->tile_bx = bx;
+  b->tile_by = by;
+  return upload(b, &b->tiles, t.data(), (long)t.size());
+}
+
+void fill_level(Batch* b, int level, int buf, int cbuf, int ncomp, const int* reflect, hipStream_t s) {
+  FillArgs f;
+  f.pool = b->pool;
+  f.patches = b->dp;
+  f.buf = buf;
+  f.cbuf = cbuf;
+  f.ncomp = ncomp;
+  for (int k = 0; k < 4; k++) f.reflect[k] = reflect[k];
+  const LevelFill& L = b->fill[level];
+  if (L.ncopy)
+    hipLaunchKernelGGL(fill_copy_kernel, dim3(grid_for(L.ncopy)), dim3(256), 0, s, f, L.ncopy, L.c_dp, L.c_dl,
+                       L.c_sp, L.c_sl);
+  if (L.ninterp)
+    hipLaunchKernelGGL(fill_interp_kernel, dim3(grid_for(L.ninterp)), dim3(256), 0, s, f, L.ninterp, L.i_dp,
+                       L.i_dl, L.i_sp, L.i_sl, L.i_fx, L.i_fy);
+  if (L.nedge)
+    hipLaunchKernelGGL(fill_bc_kernel, dim3(grid_for(L.nedge)), dim3(256), 0, s, f, L.nedge, L.e_p, L.e_d, L.e_s,
+                       L.e_n);
+  if (L.ncorner)
+    hipLaunchKernelGGL(fill_bc_kernel, dim3(grid_for(L.ncorner)), dim3(256), 0, s, f, L.ncorner, L.k_p, L.k_d,
+                       L.k_s, L.k_n);
+}
+
+void fill_all(Batch* b, int buf, const int* reflect, hipStream_t s) {
+  for (int lev = 0; lev < b->nlevels; lev++) fill_level(b, lev, buf, buf, 7, reflect, s);
+}
+
+RhsArgs base_args(Batch* b, const enl_gas_t* gas) {
+  RhsArgs a;
+  memset(&a, 0, sizeof a);
+  a.alpha = gas->weno_alpha;
+  a.kappa = gas->kappa;
+  a.mu = gas->mu;
+  a.muB = gas->muB;
+  a.R = gas->R;
+  a.T_star_n = gas->T_star_n;
+  a.tau_n = gas->tau_n;
+  a.c = gas->c;
+  a.c_lr = gas->c_lr;
+  a.ep = sensor_eps(gas->tag_tolerance[2]);
+  a.rho0 = gas->rho0;
+  a.gamma1 = gas->gamma1;
+  a.c_p = gas->c_p;
+  a.p0 = gas->p0;
+  a.T0 = gas->T0;
+  a.c_v = gas->c_v;
+  a.list = b->list;
+  a.count = b->count;
+  a.switch_map = nullptr;
+  return a;
+}
+
+template <int V, bool P, int BX, int BY>
+void stage_launch(Batch* b, const BatchArgs& ba, hipStream_t s) {
+  if (ba.ntiles <= 0) return;
+  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1), 0, s, b->count);
+  unsigned blocks = (unsigned)(ba.ntiles < 256 * 16 ? ba.ntiles : 256 * 16);
+  hipLaunchKernelGGL((batch_split_kernel<V, P, BX, BY>), dim3(blocks), dim3(BX, BY), 0, s, ba);
+  hipLaunchKernelGGL((batch_list_kernel<V, P>), dim3(grid_for(b->cells, 256, 2048)), dim3(256), 0, s, ba);
+}
+
+template <int V, bool P>
+int stage_shape(Batch* b, const BatchArgs& ba, hipStream_t s) {
+  if (b->tile_bx == 32 && b->tile_by == 8) stage_launch<V, P, 32, 8>(b, ba, s);
+  else if (b->tile_bx == 16 && b->tile_by == 16) stage_launch<V, P, 16, 16>(b, ba, s);
+  else if (b->tile_bx == 16 && b->tile_by == 8) stage_launch<V, P, 16, 8>(b, ba, s);
+  else if (b->tile_bx == 8 && b->tile_by == 8) stage_launch<V, P, 8, 8>(b, ba, s);
+  else return set_error(-2, "unsupported tile shape");
+  return 0;
+}
+
+template <bool P>
+int stage_variant(Batch* b, int variant, const BatchArgs& ba, hipStream_t s) {
+  switch (variant) {
+    case kHybridHighLlf: return stage_shape<kHybridHighLlf, P>(b, ba, s);
+    case kHybridHigh: return stage_shape<kHybridHigh, P>(b, ba, s);
+    case kHybridLowLlf: return stage_shape<kHybridLowLlf, P>(b, ba, s);
+    case kHybridLow: return stage_shape<kHybridLow, P>(b, ba, s);
+    case kWenoHighLlf: return stage_shape<kWenoHighLlf, P>(b, ba, s);
+    case kWenoHigh: return stage_shape<kWenoHigh, P>(b, ba, s);
+    case kWenoLowLlf: return stage_shape<kWenoLowLlf, P>(b, ba, s);
+    case kWenoLow: return stage_shape<kWenoLow, P>(b, ba, s);
+    case kDrp: return stage_shape<kDrp, P>(b, ba, s);
+    default: return set_error(-1, "unknown rhs variant");
+  }
+}
+
+// Enqueues nsteps batched steps on stream s.  Per stage and level: fill the
+// level's ghosts on the stage input (its coarse-to-fine refine reading the
+// coarser level's stage output), step the level's tiles, and refresh the
+// level's ghosts on its output for the next finer level's refine.
+int enqueue_batched(Batch* b, int variant, const enl_gas_t* gas, const int* reflect, int order, const double* a1,
+                    const double* a2, const double* be, double dt, int nsteps, hipStream_t s) {
+  BatchArgs ba;
+  ba.pool = b->pool;
+  ba.patches = b->dp;
+  ba.base = base_args(b, gas);
+  ba.ww = 0;
+  for (int step = 0; step < nsteps; step++) {
+    int in = 0;
+    for (int st = 0; st < order; st++) {
+      int out = (st == order - 1 && order > 1) ? 0 : (st % 2 == 0 ? 1 : 2);
+      ba.in = in;
+      ba.out = out;
+      ba.base.alpha_1 = a1[st];
+      ba.base.alpha_2 = a2[st];
+      ba.base.bdt = be[st] * dt;
+      for (int lev = 0; lev < b->nlevels; lev++) {
+        fill_level(b, lev, in, out, 7, reflect, s);
+        ba.tiles = b->tiles + b->level_tile_begin[lev];
+        ba.ntiles = b->level_tile_begin[lev + 1] - b->level_tile_begin[lev];
+        int rc = b->all_pow2 ? stage_variant<true>(b, variant, ba, s) : stage_variant<false>(b, variant, ba, s);
+        if (rc) return rc;
+        if (lev + 1 < b->nlevels) fill_level(b, lev, out, out, 7, reflect, s);
+      }
+      in = out;
+    }
+    if (order == 1) {
+      for (int p = 0; p < b->npatch; p++) {
+        const PatchDev& pd = b->hp[p];
+        hipMemcpyAsync(b->pool + pd.state[0], b->pool + pd.state[1], sizeof(double) * 7 * pd.sk,
+                       hipMemcpyDeviceToDevice, s);
+      }
+    }
+    if (b->ncoarsen)
+      hipLaunchKernelGGL(coarsen_kernel, dim3(grid_for(b->ncoarsen)), dim3(256), 0, s, b->pool, b->dp, 0, b->ratio,
+                         b->ncoarsen, b->co_cp, b->co_cl, b->co_fp, b->co_fl);
+  }
+  return check_hip(hipGetLastError(), "batched stage launch");
+}
+
+// The reference's way: the drop-in entry points called patch by patch
+// (SAMRAI's copy current -> scratch, then per stage the ghost fill and
+// rhs_ / runge_kutta_ / update_state_ on every patch).  ww goes into state
+// block 1 (interior layout), the rhs into the rhs slots.
+int enqueue_plain(Batch* b, int variant, const enl_gas_t* gas, const int* reflect, int order, const double* a1,
+                  const double* a2, const double* be, double dt, int nsteps, hipStream_t s) {
+  const int nequ = 5;
+  const int nv = variant;
+  for (int step = 0; step < nsteps; step++) {
+    for (int p = 0; p < b->npatch; p++) {
+      const PatchDev& pd = b->hp[p];
+      hipLaunchKernelGGL(copy_interior_block_kernel, dim3(grid_for(5L * pd.nx * pd.ny)), dim3(256), 0, s,
+                         b->pool + pd.state[1], b->pool + pd.state[0], pd);
+    }
+    for (int st = 0; st < order; st++) {
+      for (int lev = 0; lev < b->nlevels; lev++) {
+      fill_level(b, lev, 0, 0, 7, reflect, s);
+      for (int p = 0; p < b->npatch; p++) {
+        const PatchDev& pd = b->hp[p];
+        if (pd.level != lev) continue;
+        const enl_patch_desc_t& d = b->desc[p];
+        double* w = b->pool + pd.state[0];
+        double* pres = w + 5 * pd.sk;
+        double* temp = w + 6 * pd.sk;
+        double* ww = b->pool + pd.state[1];
+        double* KK = b->pool + pd.rhs;
+        double* src = b->pool + pd.src_int;  // the drop-in takes the source interior-only
+        int rc = enl_rhs_variant(nv, &d.box[0], &d.box[1], &d.box[2], &d.box[3], &b->g, &b->g, d.xlo, d.dx, w, src,
+                                 KK, pres, temp, &gas->weno_alpha, &gas->kappa, &gas->mu, &gas->muB, &gas->R,
+                                 &gas->T_star_n, &gas->T_star_o, &gas->tau_n, &gas->tau_o, gas->tag_tolerance,
+                                 &d.level, &gas->c, &gas->c_lr, &nequ, nullptr);
+        if (rc) return rc;
+        runge_kutta_(&d.box[0], &d.box[1], &d.box[2], &d.box[3], &b->g, &b->g, &dt, &a1[st], &a2[st], &be[st], w,
+                     ww, KK, &nequ);
+        update_state_(d.dx, d.xlo, &d.box[0], &d.box[1], &d.box[2], &d.box[3], &b->g, &b->g, w, pres, temp,
+                      &gas->rho0, &gas->c, &gas->gamma1, &gas->c_p, &gas->p0, &gas->T0, &gas->c_lr, &gas->c_v,
+                      &gas->R_tilde, &gas->M, &gas->R, &nequ);
+        char msg[8];
+        int code = enl_last_error(msg, sizeof msg);
+        if (code) return set_error(code, "plain per-patch call failed");
+      }
+      if (lev + 1 < b->nlevels) fill_level(b, lev, 0, 0, 7, reflect, s);
+      }
+    }
+    if (b->ncoarsen)
+      hipLaunchKernelGGL(coarsen_kernel, dim3(grid_for(b->ncoarsen)), dim3(256), 0, s, b->pool, b->dp, 0, b->ratio,
+                         b->ncoarsen, b->co_cp, b->co_cl, b->co_fp, b->co_fl);
+  }
+  return check_hip(hipGetLastError(), "plain hierarchy launch");
+}
+
+}  // namespace
+}  // namespace enl
+
+using namespace enl;
+
+extern "C" {
+
+int enl_batch_create(int npatch, const enl_patch_desc_t* patches, int gcw, int ratio, void** handle) {
+  *handle = nullptr;
+  if (npatch <= 0 || gcw < 3) return set_error(-2, "enl_batch_create: need patches and gcw >= 3");
+  Batch* b = new Batch();
+  b->npatch = npatch;
+  b->g = gcw;
+  b->ratio = ratio;
+  b->desc.assign(patches, patches + npatch);
+  long off = 0;
+  for (int p = 0; p < npatch; p++) {
+    const enl_patch_desc_t& d = patches[p];
+    PatchDev pd;
+    pd.nx = d.box[1] - d.box[0] + 1;
+    pd.ny = d.box[3] - d.box[2] + 1;
+    if (pd.nx <= 0 || pd.ny <= 0 || d.level < 0 || d.level > 7) {
+      delete b;
+      return set_error(-2, "enl_batch_create: bad patch %d", p);
+    }
+    pd.g = gcw;
+    pd.level = d.level;
+    pd.sy = pd.nx + 2L * gcw;
+    pd.sk = pd.sy * (pd.ny + 2L * gcw);
+    for (int s = 0; s < 3; s++) {
+      pd.state[s] = off;
+      off += 7 * pd.sk;
+      off = (off + 31) & ~31L;  // 256-byte aligned blocks
+    }
+    pd.src = off;
+    off += pd.sk;
+    off = (off + 31) & ~31L;
+    pd.src_int = off;
+    off += (long)pd.nx * pd.ny;
+    off = (off + 31) & ~31L;
+    pd.rhs = off;
+    off += 5L * pd.nx * pd.ny;
+    off = (off + 31) & ~31L;
+    pd.dx = d.dx[0];
+    pd.dy = d.dx[1];
+    pd.ylo = d.xlo[1];
+    b->all_pow2 = b->all_pow2 && pow2(pd.dx) && pow2(pd.dy);
+    b->hp.push_back(pd);
+    b->cells += (long)pd.nx * pd.ny;
+    if (d.level + 1 > b->nlevels) b->nlevels = d.level + 1;
+  }
+  b->pool_doubles = (size_t)off;
+  hipError_t e = hipMalloc((void**)&b->pool, sizeof(double) * b->pool_doubles);
+  if (e != hipSuccess) {
+    delete b;
+    return check_hip(e, "hipMalloc(batch pool)");
+  }
+  b->allocations.push_back(b->pool);
+  (void)hipMemset(b->pool, 0, sizeof(double) * b->pool_doubles);
+  int rc = upload(b, &b->dp, b->hp.data(), npatch);
+  if (!rc) {
+    e = hipMalloc((void**)&b->list, sizeof(unsigned long long) * (size_t)b->cells);
+    if (e == hipSuccess) b->allocations.push_back(b->list);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->count, sizeof(unsigned));
+    if (e == hipSuccess) b->allocations.push_back(b->count);
+    if (e != hipSuccess) rc = check_hip(e, "hipMalloc(batch list)");
+  }
+  if (!rc) rc = make_tiles(b, 32, 8);
+  if (rc) {
+    for (void* a : b->allocations)
+      if (a) (void)hipFree(a);
+    delete b;
+    return rc;
+  }
+  *handle = b;
+  return 0;
+}
+
+int enl_batch_set_fill(void* handle, int level, int ncopy, const int* c_dp, const int* c_dl, const int* c_sp,
+                       const int* c_sl, int ninterp, const int* i_dp, const int* i_dl, const int* i_sp,
+                       const int* i_sl, const double* i_fx, const double* i_fy, int nedge, const int* e_p,
+                       const int* e_d, const int* e_s, const int* e_n, int ncorner, const int* k_p, const int* k_d,
+                       const int* k_s, const int* k_n) {
+  Batch* b = (Batch*)handle;
+  if (!b || level < 0 || level >= 8) return set_error(-2, "enl_batch_set_fill: bad level");
+  LevelFill& L = b->fill[level];
+  L.ncopy = ncopy;
+  L.ninterp = ninterp;
+  L.nedge = nedge;
+  L.ncorner = ncorner;
+  int rc = 0;
+  rc = rc ? rc : upload(b, &L.c_dp, c_dp, ncopy);
+  rc = rc ? rc : upload(b, &L.c_dl, c_dl, ncopy);
+  rc = rc ? rc : upload(b, &L.c_sp, c_sp, ncopy);
+  rc = rc ? rc : upload(b, &L.c_sl, c_sl, ncopy);
+  rc = rc ? rc : upload(b, &L.i_dp, i_dp, ninterp);
+  rc = rc ? rc : upload(b, &L.i_dl, i_dl, ninterp);
+  rc = rc ? rc : upload(b, &L.i_sp, i_sp, ninterp);
+  rc = rc ? rc : upload(b, &L.i_sl, i_sl, ninterp);
+  rc = rc ? rc : upload(b, &L.i_fx, i_fx, ninterp);
+  rc = rc ? rc : upload(b, &L.i_fy, i_fy, ninterp);
+  rc = rc ? rc : upload(b, &L.e_p, e_p, nedge);
+  rc = rc ? rc : upload(b, &L.e_d, e_d, nedge);
+  rc = rc ? rc : upload(b, &L.e_s, e_s, nedge);
+  rc = rc ? rc : upload(b, &L.e_n, e_n, nedge);
+  rc = rc ? rc : upload(b, &L.k_p, k_p, ncorner);
+  rc = rc ? rc : upload(b, &L.k_d, k_d, ncorner);
+  rc = rc ? rc : upload(b, &L.k_s, k_s, ncorner);
+  rc = rc ? rc : upload(b, &L.k_n, k_n, ncorner);
+  return rc;
+}
+
+int enl_batch_set_coarsen(void* handle, int n, const int* cp, const int* cl, const int* fp, const int* fl) {
+  Batch* b = (Batch*)handle;
+  if (!b) return set_error(-2, "null batch");
+  b->ncoarsen = n;
+  int rc = 0;
+  rc = rc ? rc : upload(b, &b->co_cp, cp, n);
+  rc = rc ? rc : upload(b, &b->co_cl, cl, n);
+  rc = rc ? rc : upload(b, &b->co_fp, fp, n);
+  rc = rc ? rc : upload(b, &b->co_fl, fl, n);
+  return rc;
+}
+
+int enl_batch_set_tiles(void* handle, int bx, int by) {
+  Batch* b = (Batch*)handle;
+  if (!b) return set_error(-2, "null batch");
+  if (!((bx == 32 && by == 8) || (bx == 16 && by == 16) || (bx == 16 && by == 8) || (bx == 8 && by == 8)))
+    return set_error(-2, "unsupported tile shape %dx%d", bx, by);
+  if (b->graph_exec) {
+    (void)hipGraphExecDestroy(b->graph_exec);
+    (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+  }
+  return make_tiles(b, bx, by);
+}
+
+int enl_batch_patch_state(void* handle, int pid, double** w_n, double** pres, double** temp, double** source) {
+  Batch* b = (Batch*)handle;
+  if (!b || pid < 0 || pid >= b->npatch) return set_error(-2, "bad patch id");
+  const PatchDev& pd = b->hp[pid];
+  double* w = b->pool + pd.state[0];
+  if (w_n) *w_n = w;
+  if (pres) *pres = w + 5 * pd.sk;
+  if (temp) *temp = w + 6 * pd.sk;
+  if (source) *source = b->pool + pd.src;
+  return 0;
+}
+
+int enl_batch_copy_state(void* handle, int pid, int to_device, double* w_n, double* pres, double* temp,
+                         double* source) {
+  Batch* b = (Batch*)handle;
+  if (!b || pid < 0 || pid >= b->npatch) return set_error(-2, "bad patch id");
+  const PatchDev& pd = b->hp[pid];
+  double* w = b->pool + pd.state[0];
+  double* dev[4] = {w, w + 5 * pd.sk, w + 6 * pd.sk, b->pool + pd.src};
+  double* host[4] = {w_n, pres, temp, source};
+  size_t n[4] = {5 * (size_t)pd.sk, (size_t)pd.sk, (size_t)pd.sk, (size_t)pd.sk};
+  hipStream_t s = lib_stream();
+  for (int a = 0; a < 4; a++) {
+    if (!host[a]) continue;
+    hipError_t e = to_device ? hipMemcpyAsync(dev[a], host[a], sizeof(double) * n[a], hipMemcpyHostToDevice, s)
+                             : hipMemcpyAsync(host[a], dev[a], sizeof(double) * n[a], hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return check_hip(e, "enl_batch_copy_state");
+  }
+  return check_hip(hipStreamSynchronize(s), "enl_batch_copy_state sync");
+}
+
+int enl_batch_prepare(void* handle, const int* reflect) {
+  // ghosts of the source (constant over the run) and its interior copy for
+  // the plain mode's drop-in calls
+  Batch* b = (Batch*)handle;
+  if (!b) return set_error(-2, "null batch");
+  hipStream_t s = lib_stream();
+  for (int lev = 0; lev < b->nlevels; lev++) fill_level(b, lev, -1, -1, 1, reflect, s);
+  for (int p = 0; p < b->npatch; p++) {
+    const PatchDev& pd = b->hp[p];
+    hipError_t e = hipMemcpy2DAsync(b->pool + pd.src_int, sizeof(double) * pd.nx,
+                                    b->pool + pd.src + pd.g + pd.g * pd.sy, sizeof(double) * pd.sy,
+                                    sizeof(double) * pd.nx, pd.ny, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return check_hip(e, "source interior copy");
+  }
+  return check_hip(hipGetLastError(), "enl_batch_prepare");
+}
+
+int enl_batch_fill(void* handle, const int* reflect) {
+  Batch* b = (Batch*)handle;
+  if (!b) return set_error(-2, "null batch");
+  fill_all(b, 0, reflect, lib_stream());
+  return check_hip(hipGetLastError(), "enl_batch_fill");
+}
+
+int enl_batch_update_state(void* handle, const enl_gas_t* gas) {
+  // p, T of every patch interior from its w_n (update_state.m4)
+  Batch* b = (Batch*)handle;
+  if (!b) return set_error(-2, "null batch");
+  hipStream_t s = lib_stream();
+  for (int p = 0; p < b->npatch; p++) {
+    const PatchDev& pd = b->hp[p];
+    StateArgs a;
+    double* w = b->pool + pd.state[0];
+    a.w = w;
+    a.p = w + 5 * pd.sk;
+    a.T = w + 6 * pd.sk;
+    a.geom.nx = pd.nx;
+    a.geom.ny = pd.ny;
+    a.geom.g0 = a.geom.g1 = pd.g;
+    a.geom.sy = pd.sy;
+    a.geom.sk = pd.sk;
+    a.dy = pd.dy;
+    a.ylo = pd.ylo;
+    a.rho0 = gas->rho0;
+    a.c = gas->c;
+    a.gamma1 = gas->gamma1;
+    a.c_p = gas->c_p;
+    a.p0 = gas->p0;
+    a.T0 = gas->T0;
+    a.c_lr = gas->c_lr;
+    a.c_v = gas->c_v;
+    a.R = gas->R;
+    hipError_t e = launch_update_state(a, s);
+    if (e != hipSuccess) return check_hip(e, "batch update_state");
+  }
+  return 0;
+}
+
+int enl_batch_advance(void* handle, int mode, int variant, const enl_gas_t* gas, const int* reflect, int order,
+                      const double* alpha_1, const double* alpha_2, const double* beta, double dt, int nsteps) {
+  Batch* b = (Batch*)handle;
+  if (!b) return set_error(-2, "null batch");
+  if (variant < 0 || variant >= kNumVariants) return set_error(-1, "unknown rhs variant");
+  if (order < 1 || order > 8) return set_error(-2, "RK order out of range");
+  hipStream_t s = lib_stream();
+  if (mode == ENL_BATCH_PLAIN) return enqueue_plain(b, variant, gas, reflect, order, alpha_1, alpha_2, beta, dt, nsteps, s);
+  if (mode == ENL_BATCH_FUSED)
+    return enqueue_batched(b, variant, gas, reflect, order, alpha_1, alpha_2, beta, dt, nsteps, s);
+  if (mode != ENL_BATCH_GRAPH) return set_error(-2, "unknown batch mode");
+  // capture one step into a hipGraph (re-captured when the step changes) and
+  // replay it nsteps times
+  std::vector<unsigned char> key;
+  auto put = [&key](const void* p, size_t n) {
+    const unsigned char* c = (const unsigned char*)p;
+    key.insert(key.end(), c, c + n);
+  };
+  put(&variant, sizeof variant);
+  put(&order, sizeof order);
+  put(&dt, sizeof dt);
+  put(gas, sizeof *gas);
+  put(reflect, 4 * sizeof(int));
+  put(alpha_1, order * sizeof(double));
+  put(alpha_2, order * sizeof(double));
+  put(beta, order * sizeof(double));
+  put(&b->tile_bx, sizeof(int));
+  put(&b->tile_by, sizeof(int));
+  bool same = b->graph_exec && key == b->graph_key;
+  if (!same) {
+    if (b->graph_exec) {
+      (void)hipGraphExecDestroy(b->graph_exec);
+      (void)hipGraphDestroy(b->graph);
+      b->graph_exec = nullptr;
+      b->graph = nullptr;
+    }
+    hipStream_t cs;
+    hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e != hipSuccess) return check_hip(e, "capture stream");
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+      (void)hipStreamDestroy(cs);
+      return check_hip(e, "hipStreamBeginCapture");
+    }
+    int rc = enqueue_batched(b, variant, gas, reflect, order, alpha_1, alpha_2, beta, dt, 1, cs);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(cs, &g);
+    (void)hipStreamDestroy(cs);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return check_hip(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      return check_hip(e, "hipGraphInstantiate");
+    }
+    b->graph = g;
+    b->graph_key = key;
+  }
+  for (int step = 0; step < nsteps; step++) {
+    hipError_t e = hipGraphLaunch(b->graph_exec, s);
+    if (e != hipSuccess) return check_hip(e, "hipGraphLaunch");
+  }
+  return 0;
+}
+
+int enl_batch_destroy(void* handle) {
+  Batch* b = (Batch*)handle;
+  if (!b) return 0;
+  (void)hipDeviceSynchronize();
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (void* a : b->allocations)
+    if (a) (void)hipFree(a);
+  delete b;
+  return 0;
+}
+
+}  // extern "C"

@@ -41,7 +41,7 @@ struct RhsArgs {
   int level;
   // WENO-cell list of the split hybrid path (nx*ny ints) and its length;
   // needed when rhs_needs_list(variant, level)
-  int* list;
+  unsigned long long* list;  // entries (patch << 32) | (i + j*nx)
   unsigned* count;
   // reciprocals of the grid constants (filled by launch_rhs / launch_stage)
   double inv_dx2, inv_dy2, inv_4dxdy, inv_2dx, inv_2dy, inv_mdx, inv_mdy, inv_dx, inv_dy;
@@ -111,6 +111,6 @@ int check_hip(hipError_t e, const char* what);
 bool is_device_ptr(const void* p);
 // Device workspace of the current device (grow-only, owned by the library):
 // the WENO-cell list for `cells` cells and its counter.
-int rhs_workspace(long cells, int** list, unsigned** count);
+int rhs_workspace(long cells, unsigned long long** list, unsigned** count);
 
 }  // namespace enl

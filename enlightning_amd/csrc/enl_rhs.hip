@@ -28,10 +28,7 @@
 //   * the sensor only needs the sign of r - 0.9999f: a single-precision
 //     estimate decides it unless it lies within 1e-6 of the threshold, where
 //     the exact double quotient is computed (same decision either way).
-#include <cmath>
-
-#include "enl_device.h"
-#include "enl_kernels.h"
+#include "enl_rhs_body.h"
 
 #pragma clang fp contract(off)
 
@@ -40,462 +37,52 @@ namespace {
 
 constexpr int kBX = 32;
 constexpr int kBY = 8;
-constexpr int kHalo = 3;
-constexpr int kTW = kBX + 2 * kHalo;
-constexpr int kTH = kBY + 2 * kHalo;
-constexpr int kTN = kTW * kTH;
-constexpr int kThreads = kBX * kBY;
+using Sh = TileShape<kBX, kBY>;
 
-void fill_rhs_constants(RhsArgs& a) {
-  a.inv_dx2 = 1.0 / (a.dx * a.dx);
-  a.inv_dy2 = 1.0 / (a.dy * a.dy);
-  a.inv_4dxdy = 1.0 / (4 * a.dx * a.dy);
-  a.inv_2dx = 1.0 / (2 * a.dx);
-  a.inv_2dy = 1.0 / (2 * a.dy);
-  a.inv_mdx = 1.0 / (-a.dx);
-  a.inv_mdy = 1.0 / (-a.dy);
-  a.inv_dx = 1.0 / a.dx;
-  a.inv_dy = 1.0 / a.dy;
-}
-
-// a / c for a grid constant c; a * (1/c) when 1/c is exact (POW2).
-template <bool POW2>
-__device__ inline double divc(double a, double c, double inv_c) {
-  if constexpr (POW2)
-    return a * inv_c;
-  else
-    return a / c;
-}
-
-// Pointers into the LDS tile; offsets are relative to a cell centre, row
-// stride kTW.  w[k] is only staged by the dense kernel.
-struct Tile {
-  const double* w0;
-  const double* w[5];
-  const double* p;
-  const double* T;
-  const double* u;
-  const double* v;
-  const double* s;
-  const double* tn;
-};
-
-// Inviscid fluxes F (x) and G (y), rhs_hybrid_high_llf.m4:125-156.
-__device__ inline double flux_f(double w0, double u, double v, double s, double tn, double p, int k) {
-  switch (k) {
-    case 0: return w0 * u;
-    case 1: return w0 * sq(u) + p;
-    case 2: return w0 * u * v;
-    case 3: return w0 * u * s;
-    default: return w0 * u * tn;
+// LDS for one tile: `raw` adds the four raw momenta etc. (dense kernel).
+#define ENL_TILE_LDS(L, raw)                                             \
+  __shared__ double lds_w0[Sh::TN], lds_p[Sh::TN], lds_T[Sh::TN];        \
+  __shared__ double lds_u[Sh::TN], lds_v[Sh::TN], lds_s[Sh::TN], lds_tn[Sh::TN]; \
+  TileLds<kBX, kBY> L;                                                   \
+  L.w[0] = lds_w0;                                                       \
+  L.p = lds_p;                                                           \
+  L.T = lds_T;                                                           \
+  L.u = lds_u;                                                           \
+  L.v = lds_v;                                                           \
+  L.s = lds_s;                                                           \
+  L.tn = lds_tn;                                                         \
+  L.w[1] = L.w[2] = L.w[3] = L.w[4] = nullptr;                           \
+  if (raw) {                                                             \
+    __shared__ double lds_raw[4][Sh::TN];                                \
+    for (int k_ = 1; k_ < 5; k_++) L.w[k_] = lds_raw[k_ - 1];            \
   }
-}
-__device__ inline double flux_g(double w0, double u, double v, double s, double tn, double p, int k) {
-  switch (k) {
-    case 0: return w0 * v;
-    case 1: return w0 * v * u;
-    case 2: return w0 * sq(v) + p;
-    case 3: return w0 * v * s;
-    default: return w0 * v * tn;
-  }
-}
-__device__ inline double tflux_f(const Tile& t, int e, int k) {
-  return flux_f(t.w0[e], t.u[e], t.v[e], t.s[e], t.tn[e], t.p[e], k);
-}
-__device__ inline double tflux_g(const Tile& t, int e, int k) {
-  return flux_g(t.w0[e], t.u[e], t.v[e], t.s[e], t.tn[e], t.p[e], k);
-}
-
-// Result of one cell: store rhs (plain), or apply the runge_kutta update
-// and update_state and store the new state (fused stage).
-template <bool FUSED>
-__device__ inline void cell_output(const RhsArgs& a, int i, int j, const double K[5]) {
-  const Geom& G = a.geom;
-  const long c = (long)i + (long)j * G.nx;
-  const long ik = (long)G.nx * G.ny;
-  if constexpr (!FUSED) {
-#pragma unroll
-    for (int k = 0; k < 5; k++) a.KK[c + k * ik] = K[k];
-  } else {
-    const long g = (long)(i + G.g0) + (long)(j + G.g1) * G.sy;
-    double wn[5];
-#pragma unroll
-    for (int k = 0; k < 5; k++) {
-      wn[k] = a.alpha_1 * a.ww[a.ww_off + i + j * a.ww_sy + k * a.ww_sk] + a.alpha_2 * a.w[g + k * G.sk] +
-              a.bdt * K[k];
-      a.w_out[g + k * G.sk] = wn[k];
-    }
-    const double xc1 = a.ylo + a.dy * ((double)(j) + 0.5);
-    double p, T;
-    eos_p_T(a.c, a.c_lr, a.rho0, a.gamma1, a.c_p, a.p0, a.T0, a.c_v, a.R, xc1, wn[0], wn[3], p, T);
-    a.p_out[g] = p;
-    a.T_out[g] = T;
-  }
-}
-
-// Physical source H, rhs_hybrid_high_llf.m4:159-237.
-template <bool P>
-__device__ inline void cell_source(const Tile& t, int e, double H0, const RhsArgs& a, double H[5]) {
-  constexpr int S = kTW;
-  const double dx = a.dx, dy = a.dy;
-  const double* U = t.u;
-  const double* V = t.v;
-  const double* TT = t.T;
-  const double dx2 = sq(dx), dy2 = sq(dy), dxy4 = 4 * dx * dy, dx_2 = 2 * dx, dy_2 = 2 * dy;
-  H[0] = H0;
-
-  double u_dprime_x = divc<P>(U[e - 1] - 2 * U[e] + U[e + 1], dx2, a.inv_dx2);
-  double u_dprime_y = divc<P>(U[e - S] - 2 * U[e] + U[e + S], dy2, a.inv_dy2);
-  double v_dprime_xy = divc<P>(V[e + 1 + S] - V[e + 1 - S] - V[e - 1 + S] + V[e - 1 - S], dxy4, a.inv_4dxdy);
-  double phi_xx_prime_x = kC4_3 * u_dprime_x - kC2_3 * v_dprime_xy;
-  double phi_xy_prime_y = u_dprime_y + v_dprime_xy;
-  H[1] = a.muB * (u_dprime_x + v_dprime_xy) + a.mu * (phi_xx_prime_x + phi_xy_prime_y);
-
-  double v_dprime_y = divc<P>(V[e - S] - 2 * V[e] + V[e + S], dy2, a.inv_dy2);
-  double v_dprime_x = divc<P>(V[e - 1] - 2 * V[e] + V[e + 1], dx2, a.inv_dx2);
-  double u_dprime_yx = divc<P>(U[e + 1 + S] - U[e + 1 - S] - U[e - 1 + S] + U[e - 1 - S], dxy4, a.inv_4dxdy);
-  double phi_yx_prime_x = u_dprime_yx + v_dprime_x;
-  double phi_yy_prime_y = kC4_3 * v_dprime_y - kC2_3 * u_dprime_yx;
-  H[2] = a.muB * (v_dprime_y + u_dprime_yx) + a.mu * (phi_yx_prime_x + phi_yy_prime_y);
-
-  double u_prime_x = divc<P>(U[e + 1] - U[e - 1], dx_2, a.inv_2dx);
-  double u_prime_y = divc<P>(U[e + S] - U[e - S], dy_2, a.inv_2dy);
-  double v_prime_y = divc<P>(V[e + S] - V[e - S], dy_2, a.inv_2dy);
-  double v_prime_x = divc<P>(V[e + 1] - V[e - 1], dx_2, a.inv_2dx);
-  double t_prime_x = divc<P>(TT[e + 1] - TT[e - 1], dx_2, a.inv_2dx);
-  double t_prime_y = divc<P>(TT[e + S] - TT[e - S], dy_2, a.inv_2dy);
-  double t_dprime_x = divc<P>(TT[e - 1] - 2 * TT[e] + TT[e + 1], dx2, a.inv_dx2);
-  double t_dprime_y = divc<P>(TT[e - S] - 2 * TT[e] + TT[e + S], dy2, a.inv_dy2);
-  double w0 = t.w0[e];
-  double temp = TT[e];
-  double T_n = t.tn[e];
-  double Mn = (temp - T_n) / a.tau_n;
-  double tmp1 = a.T_star_n / (T_n);
-  double c_vn = kC078 * a.R * sq(tmp1) * exp(-tmp1);
-  double A_n = ((temp / T_n) - 1) * c_vn;
-  double phi_xx = kC4_3 * u_prime_x - kC2_3 * v_prime_y;
-  double phi_xy_yx = u_prime_y + v_prime_x;
-  double phi_yy = kC4_3 * v_prime_y - kC2_3 * u_prime_x;
-  double sigma = (a.muB / temp) * sq(u_prime_x + v_prime_y) +
-                 (a.mu / (2 * temp)) * (sq(phi_xx) + 2 * sq(phi_xy_yx) + sq(phi_yy)) +
-                 (a.kappa / sq(temp)) * (sq(t_prime_x) + sq(t_prime_y)) + (w0 / temp) * (A_n * Mn);
-  H[3] = sigma - (w0 / T_n) * c_vn * Mn +
-         (-((a.kappa * sq(t_prime_x)) / sq(temp)) + ((a.kappa * t_dprime_x) / temp) -
-          ((a.kappa * sq(t_prime_y)) / sq(temp)) + ((a.kappa * t_dprime_y) / temp));
-  H[4] = (w0 / a.tau_n) * (temp - T_n);
-}
-
-// r = (2|t1 t2| + ep) / (t1^2 + t2^2 + ep) < float(0.9999)?  The numerator
-// and denominator are formed exactly as in rhs_hybrid_high_llf.m4:246; the
-// comparison of their IEEE quotient with the threshold is decided from a
-// single-precision estimate (relative error < 3e-7) whenever that estimate
-// is more than 1e-6 away from the threshold, else from the exact quotient.
-__device__ inline bool ratio_below(double tmp1, double tmp2, double ep) {
-  double num = 2 * fabs(tmp1 * tmp2) + ep;
-  double den = sq(tmp1) + sq(tmp2) + ep;
-  if (num > 1e-30 && num < 1e30 && den > 1e-30 && den < 1e30) {
-    float q = (float)num * __builtin_amdgcn_rcpf((float)den);
-    if (q < (float)(kThresh * (1 - 1e-6))) return true;
-    if (q > (float)(kThresh * (1 + 1e-6))) return false;
-  }
-  return num / den < kThresh;
-}
-
-// Hybrid density sensor, rhs_hybrid_high_llf.m4:239-256.  1 = WENO.
-__device__ inline int cell_sensor(const double* r, int e, double ep, int level) {
-  constexpr int S = kTW;
-  int tag = 1;
-  if (level == 0) {
-    tag = 0;
-#pragma unroll
-    for (int q = -1; q <= 1; q++) {
-      if (ratio_below(r[e + q] - r[e - 1 + q], r[e + 1 + q] - r[e + q], ep)) tag = 1;
-      if (ratio_below(r[e + q * S] - r[e + (q - 1) * S], r[e + (q + 1) * S] - r[e + q * S], ep)) tag = 1;
-    }
-  }
-  return tag;
-}
-
-// DRP branch, rhs_hybrid_high_llf.m4:258-268.
-template <bool P>
-__device__ inline void cell_drp(const Tile& t, int e, const RhsArgs& a, const double H[5], double out[5]) {
-  constexpr int S = kTW;
-  const double mdx = -a.dx, mdy = -a.dy;
-#pragma unroll
-  for (int k = 0; k < 5; k++) {
-    double fx = -kA3 * tflux_f(t, e - 3, k) - kA2 * tflux_f(t, e - 2, k) - kA1 * tflux_f(t, e - 1, k) +
-                kA1 * tflux_f(t, e + 1, k) + kA2 * tflux_f(t, e + 2, k) + kA3 * tflux_f(t, e + 3, k);
-    double gy = -kA3 * tflux_g(t, e - 3 * S, k) - kA2 * tflux_g(t, e - 2 * S, k) - kA1 * tflux_g(t, e - S, k) +
-                kA1 * tflux_g(t, e + S, k) + kA2 * tflux_g(t, e + 2 * S, k) + kA3 * tflux_g(t, e + 3 * S, k);
-    out[k] = divc<P>(fx, mdx, a.inv_mdx) + divc<P>(gy, mdy, a.inv_mdy) + H[k];
-  }
-}
-
-// One direction of the WENO branch for component k: split fluxes at the
-// 2R+1 points and the four reconstructions pf1, mf1, pf2, mf2
-// (rhs_hybrid_high_llf.m4:286-354); returns pf1 + mf1 - pf2 - mf2.
-template <int ORDER>
-__device__ inline double weno_dir(const double* F, const double* W, double alpha) {
-  double fp[7], fm[7];
-  constexpr int R = ORDER == 5 ? 3 : 2;
-#pragma unroll
-  for (int m = -R; m <= R; m++) {
-    fp[m + 3] = 0.5 * (F[m + 3] + alpha * W[m + 3]);
-    fm[m + 3] = 0.5 * (F[m + 3] - alpha * W[m + 3]);
-  }
-  double p1, m1, p2, m2;
-  if constexpr (ORDER == 5) {
-    p1 = wenop5(fp[0], fp[1], fp[2], fp[3], fp[4]);
-    m1 = wenom5(fm[1], fm[2], fm[3], fm[4], fm[5]);
-    p2 = wenop5(fp[1], fp[2], fp[3], fp[4], fp[5]);
-    m2 = wenom5(fm[2], fm[3], fm[4], fm[5], fm[6]);
-  } else {
-    p1 = wenop3(fp[1], fp[2], fp[3]);
-    m1 = wenom3(fm[2], fm[3], fm[4]);
-    p2 = wenop3(fp[2], fp[3], fp[4]);
-    m2 = wenom3(fm[3], fm[4], fm[5]);
-  }
-  return p1 + m1 - p2 - m2;
-}
-
-// Per-cell LLF wave speeds, rhs_hybrid_high_llf.m4:270-284: alpha_f over
-// |u| in the x window at this row's height, alpha_g over |v| in the y
-// window at each point's height.  `u` and `v` hold the window (index m+3).
-template <int REACH>
-__device__ inline void llf_alpha(const double* u, const double* v, int j, const RhsArgs& a, double& af,
-                                 double& ag) {
-  af = 0;
-  ag = 0;
-#pragma unroll
-  for (int q = -REACH; q <= REACH; q++) {
-    double height = a.ylo + a.dy * ((double)(j) + 0.5);
-    double vf = fabs(u[q + 3]) + a.c + a.c_lr * height;
-    if (vf > af) af = vf;
-    height = a.ylo + a.dy * ((double)(j + q) + 0.5);
-    double vg = fabs(v[q + 3]) + a.c + a.c_lr * height;
-    if (vg > ag) ag = vg;
-  }
-}
-
-// WENO branch of one cell out of an LDS tile that holds the raw w_k.
-template <int V, bool P>
-__device__ inline void cell_weno_tile(const Tile& t, int e, int j, const RhsArgs& a, const double H[5],
-                                      double out[5]) {
-  using Sc = Scheme<V>;
-  constexpr int S = kTW;
-  constexpr int R = Sc::order == 5 ? 3 : 2;
-  double af, ag;
-  if constexpr (Sc::llf) {
-    double uu[7], vv[7];
-#pragma unroll
-    for (int q = -R; q <= R; q++) {
-      uu[q + 3] = t.u[e + q];
-      vv[q + 3] = t.v[e + q * S];
-    }
-    llf_alpha<R>(uu, vv, j, a, af, ag);
-  } else {
-    af = a.alpha;
-    ag = a.alpha;
-  }
-#pragma unroll
-  for (int k = 0; k < 5; k++) {
-    double F[7], Wx[7], G[7], Wy[7];
-#pragma unroll
-    for (int m = -R; m <= R; m++) {
-      F[m + 3] = tflux_f(t, e + m, k);
-      Wx[m + 3] = t.w[k][e + m];
-      G[m + 3] = tflux_g(t, e + m * S, k);
-      Wy[m + 3] = t.w[k][e + m * S];
-    }
-    double x = weno_dir<Sc::order>(F, Wx, af);
-    double y = weno_dir<Sc::order>(G, Wy, ag);
-    out[k] = divc<P>(x, a.dx, a.inv_dx) + divc<P>(y, a.dy, a.inv_dy) + H[k];
-  }
-}
-
-// Stage rho, p, T and the quotients (and the raw w_k if RAW) of the tile +
-// halo.  Cells outside the ghosted array are skipped; no interior cell of
-// the patch reads them.
-template <bool RAW>
-__device__ inline void stage_tile(const RhsArgs& a, int i0, int j0, double* s0, double* sp, double* sT,
-                                  double* su, double* sv, double* ss, double* stn, double (*sw)[kTN]) {
-  const Geom& G = a.geom;
-  const int tid = threadIdx.x + threadIdx.y * kBX;
-  for (int e = tid; e < kTN; e += kThreads) {
-    int gi = i0 + (e % kTW) - kHalo, gj = j0 + (e / kTW) - kHalo;
-    if (gi >= -G.g0 && gi < G.nx + G.g0 && gj >= -G.g1 && gj < G.ny + G.g1) {
-      long gidx = (long)(gi + G.g0) + (long)(gj + G.g1) * G.sy;
-      double w0 = a.w[gidx], w1 = a.w[gidx + G.sk], w2 = a.w[gidx + 2 * G.sk];
-      double w3 = a.w[gidx + 3 * G.sk], w4 = a.w[gidx + 4 * G.sk];
-      s0[e] = w0;
-      sp[e] = a.p[gidx];
-      sT[e] = a.T[gidx];
-      su[e] = w1 / w0;
-      sv[e] = w2 / w0;
-      ss[e] = w3 / w0;
-      stn[e] = w4 / w0;
-      if constexpr (RAW) {
-        sw[1][e] = w1;
-        sw[2][e] = w2;
-        sw[3][e] = w3;
-        sw[4][e] = w4;
-      }
-    }
-  }
-}
 
 template <int V, bool P, bool FUSED>
-__device__ inline void split_body(const RhsArgs& a) {
-  __shared__ double s0[kTN], sp[kTN], sT[kTN], su[kTN], sv[kTN], ss[kTN], stn[kTN];
-  using Sc = Scheme<V>;
-  const Geom& G = a.geom;
-  const int i0 = blockIdx.x * kBX, j0 = blockIdx.y * kBY;
-  stage_tile<false>(a, i0, j0, s0, sp, sT, su, sv, ss, stn, nullptr);
-  __syncthreads();
-
-  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
-  const bool valid = i < G.nx && j < G.ny;
-  int tag = 0;
-  if (valid) {
-    Tile t;
-    t.w0 = s0;
-    t.p = sp;
-    t.T = sT;
-    t.u = su;
-    t.v = sv;
-    t.s = ss;
-    t.tn = stn;
-    const int e = (threadIdx.y + kHalo) * kTW + threadIdx.x + kHalo;
-    const long icell = (long)i + (long)j * G.nx;
-    const long ik = (long)G.nx * G.ny;
-    double H[5];
-    cell_source<P>(t, e, a.H0[(long)i + (long)j * a.h0_sy], a, H);
-    if constexpr (Sc::hybrid) tag = cell_sensor(s0, e, a.ep, 0);
-    if (tag == 0) {
-      double out[5];
-      cell_drp<P>(t, e, a, H, out);
-      cell_output<FUSED>(a, i, j, out);
-    } else {
-      // WENO cell: keep H in the rhs slots; rhs_weno_list_kernel finishes it
-#pragma unroll
-      for (int k = 0; k < 5; k++) a.KK[icell + k * ik] = H[k];
-    }
-    if (a.switch_map) a.switch_map[icell] = tag;
-  }
-  if constexpr (Sc::hybrid) {
-    // append this wave's WENO cells to the list: one atomic per wave
-    unsigned long long mask = __ballot(tag);
-    if (mask) {
-      int lane = __lane_id();
-      int leader = __ffsll((unsigned long long)mask) - 1;
-      unsigned base = 0;
-      if (lane == leader) base = atomicAdd(a.count, (unsigned)__popcll(mask));
-      base = __shfl(base, leader);
-      if (tag) a.list[base + __popcll(mask & __lanemask_lt())] = i + j * G.nx;
-    }
-  }
+__global__ void __launch_bounds__(Sh::threads) rhs_split_kernel(RhsArgs a) {
+  ENL_TILE_LDS(L, false)
+  split_body<V, P, FUSED, kBX, kBY>(a, blockIdx.x * kBX, blockIdx.y * kBY, 0, L);
 }
-
 // The fused form carries the runge_kutta and update_state epilogue; capping
-// it at 128 VGPRs keeps 4 waves per SIMD without spilling.
+// it at 128 VGPRs keeps 4 waves per SIMD.
 template <int V, bool P, bool FUSED>
-__global__ void __launch_bounds__(kThreads) rhs_split_kernel(RhsArgs a) {
-  split_body<V, P, FUSED>(a);
-}
-template <int V, bool P, bool FUSED>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(Sh::threads) __attribute__((amdgpu_waves_per_eu(4)))
 rhs_split_fused_kernel(RhsArgs a) {
-  split_body<V, P, FUSED>(a);
+  ENL_TILE_LDS(L, false)
+  split_body<V, P, FUSED, kBX, kBY>(a, blockIdx.x * kBX, blockIdx.y * kBY, 0, L);
 }
 
-// WENO cells of the list, one lane each, stencil from global memory.
+// WENO cells of the list, one lane each.
 template <int V, bool P, bool FUSED>
 __global__ void __launch_bounds__(256) rhs_weno_list_kernel(RhsArgs a) {
-  using Sc = Scheme<V>;
-  constexpr int R = Sc::order == 5 ? 3 : 2;
-  const Geom& G = a.geom;
   const unsigned n = *a.count;
-  const long ik = (long)G.nx * G.ny;
-  for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-    const int cell = a.list[q];
-    const int i = cell % G.nx, j = cell / G.nx;
-    const long g = (long)(i + G.g0) + (long)(j + G.g1) * G.sy;
-    double X[5], Y[5];
-#pragma unroll
-    for (int dir = 0; dir < 2; dir++) {
-      const long step = dir == 0 ? 1 : G.sy;
-      double w0[7], u[7], v[7], s[7], tn[7], p[7];
-#pragma unroll
-      for (int m = -R; m <= R; m++) {
-        long gm = g + m * step;
-        double r = a.w[gm];
-        w0[m + 3] = r;
-        u[m + 3] = a.w[gm + G.sk] / r;
-        v[m + 3] = a.w[gm + 2 * G.sk] / r;
-        s[m + 3] = a.w[gm + 3 * G.sk] / r;
-        tn[m + 3] = a.w[gm + 4 * G.sk] / r;
-        p[m + 3] = a.p[gm];
-      }
-      double alpha = a.alpha;
-      if constexpr (Sc::llf) {
-        // the x window gives alpha_f (height of row j), the y window alpha_g
-        alpha = 0;
-#pragma unroll
-        for (int m = -R; m <= R; m++) {
-          double height = a.ylo + a.dy * ((double)(dir == 0 ? j : j + m) + 0.5);
-          double vv = fabs(dir == 0 ? u[m + 3] : v[m + 3]) + a.c + a.c_lr * height;
-          if (vv > alpha) alpha = vv;
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < 5; k++) {
-        double F[7], W[7];
-#pragma unroll
-        for (int m = -R; m <= R; m++) {
-          F[m + 3] = dir == 0 ? flux_f(w0[m + 3], u[m + 3], v[m + 3], s[m + 3], tn[m + 3], p[m + 3], k)
-                              : flux_g(w0[m + 3], u[m + 3], v[m + 3], s[m + 3], tn[m + 3], p[m + 3], k);
-          W[m + 3] = a.w[g + m * step + k * G.sk];
-        }
-        double d = weno_dir<Sc::order>(F, W, alpha);
-        if (dir == 0)
-          X[k] = divc<P>(d, a.dx, a.inv_dx);
-        else
-          Y[k] = divc<P>(d, a.dy, a.inv_dy);
-      }
-    }
-    double K[5];
-#pragma unroll
-    for (int k = 0; k < 5; k++) K[k] = X[k] + Y[k] + a.KK[(long)cell + k * ik];  // the slot holds H(k)
-    cell_output<FUSED>(a, i, j, K);
-  }
+  for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+    list_cell<V, P, FUSED>(a, (int)(a.list[q] & 0xffffffffu));
 }
 
-// Every cell on the WENO path.
 template <int V, bool P, bool FUSED>
-__global__ void __launch_bounds__(kThreads) rhs_dense_kernel(RhsArgs a) {
-  __shared__ double sw[5][kTN];
-  __shared__ double sp[kTN], sT[kTN], su[kTN], sv[kTN], ss[kTN], stn[kTN];
-  const Geom& G = a.geom;
-  const int i0 = blockIdx.x * kBX, j0 = blockIdx.y * kBY;
-  stage_tile<true>(a, i0, j0, sw[0], sp, sT, su, sv, ss, stn, sw);
-  __syncthreads();
-  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
-  if (i >= G.nx || j >= G.ny) return;
-  Tile t;
-  t.w0 = sw[0];
-  for (int k = 0; k < 5; k++) t.w[k] = sw[k];
-  t.p = sp;
-  t.T = sT;
-  t.u = su;
-  t.v = sv;
-  t.s = ss;
-  t.tn = stn;
-  const int e = (threadIdx.y + kHalo) * kTW + threadIdx.x + kHalo;
-  double H[5], out[5];
-  cell_source<P>(t, e, a.H0[(long)i + (long)j * a.h0_sy], a, H);
-  cell_weno_tile<V, P>(t, e, j, a, H, out);
-  cell_output<FUSED>(a, i, j, out);
-  if (a.switch_map) a.switch_map[(long)i + (long)j * G.nx] = 1;
+__global__ void __launch_bounds__(Sh::threads) rhs_dense_kernel(RhsArgs a) {
+  ENL_TILE_LDS(L, true)
+  dense_body<V, P, FUSED, kBX, kBY>(a, blockIdx.x * kBX, blockIdx.y * kBY, L);
 }
 
 __global__ void reset_count_kernel(unsigned* count) { *count = 0; }

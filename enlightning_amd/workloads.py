@@ -65,38 +65,57 @@ SMALL_256 = Workload("pulse_256_single", n=256, steps=100, seed=0, fixed=True, a
 PULSES_4096 = Workload("pulses_4096_8", n=4096, steps=500, seed=1, npulses=8)
 
 
-def overpressure(wl):
-    """dp [Pa / p0 units] on the ghosted cell centres, shape (Ny, Nx)."""
-    n, g, dx = wl.n, wl.g, wl.dx
-    xc = (np.arange(-g, n + g) + 0.5) * dx
-    X = xc[None, :]
-    Y = xc[:, None]
-    dp = np.zeros((n + 2 * g, n + 2 * g))
+def pulses_at(wl, x, y):
+    """Sum of the workload's pulses (dp/p0) on the grid x (row) times y
+    (column) of cell centres; shape (len(y), len(x))."""
+    X = np.asarray(x)[None, :]
+    Y = np.asarray(y)[:, None]
+    dp = np.zeros((Y.shape[0], X.shape[1]))
     centers, amps, sigmas = wl.pulses()
     for (cx, cy), a, s in zip(centers, amps, sigmas):
         # separable evaluation keeps this cheap at 4096^2
         ex = np.exp(-((X - cx) ** 2) / (2 * s * s))
         ey = np.exp(-((Y - cy) ** 2) / (2 * s * s))
         dp += a * (ey * ex)
+    return dp
+
+
+def overpressure(wl):
+    """dp [Pa / p0 units] on the ghosted cell centres, shape (Ny, Nx)."""
+    n, g, dx = wl.n, wl.g, wl.dx
+    xc = (np.arange(-g, n + g) + 0.5) * dx
+    dp = pulses_at(wl, xc, xc)
     if wl.noise:
         rng = np.random.default_rng(wl.seed)
         dp[g:g + n, g:g + n] += rng.uniform(-1.0, 1.0, size=(n, n)) * wl.noise
     return dp
 
 
+def state_from_overpressure(dp_over_p0, gas):
+    """w (5, Ny, Nx) at zero entropy and rest for an overpressure field."""
+    dp = dp_over_p0 * gas.p0
+    a = (gas.gamma1 - 1.0) / (2.0 * gas.rho0)
+    rho_p = (-1.0 + np.sqrt(1.0 + 4.0 * a * dp / gas.c ** 2)) / (2.0 * a)
+    rho = gas.rho0 + rho_p
+    w = np.zeros((5,) + rho.shape)
+    w[0] = rho
+    w[4] = rho * gas.T0
+    return w
+
+
 def initial_fields(wl, gas):
     """Host arrays (w, p, T, source) of the initial state.  p and T hold the
     ambient values; the caller derives them from w with update_state and
     fills the ghosts with the physical boundary routine."""
-    dp = overpressure(wl) * gas.p0
-    a = (gas.gamma1 - 1.0) / (2.0 * gas.rho0)
-    rho_p = (-1.0 + np.sqrt(1.0 + 4.0 * a * dp / gas.c ** 2)) / (2.0 * a)
-    rho = gas.rho0 + rho_p
+    w = state_from_overpressure(overpressure(wl), gas)
     N = wl.n + 2 * wl.g
-    w = np.zeros((5, N, N))
-    w[0] = rho
-    w[4] = rho * gas.T0
     p = np.full((N, N), gas.p0)
     T = np.full((N, N), gas.T0)
     src = np.zeros((N, N))
     return w, p, T, src
+
+
+# The batched-hierarchy workload: base 1024^2 on [0,1]^2 m in 64^2 patches,
+# one finer level (ratio 4) from tag_cells, eight seeded pulses as in
+# PULSES_4096 (seed 2).
+AMR_1024 = Workload("amr_1024_r4_8pulses", n=1024, steps=100, seed=2, npulses=8)

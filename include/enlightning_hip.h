@@ -168,6 +168,64 @@ int enl_advance_patch_fused(int variant, int ifirst0, int ilast0, int ifirst1, i
                             double dt, int nsteps, int level, double *w_n, double *pres, double *temp,
                             double *source, void *scratch, size_t scratch_bytes);
 
+/* ---- 3. batched hierarchy (device patch table) -------------------------- */
+
+/* One patch of a hierarchy: level, box in the level's index space
+ * (ifirst0, ilast0, ifirst1, ilast1) and geometry (SAMRAI's
+ * CartesianPatchGeometry xlo, dx). */
+typedef struct enl_patch_desc {
+  int level;
+  int box[4];
+  double xlo[2];
+  double dx[2];
+} enl_patch_desc_t;
+
+enum { ENL_BATCH_PLAIN = 0, ENL_BATCH_FUSED = 1, ENL_BATCH_GRAPH = 2 };
+
+/* Creates a batch: one device pool holding, per patch, three ghosted states
+ * [w_n(5), pres, temp] (SAMRAI CellData layout, 7 components), a ghosted
+ * source and interior scratch.  ratio: refinement ratio between levels. */
+int enl_batch_create(int npatch, const enl_patch_desc_t *patches, int gcw, int ratio, void **handle);
+/* Ghost-fill schedule of one level, as (patch, local index) pairs; local
+ * index = (i-ifirst0+g) + Nx*(j-ifirst1+g):
+ *   copy:    destination <- same-level source cell, all components;
+ *   interp:  destination <- linear refine of the coarse source cell,
+ *            c + fx*(c[+x]-c[-x])/2 + fy*(c[+y]-c[-y])/2;
+ *   edge:    destination <- same-patch source, side 1..4 (left, right,
+ *            bottom, top) negating the normal momentum at REFLECT walls
+ *            (enlightning.C:836-944);
+ *   corner:  same, applied after the edges (enlightning.C:948-1008). */
+int enl_batch_set_fill(void *handle, int level, int ncopy, const int *c_dp, const int *c_dl, const int *c_sp,
+                       const int *c_sl, int ninterp, const int *i_dp, const int *i_dl, const int *i_sp,
+                       const int *i_sl, const double *i_fx, const double *i_fy, int nedge, const int *e_p,
+                       const int *e_d, const int *e_s, const int *e_n, int ncorner, const int *k_p,
+                       const int *k_d, const int *k_s, const int *k_n);
+/* Fine-to-coarse average after each step: (coarse patch, coarse local,
+ * fine patch, fine local of the lower-left fine cell), ratio^2 cells. */
+int enl_batch_set_coarsen(void *handle, int n, const int *cp, const int *cl, const int *fp, const int *fl);
+/* Tile shape of the persistent stage kernel: 32x8, 16x16, 16x8 or 8x8. */
+int enl_batch_set_tiles(void *handle, int bx, int by);
+/* Device pointers of a patch's current state and ghosted source. */
+int enl_batch_patch_state(void *handle, int pid, double **w_n, double **pres, double **temp, double **source);
+/* Copies a patch's current state (any argument may be NULL) between host
+ * arrays of its ghosted size and the device: to_device = 1 uploads. */
+int enl_batch_copy_state(void *handle, int pid, int to_device, double *w_n, double *pres, double *temp,
+                         double *source);
+/* Fills the source ghosts (constant over a run) and its interior copy. */
+int enl_batch_prepare(void *handle, const int *reflect);
+/* Ghost fill of the current state of every level. */
+int enl_batch_fill(void *handle, const int *reflect);
+/* pres, temp of every patch interior from w_n (update_state.m4). */
+int enl_batch_update_state(void *handle, const enl_gas_t *gas);
+/* nsteps method-of-lines steps of the whole hierarchy (one dt for every
+ * level, no subcycling, enlightning.C:446-447).  mode: ENL_BATCH_PLAIN = the
+ * drop-in entry points patch by patch; ENL_BATCH_FUSED = batched fused
+ * stages; ENL_BATCH_GRAPH = the same captured once into a hipGraph and
+ * replayed.  All three give bit-identical results. */
+int enl_batch_advance(void *handle, int mode, int variant, const enl_gas_t *gas, const int *reflect, int order,
+                      const double *alpha_1, const double *alpha_2, const double *beta, double dt, int nsteps);
+int enl_batch_destroy(void *handle);
+
 /* ---- measurement hooks (no reference counterpart) ---------------------- */
 
 /* Kernel classes timed by the patch-loop entry points. */

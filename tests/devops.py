@@ -78,3 +78,28 @@ else:
 
     def use_stream():
         pass
+
+
+# ---- hierarchies ----------------------------------------------------------
+if not DRY:
+    from enlightning_amd.hierarchy import DeviceHierarchy as Hierarchy
+else:
+    from oracle.amr import HostHierarchy as _HostHierarchy
+
+    class Hierarchy(_HostHierarchy):
+        """The CPU twin behind DeviceHierarchy's interface."""
+
+        def load_initial(self, wl, states=None):
+            from enlightning_amd.hierarchy import patch_initial_state
+            if states is None:
+                states = [patch_initial_state(self.h, p, wl, self.gas) for p in self.h.patches]
+            super().load_initial(states)
+
+        def advance(self, nsteps, dt, mode="graph", variant="hybrid_high_llf", rk=RK3):
+            super().advance(nsteps, dt, rk, variant=variant)
+
+        def set_tiles(self, bx, by):
+            pass
+
+        def close(self):
+            pass
