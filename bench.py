@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--n", type=int, default=4096, help="interior cells per side")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-amr", action="store_true")
+    ap.add_argument("--amr-steps", type=int, default=40)
+    ap.add_argument("--amr-plain-steps", type=int, default=4)
     return ap.parse_args()
 
 
@@ -91,6 +94,72 @@ def cpu_baseline(wl, gas, steps):
         "sample": "%d RK3 step(s) of %s from its initial state: the C restatement of the reference kernels "
                   "(oracle/enl_oracle.c, gcc -O2, no FMA), 64x64 tiles (the largest level-0 patch of "
                   "input.txt:114-116) over %d OpenMP threads; %.2f s wall" % (steps, wl.name, threads, dt),
+    }
+
+
+def amr_bench(args):
+    """The batched two-level hierarchy (base 1024^2 in 64^2 patches, one
+    level refined by 4 over tag_cells' tags of the initial state): the
+    drop-in entry points patch by patch (plain) against the batched fused
+    stages replayed from a hipGraph, with the stage kernel's tile shape
+    picked by timing each candidate."""
+    import numpy as np
+    import torch
+    from enlightning_amd import amr
+    from enlightning_amd import patch as P
+    from enlightning_amd.gas import Gas
+    from enlightning_amd.hierarchy import DeviceHierarchy, level_tags
+    from enlightning_amd.workloads import AMR_1024
+
+    gas = Gas()
+    wl = AMR_1024
+    tags = level_tags(gas, wl, P.tag_cells)  # HIP tag_cells_ on the base grid
+    h = amr.build(wl.n, 4, 4, 1.0, tags, tile0=64)
+    dt = 0.5 * h.dx(h.nlevels - 1) / gas.c  # one dt for all levels (enlightning.C:446-447)
+    cells = h.interior_cells()
+    d = DeviceHierarchy(h, gas)
+
+    def run(mode, steps):
+        d.load_initial(wl)
+        d.advance(1, dt, mode=mode)  # warmup (and graph capture)
+        d.load_initial(wl)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        d.advance(steps, dt, mode=mode)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    tune = {}
+    for bx, by in ((32, 8), (16, 16), (16, 8), (8, 8)):
+        d.set_tiles(bx, by)
+        tune["%dx%d" % (bx, by)] = run("graph", 3) / 3 * 1e3
+    best = min(tune, key=tune.get)
+    d.set_tiles(*map(int, best.split("x")))
+    t_graph = run("graph", args.amr_steps)
+    t_plain = run("plain", args.amr_plain_steps)
+    plain_states = d.states()
+    d.load_initial(wl)
+    d.advance(args.amr_plain_steps, dt, mode="graph")
+    graph_states = d.states()
+    same = all(np.array_equal(a[k][..., p.g:-p.g, p.g:-p.g], b[k][..., p.g:-p.g, p.g:-p.g])
+               for a, b, p in zip(plain_states, graph_states, h.patches) for k in range(3))
+    d.close()
+    sizes = [max(p.nx, p.ny) for p in h.level_patches(1)]
+    return {
+        "workload": wl.name,
+        "patches_per_level": [len(h.level_patches(l)) for l in range(h.nlevels)],
+        "cells_per_level": [h.interior_cells(l) for l in range(h.nlevels)],
+        "fine_patch_side_min_max": [int(min(sizes)), int(max(sizes))] if sizes else None,
+        "tiles_ms_per_step": tune,
+        "tiles": best,
+        "graph": {"value": cells * args.amr_steps / t_graph, "ms_per_step": t_graph * 1e3 / args.amr_steps,
+                  "steps": args.amr_steps},
+        "plain_dropin_per_patch": {"value": cells * args.amr_plain_steps / t_plain,
+                                   "ms_per_step": t_plain * 1e3 / args.amr_plain_steps,
+                                   "steps": args.amr_plain_steps},
+        "speedup_graph_vs_plain": (t_plain / args.amr_plain_steps) / (t_graph / args.amr_steps),
+        "graph_bit_identical_to_plain": same,
+        "unit": "cell-updates/s",
     }
 
 
@@ -203,11 +272,14 @@ def main():
         "state_finite": finite,
         "cpu_baseline": None,
     }
-    if not args.no_cpu_baseline:
-        del plain, fused
+    del plain, fused
+    torch.cuda.empty_cache()
+    if not args.no_amr:
+        out["amr"] = amr_bench(args)
         torch.cuda.empty_cache()
+    if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, gas, args.cpu_steps)
-    print(json.dumps(out), flush=True)
+    print(json.dumps(out, default=lambda o: o.item()), flush=True)
 
 
 if __name__ == "__main__":

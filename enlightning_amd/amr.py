@@ -85,7 +85,7 @@ class Hierarchy:
         return [p for p in self.patches if p.level == level]
 
     def interior_cells(self, level=None):
-        return sum(p.nx * p.ny for p in self.patches if level is None or p.level == level)
+        return int(sum(p.nx * p.ny for p in self.patches if level is None or p.level == level))
 
 
 def base_boxes(n0, tile0):

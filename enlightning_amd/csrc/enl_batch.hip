@@ -40,6 +40,8 @@ namespace enl {
 namespace {
 
 // One patch in device memory.  Offsets are in doubles from the pool base.
+constexpr int kMaxStages = 8;
+
 struct PatchDev {
   int nx, ny, g, level;
   long sy, sk;      // ghosted row / component strides
@@ -50,47 +52,11 @@ struct PatchDev {
   double dx, dy, ylo;
 };
 
-struct BatchArgs {
-  double* pool;
-  const PatchDev* patches;
-  const int4* tiles;  // (patch, i0, j0, -) of the level being stepped
-  int ntiles;
-  int in, out, ww;  // state blocks read, written, and holding the step's start state
-  RhsArgs base;     // uniform part of the arguments (gas constants, RK coefficients, list)
-};
-
-__device__ inline void patch_args(const BatchArgs& b, int pid, RhsArgs& a) {
-  const PatchDev& pd = b.patches[pid];
-  a = b.base;
-  a.geom.nx = pd.nx;
-  a.geom.ny = pd.ny;
-  a.geom.g0 = pd.g;
-  a.geom.g1 = pd.g;
-  a.geom.sy = pd.sy;
-  a.geom.sk = pd.sk;
-  const double* in = b.pool + pd.state[b.in];
-  a.w = in;
-  a.p = in + 5 * pd.sk;
-  a.T = in + 6 * pd.sk;
-  a.H0 = b.pool + pd.src + pd.g + pd.g * pd.sy;
-  a.h0_sy = pd.sy;
-  a.KK = b.pool + pd.rhs;
-  a.dx = pd.dx;
-  a.dy = pd.dy;
-  a.ylo = pd.ylo;
-  a.level = pd.level;
-  fill_rhs_constants(a);
-  a.ww = b.pool + pd.state[b.ww];
-  a.ww_off = pd.g + pd.g * pd.sy;
-  a.ww_sy = pd.sy;
-  a.ww_sk = pd.sk;
-  double* out = b.pool + pd.state[b.out];
-  a.w_out = out;
-  a.p_out = out + 5 * pd.sk;
-  a.T_out = out + 6 * pd.sk;
-}
-
-#define ENL_BATCH_LDS(BX, BY, L)                                             \
+// Per stage and patch the kernels read a prebuilt argument block (pointers
+// into the pool, grid reciprocals, gas constants, RK coefficients) through a
+// restrict pointer: uniform scalar loads, as for the single-patch kernels'
+// kernel-argument block, instead of per-tile arithmetic held in registers.
+#define ENL_BATCH_LDS(BX, BY, L, raw)                                        \
   using ShB = TileShape<BX, BY>;                                             \
   __shared__ double lds_w0[ShB::TN], lds_p[ShB::TN], lds_T[ShB::TN];         \
   __shared__ double lds_u[ShB::TN], lds_v[ShB::TN], lds_s[ShB::TN], lds_tn[ShB::TN]; \
@@ -102,31 +68,37 @@ __device__ inline void patch_args(const BatchArgs& b, int pid, RhsArgs& a) {
   L.v = lds_v;                                                               \
   L.s = lds_s;                                                               \
   L.tn = lds_tn;                                                             \
-  L.w[1] = L.w[2] = L.w[3] = L.w[4] = nullptr;
+  L.w[1] = L.w[2] = L.w[3] = L.w[4] = nullptr;                               \
+  if (raw) {                                                                 \
+    __shared__ double lds_raw[4][ShB::TN];                                   \
+    for (int k_ = 1; k_ < 5; k_++) L.w[k_] = lds_raw[k_ - 1];                \
+  }
 
-// Persistent over the tile list of all patches.
+// One tile per workgroup: sensor + DRP cells finish here, WENO cells go to
+// the list (split_body).  The fused epilogue is capped at 128 VGPRs.
 template <int V, bool P, int BX, int BY>
 __global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(4)))
-batch_split_kernel(BatchArgs b) {
-  ENL_BATCH_LDS(BX, BY, L)
-  for (int t = blockIdx.x; t < b.ntiles; t += gridDim.x) {
-    const int4 tl = b.tiles[t];
-    RhsArgs a;
-    patch_args(b, tl.x, a);
-    split_body<V, P, true, BX, BY>(a, tl.y, tl.z, (unsigned)tl.x, L);
-    __syncthreads();  // the next tile reuses the LDS
-  }
+batch_split_kernel(const int4* __restrict__ tiles, int ntiles, const RhsArgs* __restrict__ args) {
+  ENL_BATCH_LDS(BX, BY, L, false)
+  const int4 tl = tiles[xcd_tile((int)blockIdx.x, ntiles)];
+  split_body<V, P, true, BX, BY>(args[tl.x], tl.y, tl.z, (unsigned)tl.x, L);
 }
 
+// Every cell of the tile WENO (levels above 0, rhs_weno_* variants).
+template <int V, bool P, int BX, int BY>
+__global__ void __launch_bounds__(BX* BY) batch_dense_kernel(const int4* __restrict__ tiles, int ntiles,
+                                                              const RhsArgs* __restrict__ args) {
+  ENL_BATCH_LDS(BX, BY, L, true)
+  const int4 tl = tiles[xcd_tile((int)blockIdx.x, ntiles)];
+  dense_body<V, P, true, BX, BY>(args[tl.x], tl.y, tl.z, L);
+}
+
+// The listed WENO cells of the level, ten lanes each (list_block).
 template <int V, bool P>
-__global__ void __launch_bounds__(256) batch_list_kernel(BatchArgs b) {
-  const unsigned n = *b.base.count;
-  for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-    const unsigned long long e = b.base.list[q];
-    RhsArgs a;
-    patch_args(b, (int)(e >> 32), a);
-    list_cell<V, P, true>(a, (int)(e & 0xffffffffu));
-  }
+__global__ void __launch_bounds__(256) batch_list_kernel(const RhsArgs* __restrict__ args,
+                                                         const unsigned long long* __restrict__ list,
+                                                         const unsigned* __restrict__ count) {
+  list_block<V, P, true>([=](int pid) -> const RhsArgs& { return args[pid]; }, list, *count);
 }
 
 __global__ void batch_reset_kernel(unsigned* count) { *count = 0; }
@@ -264,6 +236,9 @@ struct Batch {
   unsigned long long* list = nullptr;
   unsigned* count = nullptr;
   long cells = 0;
+  long level_cells[8] = {0};
+  RhsArgs* d_args = nullptr;  // kMaxStages x npatch argument blocks
+  std::vector<RhsArgs> h_args;
   std::vector<void*> allocations;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -330,95 +305,144 @@ void fill_all(Batch* b, int buf, const int* reflect, hipStream_t s) {
   for (int lev = 0; lev < b->nlevels; lev++) fill_level(b, lev, buf, buf, 7, reflect, s);
 }
 
-RhsArgs base_args(Batch* b, const enl_gas_t* gas) {
-  RhsArgs a;
-  memset(&a, 0, sizeof a);
-  a.alpha = gas->weno_alpha;
-  a.kappa = gas->kappa;
-  a.mu = gas->mu;
-  a.muB = gas->muB;
-  a.R = gas->R;
-  a.T_star_n = gas->T_star_n;
-  a.tau_n = gas->tau_n;
-  a.c = gas->c;
-  a.c_lr = gas->c_lr;
-  a.ep = sensor_eps(gas->tag_tolerance[2]);
-  a.rho0 = gas->rho0;
-  a.gamma1 = gas->gamma1;
-  a.c_p = gas->c_p;
-  a.p0 = gas->p0;
-  a.T0 = gas->T0;
-  a.c_v = gas->c_v;
-  a.list = b->list;
-  a.count = b->count;
-  a.switch_map = nullptr;
-  return a;
+// State blocks of RK stage st of an order-stage step: stage 0 reads block 0
+// (the step's start state, also ww), stages alternate blocks 1 and 2, the
+// last stage writes block 0.
+int stage_out(int st, int order) { return (st == order - 1 && order > 1) ? 0 : (st % 2 == 0 ? 1 : 2); }
+int stage_in(int st, int order) { return st == 0 ? 0 : stage_out(st - 1, order); }
+
+// Argument blocks of every (stage, patch), built on the host (the same
+// fill_rhs_constants as the single-patch launch) and uploaded when they
+// change; the kernels of stage st read b->d_args + st*npatch.
+int prepare_args(Batch* b, const enl_gas_t* gas, int order, const double* a1, const double* a2, const double* be,
+                 double dt, hipStream_t s) {
+  RhsArgs base;
+  memset(&base, 0, sizeof base);
+  base.alpha = gas->weno_alpha;
+  base.kappa = gas->kappa;
+  base.mu = gas->mu;
+  base.muB = gas->muB;
+  base.R = gas->R;
+  base.T_star_n = gas->T_star_n;
+  base.tau_n = gas->tau_n;
+  base.c = gas->c;
+  base.c_lr = gas->c_lr;
+  base.ep = sensor_eps(gas->tag_tolerance[2]);
+  base.rho0 = gas->rho0;
+  base.gamma1 = gas->gamma1;
+  base.c_p = gas->c_p;
+  base.p0 = gas->p0;
+  base.T0 = gas->T0;
+  base.c_v = gas->c_v;
+  base.list = b->list;
+  base.count = b->count;
+  base.switch_map = nullptr;
+  std::vector<RhsArgs> t((size_t)order * b->npatch, base);
+  for (int st = 0; st < order; st++) {
+    for (int p = 0; p < b->npatch; p++) {
+      const PatchDev& pd = b->hp[p];
+      RhsArgs& a = t[(size_t)st * b->npatch + p];
+      a.alpha_1 = a1[st];
+      a.alpha_2 = a2[st];
+      a.bdt = be[st] * dt;
+      a.geom.nx = pd.nx;
+      a.geom.ny = pd.ny;
+      a.geom.g0 = pd.g;
+      a.geom.g1 = pd.g;
+      a.geom.sy = pd.sy;
+      a.geom.sk = pd.sk;
+      const double* in = b->pool + pd.state[stage_in(st, order)];
+      a.w = in;
+      a.p = in + 5 * pd.sk;
+      a.T = in + 6 * pd.sk;
+      a.H0 = b->pool + pd.src + pd.g + pd.g * pd.sy;
+      a.h0_sy = pd.sy;
+      a.KK = b->pool + pd.rhs;
+      a.dx = pd.dx;
+      a.dy = pd.dy;
+      a.ylo = pd.ylo;
+      a.level = pd.level;
+      fill_rhs_constants(a);
+      a.ww = b->pool + pd.state[0];
+      a.ww_off = pd.g + pd.g * pd.sy;
+      a.ww_sy = pd.sy;
+      a.ww_sk = pd.sk;
+      double* out = b->pool + pd.state[stage_out(st, order)];
+      a.w_out = out;
+      a.p_out = out + 5 * pd.sk;
+      a.T_out = out + 6 * pd.sk;
+    }
+  }
+  const size_t bytes = sizeof(RhsArgs) * t.size();
+  if (b->h_args.size() == t.size() && memcmp(b->h_args.data(), t.data(), bytes) == 0) return 0;
+  b->h_args.swap(t);
+  hipError_t e = hipMemcpyAsync(b->d_args, b->h_args.data(), bytes, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return check_hip(e, "batch argument upload");
+  return check_hip(hipStreamSynchronize(s), "batch argument upload sync");
 }
 
 template <int V, bool P, int BX, int BY>
-void stage_launch(Batch* b, const BatchArgs& ba, hipStream_t s) {
-  if (ba.ntiles <= 0) return;
-  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1), 0, s, b->count);
-  unsigned blocks = (unsigned)(ba.ntiles < 256 * 16 ? ba.ntiles : 256 * 16);
-  hipLaunchKernelGGL((batch_split_kernel<V, P, BX, BY>), dim3(blocks), dim3(BX, BY), 0, s, ba);
-  hipLaunchKernelGGL((batch_list_kernel<V, P>), dim3(grid_for(b->cells, 256, 2048)), dim3(256), 0, s, ba);
+void stage_launch(Batch* b, const RhsArgs* args, int lev, hipStream_t s) {
+  using Sc = Scheme<V>;
+  const int t0 = b->level_tile_begin[lev];
+  const int nt = b->level_tile_begin[lev + 1] - t0;
+  if (nt <= 0) return;
+  const int4* tiles = b->tiles + t0;
+  if (!Sc::drp || (Sc::hybrid && lev != 0)) {
+    // rhs_weno_*, and hybrid variants above level 0 (the sensor is off)
+    hipLaunchKernelGGL((batch_dense_kernel<V, P, BX, BY>), dim3(nt), dim3(BX, BY), 0, s, tiles, nt, args);
+    return;
+  }
+  if (Sc::hybrid) hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1), 0, s, b->count);
+  hipLaunchKernelGGL((batch_split_kernel<V, P, BX, BY>), dim3(nt), dim3(BX, BY), 0, s, tiles, nt, args);
+  if (Sc::hybrid)
+    hipLaunchKernelGGL((batch_list_kernel<V, P>), dim3(grid_for(b->level_cells[lev], kListCellsPerBlock, 4096)), dim3(256), 0, s,
+                       args, (const unsigned long long*)b->list, (const unsigned*)b->count);
 }
 
 template <int V, bool P>
-int stage_shape(Batch* b, const BatchArgs& ba, hipStream_t s) {
-  if (b->tile_bx == 32 && b->tile_by == 8) stage_launch<V, P, 32, 8>(b, ba, s);
-  else if (b->tile_bx == 16 && b->tile_by == 16) stage_launch<V, P, 16, 16>(b, ba, s);
-  else if (b->tile_bx == 16 && b->tile_by == 8) stage_launch<V, P, 16, 8>(b, ba, s);
-  else if (b->tile_bx == 8 && b->tile_by == 8) stage_launch<V, P, 8, 8>(b, ba, s);
+int stage_shape(Batch* b, const RhsArgs* args, int lev, hipStream_t s) {
+  if (b->tile_bx == 32 && b->tile_by == 8) stage_launch<V, P, 32, 8>(b, args, lev, s);
+  else if (b->tile_bx == 16 && b->tile_by == 16) stage_launch<V, P, 16, 16>(b, args, lev, s);
+  else if (b->tile_bx == 16 && b->tile_by == 8) stage_launch<V, P, 16, 8>(b, args, lev, s);
+  else if (b->tile_bx == 8 && b->tile_by == 8) stage_launch<V, P, 8, 8>(b, args, lev, s);
   else return set_error(-2, "unsupported tile shape");
   return 0;
 }
 
 template <bool P>
-int stage_variant(Batch* b, int variant, const BatchArgs& ba, hipStream_t s) {
+int stage_variant(Batch* b, int variant, const RhsArgs* args, int lev, hipStream_t s) {
   switch (variant) {
-    case kHybridHighLlf: return stage_shape<kHybridHighLlf, P>(b, ba, s);
-    case kHybridHigh: return stage_shape<kHybridHigh, P>(b, ba, s);
-    case kHybridLowLlf: return stage_shape<kHybridLowLlf, P>(b, ba, s);
-    case kHybridLow: return stage_shape<kHybridLow, P>(b, ba, s);
-    case kWenoHighLlf: return stage_shape<kWenoHighLlf, P>(b, ba, s);
-    case kWenoHigh: return stage_shape<kWenoHigh, P>(b, ba, s);
-    case kWenoLowLlf: return stage_shape<kWenoLowLlf, P>(b, ba, s);
-    case kWenoLow: return stage_shape<kWenoLow, P>(b, ba, s);
-    case kDrp: return stage_shape<kDrp, P>(b, ba, s);
+    case kHybridHighLlf: return stage_shape<kHybridHighLlf, P>(b, args, lev, s);
+    case kHybridHigh: return stage_shape<kHybridHigh, P>(b, args, lev, s);
+    case kHybridLowLlf: return stage_shape<kHybridLowLlf, P>(b, args, lev, s);
+    case kHybridLow: return stage_shape<kHybridLow, P>(b, args, lev, s);
+    case kWenoHighLlf: return stage_shape<kWenoHighLlf, P>(b, args, lev, s);
+    case kWenoHigh: return stage_shape<kWenoHigh, P>(b, args, lev, s);
+    case kWenoLowLlf: return stage_shape<kWenoLowLlf, P>(b, args, lev, s);
+    case kWenoLow: return stage_shape<kWenoLow, P>(b, args, lev, s);
+    case kDrp: return stage_shape<kDrp, P>(b, args, lev, s);
     default: return set_error(-1, "unknown rhs variant");
   }
 }
 
-// Enqueues nsteps batched steps on stream s.  Per stage and level: fill the
-// level's ghosts on the stage input (its coarse-to-fine refine reading the
-// coarser level's stage output), step the level's tiles, and refresh the
-// level's ghosts on its output for the next finer level's refine.
-int enqueue_batched(Batch* b, int variant, const enl_gas_t* gas, const int* reflect, int order, const double* a1,
-                    const double* a2, const double* be, double dt, int nsteps, hipStream_t s) {
-  BatchArgs ba;
-  ba.pool = b->pool;
-  ba.patches = b->dp;
-  ba.base = base_args(b, gas);
-  ba.ww = 0;
+// Enqueues nsteps batched steps on stream s (argument blocks already
+// prepared).  Per stage and level: fill the level's ghosts on the stage
+// input (its coarse-to-fine refine reading the coarser level's stage
+// output), step the level's tiles, and refresh the level's ghosts on its
+// output for the next finer level's refine.
+int enqueue_batched(Batch* b, int variant, const int* reflect, int order, int nsteps, hipStream_t s) {
   for (int step = 0; step < nsteps; step++) {
-    int in = 0;
     for (int st = 0; st < order; st++) {
-      int out = (st == order - 1 && order > 1) ? 0 : (st % 2 == 0 ? 1 : 2);
-      ba.in = in;
-      ba.out = out;
-      ba.base.alpha_1 = a1[st];
-      ba.base.alpha_2 = a2[st];
-      ba.base.bdt = be[st] * dt;
+      const int in = stage_in(st, order), out = stage_out(st, order);
+      const RhsArgs* args = b->d_args + (size_t)st * b->npatch;
       for (int lev = 0; lev < b->nlevels; lev++) {
         fill_level(b, lev, in, out, 7, reflect, s);
-        ba.tiles = b->tiles + b->level_tile_begin[lev];
-        ba.ntiles = b->level_tile_begin[lev + 1] - b->level_tile_begin[lev];
-        int rc = b->all_pow2 ? stage_variant<true>(b, variant, ba, s) : stage_variant<false>(b, variant, ba, s);
+        int rc = b->all_pow2 ? stage_variant<true>(b, variant, args, lev, s)
+                             : stage_variant<false>(b, variant, args, lev, s);
         if (rc) return rc;
         if (lev + 1 < b->nlevels) fill_level(b, lev, out, out, 7, reflect, s);
       }
-      in = out;
     }
     if (order == 1) {
       for (int p = 0; p < b->npatch; p++) {
@@ -534,6 +558,7 @@ int enl_batch_create(int npatch, const enl_patch_desc_t* patches, int gcw, int r
     b->all_pow2 = b->all_pow2 && pow2(pd.dx) && pow2(pd.dy);
     b->hp.push_back(pd);
     b->cells += (long)pd.nx * pd.ny;
+    b->level_cells[d.level] += (long)pd.nx * pd.ny;
     if (d.level + 1 > b->nlevels) b->nlevels = d.level + 1;
   }
   b->pool_doubles = (size_t)off;
@@ -550,6 +575,8 @@ int enl_batch_create(int npatch, const enl_patch_desc_t* patches, int gcw, int r
     if (e == hipSuccess) b->allocations.push_back(b->list);
     if (e == hipSuccess) e = hipMalloc((void**)&b->count, sizeof(unsigned));
     if (e == hipSuccess) b->allocations.push_back(b->count);
+    if (e == hipSuccess) e = hipMalloc((void**)&b->d_args, sizeof(RhsArgs) * (size_t)kMaxStages * npatch);
+    if (e == hipSuccess) b->allocations.push_back(b->d_args);
     if (e != hipSuccess) rc = check_hip(e, "hipMalloc(batch list)");
   }
   if (!rc) rc = make_tiles(b, 32, 8);
@@ -717,12 +744,13 @@ int enl_batch_advance(void* handle, int mode, int variant, const enl_gas_t* gas,
   Batch* b = (Batch*)handle;
   if (!b) return set_error(-2, "null batch");
   if (variant < 0 || variant >= kNumVariants) return set_error(-1, "unknown rhs variant");
-  if (order < 1 || order > 8) return set_error(-2, "RK order out of range");
+  if (order < 1 || order > kMaxStages) return set_error(-2, "RK order out of range");
   hipStream_t s = lib_stream();
   if (mode == ENL_BATCH_PLAIN) return enqueue_plain(b, variant, gas, reflect, order, alpha_1, alpha_2, beta, dt, nsteps, s);
-  if (mode == ENL_BATCH_FUSED)
-    return enqueue_batched(b, variant, gas, reflect, order, alpha_1, alpha_2, beta, dt, nsteps, s);
-  if (mode != ENL_BATCH_GRAPH) return set_error(-2, "unknown batch mode");
+  if (mode != ENL_BATCH_FUSED && mode != ENL_BATCH_GRAPH) return set_error(-2, "unknown batch mode");
+  int prc = prepare_args(b, gas, order, alpha_1, alpha_2, beta, dt, s);
+  if (prc) return prc;
+  if (mode == ENL_BATCH_FUSED) return enqueue_batched(b, variant, reflect, order, nsteps, s);
   // capture one step into a hipGraph (re-captured when the step changes) and
   // replay it nsteps times
   std::vector<unsigned char> key;
@@ -756,7 +784,7 @@ int enl_batch_advance(void* handle, int mode, int variant, const enl_gas_t* gas,
       (void)hipStreamDestroy(cs);
       return check_hip(e, "hipStreamBeginCapture");
     }
-    int rc = enqueue_batched(b, variant, gas, reflect, order, alpha_1, alpha_2, beta, dt, 1, cs);
+    int rc = enqueue_batched(b, variant, reflect, order, 1, cs);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);

@@ -57,10 +57,19 @@ using Sh = TileShape<kBX, kBY>;
     for (int k_ = 1; k_ < 5; k_++) L.w[k_] = lds_raw[k_ - 1];            \
   }
 
+// Tile origin of this workgroup: a 1-D grid of nbx * nby tiles, in XCD
+// chunks, row-major within a chunk.
+#define ENL_TILE_ORIGIN(a, i0, j0)                                      \
+  const int nbx_ = (a.geom.nx + kBX - 1) / kBX;                         \
+  const int nby_ = (a.geom.ny + kBY - 1) / kBY;                         \
+  const int t_ = xcd_tile((int)blockIdx.x, nbx_ * nby_);                \
+  const int i0 = (t_ % nbx_) * kBX, j0 = (t_ / nbx_) * kBY;
+
 template <int V, bool P, bool FUSED>
 __global__ void __launch_bounds__(Sh::threads) rhs_split_kernel(RhsArgs a) {
   ENL_TILE_LDS(L, false)
-  split_body<V, P, FUSED, kBX, kBY>(a, blockIdx.x * kBX, blockIdx.y * kBY, 0, L);
+  ENL_TILE_ORIGIN(a, i0, j0)
+  split_body<V, P, FUSED, kBX, kBY>(a, i0, j0, 0, L);
 }
 // The fused form carries the runge_kutta and update_state epilogue; capping
 // it at 128 VGPRs keeps 4 waves per SIMD.
@@ -68,21 +77,21 @@ template <int V, bool P, bool FUSED>
 __global__ void __launch_bounds__(Sh::threads) __attribute__((amdgpu_waves_per_eu(4)))
 rhs_split_fused_kernel(RhsArgs a) {
   ENL_TILE_LDS(L, false)
-  split_body<V, P, FUSED, kBX, kBY>(a, blockIdx.x * kBX, blockIdx.y * kBY, 0, L);
+  ENL_TILE_ORIGIN(a, i0, j0)
+  split_body<V, P, FUSED, kBX, kBY>(a, i0, j0, 0, L);
 }
 
-// WENO cells of the list, one lane each.
+// WENO cells of the list, ten lanes each (list_block).
 template <int V, bool P, bool FUSED>
 __global__ void __launch_bounds__(256) rhs_weno_list_kernel(RhsArgs a) {
-  const unsigned n = *a.count;
-  for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
-    list_cell<V, P, FUSED>(a, (int)(a.list[q] & 0xffffffffu));
+  list_block<V, P, FUSED>([&](int) -> const RhsArgs& { return a; }, a.list, *a.count);
 }
 
 template <int V, bool P, bool FUSED>
 __global__ void __launch_bounds__(Sh::threads) rhs_dense_kernel(RhsArgs a) {
   ENL_TILE_LDS(L, true)
-  dense_body<V, P, FUSED, kBX, kBY>(a, blockIdx.x * kBX, blockIdx.y * kBY, L);
+  ENL_TILE_ORIGIN(a, i0, j0)
+  dense_body<V, P, FUSED, kBX, kBY>(a, i0, j0, L);
 }
 
 __global__ void reset_count_kernel(unsigned* count) { *count = 0; }
@@ -98,7 +107,7 @@ hipError_t launch_t(const RhsArgs& a, hipStream_t s) {
   using Sc = Scheme<V>;
   const Geom& G = a.geom;
   dim3 block(kBX, kBY);
-  dim3 grid((G.nx + kBX - 1) / kBX, (G.ny + kBY - 1) / kBY);
+  dim3 grid((unsigned)(((G.nx + kBX - 1) / kBX) * ((G.ny + kBY - 1) / kBY)));
   if constexpr (!Sc::drp) {
     // rhs_weno_*: every cell on the WENO path
     hipLaunchKernelGGL((rhs_dense_kernel<V, P, F>), grid, block, 0, s, a);
@@ -118,10 +127,10 @@ hipError_t launch_t(const RhsArgs& a, hipStream_t s) {
       hipLaunchKernelGGL((rhs_split_fused_kernel<V, P, F>), grid, block, 0, s, a);
     else
       hipLaunchKernelGGL((rhs_split_kernel<V, P, F>), grid, block, 0, s, a);
-    // a grid that covers the machine; lanes stride over the list
+    // a grid that covers the machine; blocks stride over the list
     long cells = (long)G.nx * G.ny;
-    long blocks = (cells + 255) / 256;
-    if (blocks > 2048) blocks = 2048;
+    long blocks = (cells + kListCellsPerBlock - 1) / kListCellsPerBlock;
+    if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL((rhs_weno_list_kernel<V, P, F>), dim3((unsigned)blocks), dim3(256), 0, s, a);
   }
   return hipGetLastError();

@@ -21,6 +21,16 @@ struct TileShape {
   static constexpr int threads = BX * BY;
 };
 
+// Workgroups of a 1-D grid are dispatched round-robin over the 8 XCDs
+// (workgroup b runs on XCD b % 8), each XCD with its own L2.  xcd_tile maps
+// the dispatch index b of n workgroups to a tile index such that each XCD
+// works through one contiguous range of tiles: neighbouring tiles, whose
+// halos overlap, then share an L2.  A bijection on [0, n).
+__device__ inline int xcd_tile(int b, int n) {
+  const int q = n >> 3, r = n & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 // LDS of one staged tile.  w[1..4] (the raw momenta etc.) are only staged by
 // the dense kernel; w[0] is rho.
 template <int BX, int BY>
@@ -475,6 +485,94 @@ __device__ inline void list_cell(const RhsArgs& a, int cell) {
 #pragma unroll
   for (int k = 0; k < 5; k++) K[k] = X[k] + Y[k] + a.KK[(long)cell + k * ik];  // the slot holds H(k)
   cell_output<FUSED>(a, i, j, K);
+}
+
+// One (direction, component) of a listed WENO cell: the quantity list_cell
+// forms as X[k] (dir 0) or Y[k] (dir 1), by the same operations.
+template <int V, bool P>
+__device__ inline double list_item(const RhsArgs& a, int cell, int dir, int k) {
+  using Sc = Scheme<V>;
+  constexpr int R = Sc::order == 5 ? 3 : 2;
+  const Geom& G = a.geom;
+  const int i = cell % G.nx, j = cell / G.nx;
+  const long g = (long)(i + G.g0) + (long)(j + G.g1) * G.sy;
+  const long step = dir == 0 ? 1 : G.sy;
+  const long kn = dir == 0 ? G.sk : 2 * G.sk;
+  double w0[7], un[7];
+#pragma unroll
+  for (int m = -R; m <= R; m++) {
+    long gm = g + m * step;
+    w0[m + 3] = a.w[gm];
+    un[m + 3] = a.w[gm + kn] / w0[m + 3];
+  }
+  double alpha = a.alpha;
+  if constexpr (Sc::llf) {
+    alpha = 0;
+#pragma unroll
+    for (int m = -R; m <= R; m++) {
+      double height = a.ylo + a.dy * ((double)(dir == 0 ? j : j + m) + 0.5);
+      double vv = fabs(un[m + 3]) + a.c + a.c_lr * height;
+      if (vv > alpha) alpha = vv;
+    }
+  }
+  return list_dir<V, P, false>(a, g, j, dir, k, w0, un, alpha);
+}
+
+// The WENO list, ten lanes per cell (two directions x five components) so a
+// short list still spreads over the machine: lanes store their term in LDS,
+// then one lane per cell forms rhs = X + Y + H in list_cell's order and
+// writes the cell.  Entries are (patch << 32) | (i + j*nx); args_of(patch)
+// gives the patch's arguments.  A wave serves its lanes patch by patch
+// (consecutive entries come from one tile: usually a single pass) so the
+// arguments stay uniform.  Must be called by all 256 lanes of the block.
+constexpr int kListCellsPerBlock = 25;
+
+template <int V, bool P, bool FUSED, class ArgsOf>
+__device__ inline void list_block(ArgsOf args_of, const unsigned long long* list, unsigned n) {
+  __shared__ double term[kListCellsPerBlock][10];
+  const int t = threadIdx.x;
+  for (unsigned c0 = blockIdx.x * kListCellsPerBlock; c0 < n; c0 += gridDim.x * kListCellsPerBlock) {
+    {
+      const int lc = t / 10, item = t % 10;
+      const unsigned q = c0 + lc;
+      bool todo = lc < kListCellsPerBlock && q < n;
+      const unsigned long long e = todo ? list[q] : 0ull;
+      const int pid = (int)(e >> 32);
+      for (;;) {
+        const unsigned long long m = __ballot(todo);
+        if (!m) break;
+        const int cur = __builtin_amdgcn_readfirstlane(__shfl(pid, __ffsll((long long)m) - 1));
+        if (todo && pid == cur) {
+          term[lc][item] = list_item<V, P>(args_of(cur), (int)(e & 0xffffffffu), item / 5, item % 5);
+          todo = false;
+        }
+      }
+    }
+    __syncthreads();
+    {
+      const unsigned q = c0 + t;
+      bool todo = t < kListCellsPerBlock && q < n;
+      const unsigned long long e = todo ? list[q] : 0ull;
+      const int pid = (int)(e >> 32);
+      for (;;) {
+        const unsigned long long m = __ballot(todo);
+        if (!m) break;
+        const int cur = __builtin_amdgcn_readfirstlane(__shfl(pid, __ffsll((long long)m) - 1));
+        if (todo && pid == cur) {
+          const RhsArgs& a = args_of(cur);
+          const int cell = (int)(e & 0xffffffffu);
+          const Geom& G = a.geom;
+          const long ik = (long)G.nx * G.ny;
+          double K[5];
+#pragma unroll
+          for (int k = 0; k < 5; k++) K[k] = term[t][k] + term[t][5 + k] + a.KK[(long)cell + k * ik];
+          cell_output<FUSED>(a, cell % G.nx, cell / G.nx, K);
+          todo = false;
+        }
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // Dense tile: every cell on the WENO path.
