@@ -98,47 +98,53 @@ def cpu_baseline(wl, gas, steps):
 
 
 def amr_bench(args):
-    """The batched two-level hierarchy (base 1024^2 in 64^2 patches, one
-    level refined by 4 over tag_cells' tags of the initial state): the
-    drop-in entry points patch by patch (plain) against the batched fused
-    stages replayed from a hipGraph, with the stage kernel's tile shape
-    picked by timing each candidate."""
+    """The batched two-level hierarchy: base 1024^2 in 64^2 patches, advanced
+    on the uniform grid until the pulses have spread, then one level refined
+    by 4 over tag_cells' tags.  The drop-in entry points patch by patch
+    (plain) against the batched fused stages replayed from a hipGraph, with
+    the stage kernels' tile shape picked by timing each candidate."""
     import numpy as np
     import torch
     from enlightning_amd import amr
     from enlightning_amd import patch as P
     from enlightning_amd.gas import Gas
-    from enlightning_amd.hierarchy import DeviceHierarchy, level_tags
-    from enlightning_amd.workloads import AMR_1024
+    from enlightning_amd.hierarchy import DeviceHierarchy, refine_from_base
+    from enlightning_amd.workloads import AMR_1024, initial_fields
 
     gas = Gas()
     wl = AMR_1024
-    tags = level_tags(gas, wl, P.tag_cells)  # HIP tag_cells_ on the base grid
-    h = amr.build(wl.n, 4, 4, 1.0, tags, tile0=64)
+    n, g = wl.n, wl.g
+    base = P.DevicePatch(wl, gas)
+    base.init_state(*initial_fields(wl, gas))
+    if wl.presteps:
+        base.advance_fused(wl.presteps, wl.dt(gas))
+    P.fill_physical_bc(base.box, g, (1, 1, 1, 1), base.reflect, base.w, base.p, base.T, base.src)
+    torch.cuda.synchronize()
+    wb = base.w.cpu().numpy()
+    del base
+    tags = np.zeros((n, n), np.int32)
+    P.tag_cells((0, n - 1, 0, n - 1), g, tags, wb, np.zeros((n + 2 * g, n + 2 * g)), 1, gas.tag_tolerance)
+    h = amr.build(n, 4, 4, 1.0, tags, tile0=64)
+    states = [refine_from_base(h, p, wb) for p in h.patches]
     dt = 0.5 * h.dx(h.nlevels - 1) / gas.c  # one dt for all levels (enlightning.C:446-447)
     cells = h.interior_cells()
     d = DeviceHierarchy(h, gas)
 
     def run(mode, steps):
-        d.load_initial(wl)
+        d.load_initial(wl, states)
         d.advance(1, dt, mode=mode)  # warmup (and graph capture)
-        d.load_initial(wl)
+        d.load_initial(wl, states)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         d.advance(steps, dt, mode=mode)
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
-    tune = {}
-    for bx, by in ((32, 8), (16, 16), (16, 8), (8, 8)):
-        d.set_tiles(bx, by)
-        tune["%dx%d" % (bx, by)] = run("graph", 3) / 3 * 1e3
-    best = min(tune, key=tune.get)
-    d.set_tiles(*map(int, best.split("x")))
+    best, tune = d.autotune(dt, states, wl, steps=3)
     t_graph = run("graph", args.amr_steps)
     t_plain = run("plain", args.amr_plain_steps)
     plain_states = d.states()
-    d.load_initial(wl)
+    d.load_initial(wl, states)
     d.advance(args.amr_plain_steps, dt, mode="graph")
     graph_states = d.states()
     same = all(np.array_equal(a[k][..., p.g:-p.g, p.g:-p.g], b[k][..., p.g:-p.g, p.g:-p.g])
@@ -147,11 +153,13 @@ def amr_bench(args):
     sizes = [max(p.nx, p.ny) for p in h.level_patches(1)]
     return {
         "workload": wl.name,
+        "base_steps_before_tagging": wl.presteps,
+        "tagged_base_cells": int(tags.sum()),
         "patches_per_level": [len(h.level_patches(l)) for l in range(h.nlevels)],
         "cells_per_level": [h.interior_cells(l) for l in range(h.nlevels)],
         "fine_patch_side_min_max": [int(min(sizes)), int(max(sizes))] if sizes else None,
-        "tiles_ms_per_step": tune,
-        "tiles": best,
+        "autotune_ms_per_step": tune,
+        "tiles_per_level": ["%dx%d" % best[l] for l in sorted(best)],
         "graph": {"value": cells * args.amr_steps / t_graph, "ms_per_step": t_graph * 1e3 / args.amr_steps,
                   "steps": args.amr_steps},
         "plain_dropin_per_patch": {"value": cells * args.amr_plain_steps / t_plain,

@@ -101,7 +101,12 @@ __global__ void __launch_bounds__(256) batch_list_kernel(const RhsArgs* __restri
   list_block<V, P, true>([=](int pid) -> const RhsArgs& { return args[pid]; }, list, *count);
 }
 
-__global__ void batch_reset_kernel(unsigned* count) { *count = 0; }
+// Zeroes the WENO-list counters of every (stage, level) of a step.
+__global__ void batch_reset_kernel(unsigned* count, int n) {
+  if ((int)threadIdx.x < n) count[threadIdx.x] = 0;
+}
+
+constexpr int kMaxLevels = 8;
 
 // ---- ghost fill and coarsen -------------------------------------------------
 
@@ -227,7 +232,8 @@ struct Batch {
   size_t pool_doubles = 0;
   PatchDev* dp = nullptr;
   int4* tiles = nullptr;
-  int ntiles = 0, tile_bx = 0, tile_by = 0;
+  int ntiles = 0;
+  int tile_bx[8], tile_by[8];     // tile shape per level
   int level_tile_begin[9] = {0};  // tiles sorted by level
   bool all_pow2 = true;
   LevelFill fill[8];
@@ -257,24 +263,23 @@ int upload(Batch* b, T** dst, const T* src, long n) {
   return check_hip(hipMemcpy(*dst, src, sizeof(T) * (size_t)n, hipMemcpyHostToDevice), "upload batch table");
 }
 
-int make_tiles(Batch* b, int bx, int by) {
+int make_tiles(Batch* b) {
   std::vector<int4> t;
-  for (int lev = 0; lev < 8; lev++) {
+  for (int lev = 0; lev < kMaxLevels; lev++) {
     b->level_tile_begin[lev] = (int)t.size();
+    const int bx = b->tile_bx[lev], by = b->tile_by[lev];
     for (int p = 0; p < b->npatch; p++)
       if (b->hp[p].level == lev)
         for (int j0 = 0; j0 < b->hp[p].ny; j0 += by)
           for (int i0 = 0; i0 < b->hp[p].nx; i0 += bx) t.push_back(make_int4(p, i0, j0, 0));
   }
-  b->level_tile_begin[8] = (int)t.size();
+  b->level_tile_begin[kMaxLevels] = (int)t.size();
   if (b->tiles) {
     (void)hipFree(b->tiles);
     for (auto& a : b->allocations)
       if (a == b->tiles) a = nullptr;
   }
   b->ntiles = (int)t.size();
-  b->tile_bx = bx;
-  b->tile_by = by;
   return upload(b, &b->tiles, t.data(), (long)t.size());
 }
 
@@ -335,7 +340,6 @@ int prepare_args(Batch* b, const enl_gas_t* gas, int order, const double* a1, co
   base.T0 = gas->T0;
   base.c_v = gas->c_v;
   base.list = b->list;
-  base.count = b->count;
   base.switch_map = nullptr;
   std::vector<RhsArgs> t((size_t)order * b->npatch, base);
   for (int st = 0; st < order; st++) {
@@ -362,6 +366,7 @@ int prepare_args(Batch* b, const enl_gas_t* gas, int order, const double* a1, co
       a.dy = pd.dy;
       a.ylo = pd.ylo;
       a.level = pd.level;
+      a.count = b->count + st * kMaxLevels + pd.level;
       fill_rhs_constants(a);
       a.ww = b->pool + pd.state[0];
       a.ww_off = pd.g + pd.g * pd.sy;
@@ -382,7 +387,7 @@ int prepare_args(Batch* b, const enl_gas_t* gas, int order, const double* a1, co
 }
 
 template <int V, bool P, int BX, int BY>
-void stage_launch(Batch* b, const RhsArgs* args, int lev, hipStream_t s) {
+void stage_launch(Batch* b, const RhsArgs* args, int st, int lev, hipStream_t s) {
   using Sc = Scheme<V>;
   const int t0 = b->level_tile_begin[lev];
   const int nt = b->level_tile_begin[lev + 1] - t0;
@@ -393,35 +398,36 @@ void stage_launch(Batch* b, const RhsArgs* args, int lev, hipStream_t s) {
     hipLaunchKernelGGL((batch_dense_kernel<V, P, BX, BY>), dim3(nt), dim3(BX, BY), 0, s, tiles, nt, args);
     return;
   }
-  if (Sc::hybrid) hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1), 0, s, b->count);
   hipLaunchKernelGGL((batch_split_kernel<V, P, BX, BY>), dim3(nt), dim3(BX, BY), 0, s, tiles, nt, args);
   if (Sc::hybrid)
     hipLaunchKernelGGL((batch_list_kernel<V, P>), dim3(grid_for(b->level_cells[lev], kListCellsPerBlock, 4096)), dim3(256), 0, s,
-                       args, (const unsigned long long*)b->list, (const unsigned*)b->count);
+                       args, (const unsigned long long*)b->list,
+                       (const unsigned*)(b->count + st * kMaxLevels + lev));
 }
 
 template <int V, bool P>
-int stage_shape(Batch* b, const RhsArgs* args, int lev, hipStream_t s) {
-  if (b->tile_bx == 32 && b->tile_by == 8) stage_launch<V, P, 32, 8>(b, args, lev, s);
-  else if (b->tile_bx == 16 && b->tile_by == 16) stage_launch<V, P, 16, 16>(b, args, lev, s);
-  else if (b->tile_bx == 16 && b->tile_by == 8) stage_launch<V, P, 16, 8>(b, args, lev, s);
-  else if (b->tile_bx == 8 && b->tile_by == 8) stage_launch<V, P, 8, 8>(b, args, lev, s);
+int stage_shape(Batch* b, const RhsArgs* args, int st, int lev, hipStream_t s) {
+  const int bx = b->tile_bx[lev], by = b->tile_by[lev];
+  if (bx == 32 && by == 8) stage_launch<V, P, 32, 8>(b, args, st, lev, s);
+  else if (bx == 16 && by == 16) stage_launch<V, P, 16, 16>(b, args, st, lev, s);
+  else if (bx == 16 && by == 8) stage_launch<V, P, 16, 8>(b, args, st, lev, s);
+  else if (bx == 8 && by == 8) stage_launch<V, P, 8, 8>(b, args, st, lev, s);
   else return set_error(-2, "unsupported tile shape");
   return 0;
 }
 
 template <bool P>
-int stage_variant(Batch* b, int variant, const RhsArgs* args, int lev, hipStream_t s) {
+int stage_variant(Batch* b, int variant, const RhsArgs* args, int st, int lev, hipStream_t s) {
   switch (variant) {
-    case kHybridHighLlf: return stage_shape<kHybridHighLlf, P>(b, args, lev, s);
-    case kHybridHigh: return stage_shape<kHybridHigh, P>(b, args, lev, s);
-    case kHybridLowLlf: return stage_shape<kHybridLowLlf, P>(b, args, lev, s);
-    case kHybridLow: return stage_shape<kHybridLow, P>(b, args, lev, s);
-    case kWenoHighLlf: return stage_shape<kWenoHighLlf, P>(b, args, lev, s);
-    case kWenoHigh: return stage_shape<kWenoHigh, P>(b, args, lev, s);
-    case kWenoLowLlf: return stage_shape<kWenoLowLlf, P>(b, args, lev, s);
-    case kWenoLow: return stage_shape<kWenoLow, P>(b, args, lev, s);
-    case kDrp: return stage_shape<kDrp, P>(b, args, lev, s);
+    case kHybridHighLlf: return stage_shape<kHybridHighLlf, P>(b, args, st, lev, s);
+    case kHybridHigh: return stage_shape<kHybridHigh, P>(b, args, st, lev, s);
+    case kHybridLowLlf: return stage_shape<kHybridLowLlf, P>(b, args, st, lev, s);
+    case kHybridLow: return stage_shape<kHybridLow, P>(b, args, st, lev, s);
+    case kWenoHighLlf: return stage_shape<kWenoHighLlf, P>(b, args, st, lev, s);
+    case kWenoHigh: return stage_shape<kWenoHigh, P>(b, args, st, lev, s);
+    case kWenoLowLlf: return stage_shape<kWenoLowLlf, P>(b, args, st, lev, s);
+    case kWenoLow: return stage_shape<kWenoLow, P>(b, args, st, lev, s);
+    case kDrp: return stage_shape<kDrp, P>(b, args, st, lev, s);
     default: return set_error(-1, "unknown rhs variant");
   }
 }
@@ -430,16 +436,19 @@ int stage_variant(Batch* b, int variant, const RhsArgs* args, int lev, hipStream
 // prepared).  Per stage and level: fill the level's ghosts on the stage
 // input (its coarse-to-fine refine reading the coarser level's stage
 // output), step the level's tiles, and refresh the level's ghosts on its
-// output for the next finer level's refine.
+// output for the next finer level's refine.  Level 0 has no coarse-to-fine
+// part, so after stage 0 its refresh is already the next stage's fill.
 int enqueue_batched(Batch* b, int variant, const int* reflect, int order, int nsteps, hipStream_t s) {
   for (int step = 0; step < nsteps; step++) {
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, b->count, kMaxStages * kMaxLevels);
     for (int st = 0; st < order; st++) {
       const int in = stage_in(st, order), out = stage_out(st, order);
       const RhsArgs* args = b->d_args + (size_t)st * b->npatch;
       for (int lev = 0; lev < b->nlevels; lev++) {
-        fill_level(b, lev, in, out, 7, reflect, s);
-        int rc = b->all_pow2 ? stage_variant<true>(b, variant, args, lev, s)
-                             : stage_variant<false>(b, variant, args, lev, s);
+        const bool refreshed = st > 0 && lev == 0 && b->nlevels > 1;
+        if (!refreshed) fill_level(b, lev, in, out, 7, reflect, s);
+        int rc = b->all_pow2 ? stage_variant<true>(b, variant, args, st, lev, s)
+                             : stage_variant<false>(b, variant, args, st, lev, s);
         if (rc) return rc;
         if (lev + 1 < b->nlevels) fill_level(b, lev, out, out, 7, reflect, s);
       }
@@ -573,13 +582,18 @@ int enl_batch_create(int npatch, const enl_patch_desc_t* patches, int gcw, int r
   if (!rc) {
     e = hipMalloc((void**)&b->list, sizeof(unsigned long long) * (size_t)b->cells);
     if (e == hipSuccess) b->allocations.push_back(b->list);
-    if (e == hipSuccess) e = hipMalloc((void**)&b->count, sizeof(unsigned));
+    if (e == hipSuccess) e = hipMalloc((void**)&b->count, sizeof(unsigned) * kMaxStages * kMaxLevels);
     if (e == hipSuccess) b->allocations.push_back(b->count);
+    if (e == hipSuccess) e = hipMemset(b->count, 0, sizeof(unsigned) * kMaxStages * kMaxLevels);
     if (e == hipSuccess) e = hipMalloc((void**)&b->d_args, sizeof(RhsArgs) * (size_t)kMaxStages * npatch);
     if (e == hipSuccess) b->allocations.push_back(b->d_args);
     if (e != hipSuccess) rc = check_hip(e, "hipMalloc(batch list)");
   }
-  if (!rc) rc = make_tiles(b, 32, 8);
+  for (int lev = 0; lev < kMaxLevels; lev++) {
+    b->tile_bx[lev] = 32;
+    b->tile_by[lev] = 8;
+  }
+  if (!rc) rc = make_tiles(b);
   if (rc) {
     for (void* a : b->allocations)
       if (a) (void)hipFree(a);
@@ -636,18 +650,41 @@ int enl_batch_set_coarsen(void* handle, int n, const int* cp, const int* cl, con
   return rc;
 }
 
-int enl_batch_set_tiles(void* handle, int bx, int by) {
-  Batch* b = (Batch*)handle;
-  if (!b) return set_error(-2, "null batch");
-  if (!((bx == 32 && by == 8) || (bx == 16 && by == 16) || (bx == 16 && by == 8) || (bx == 8 && by == 8)))
-    return set_error(-2, "unsupported tile shape %dx%d", bx, by);
+namespace {
+bool tile_ok(int bx, int by) {
+  return (bx == 32 && by == 8) || (bx == 16 && by == 16) || (bx == 16 && by == 8) || (bx == 8 && by == 8);
+}
+void drop_graph(Batch* b) {
   if (b->graph_exec) {
     (void)hipGraphExecDestroy(b->graph_exec);
     (void)hipGraphDestroy(b->graph);
     b->graph_exec = nullptr;
     b->graph = nullptr;
   }
-  return make_tiles(b, bx, by);
+}
+}  // namespace
+
+int enl_batch_set_tiles(void* handle, int bx, int by) {
+  Batch* b = (Batch*)handle;
+  if (!b) return set_error(-2, "null batch");
+  if (!tile_ok(bx, by)) return set_error(-2, "unsupported tile shape %dx%d", bx, by);
+  drop_graph(b);
+  for (int lev = 0; lev < kMaxLevels; lev++) {
+    b->tile_bx[lev] = bx;
+    b->tile_by[lev] = by;
+  }
+  return make_tiles(b);
+}
+
+int enl_batch_set_level_tiles(void* handle, int level, int bx, int by) {
+  Batch* b = (Batch*)handle;
+  if (!b) return set_error(-2, "null batch");
+  if (level < 0 || level >= kMaxLevels) return set_error(-2, "bad level %d", level);
+  if (!tile_ok(bx, by)) return set_error(-2, "unsupported tile shape %dx%d", bx, by);
+  drop_graph(b);
+  b->tile_bx[level] = bx;
+  b->tile_by[level] = by;
+  return make_tiles(b);
 }
 
 int enl_batch_patch_state(void* handle, int pid, double** w_n, double** pres, double** temp, double** source) {
@@ -766,8 +803,8 @@ int enl_batch_advance(void* handle, int mode, int variant, const enl_gas_t* gas,
   put(alpha_1, order * sizeof(double));
   put(alpha_2, order * sizeof(double));
   put(beta, order * sizeof(double));
-  put(&b->tile_bx, sizeof(int));
-  put(&b->tile_by, sizeof(int));
+  put(b->tile_bx, sizeof b->tile_bx);
+  put(b->tile_by, sizeof b->tile_by);
   bool same = b->graph_exec && key == b->graph_key;
   if (!same) {
     if (b->graph_exec) {

@@ -38,6 +38,34 @@ def patch_initial_state(h, patch, wl, gas):
     return np.ascontiguousarray(state_from_overpressure(pulses_at(wl, xc, yc), gas))
 
 
+def refine_from_base(h, patch, wb):
+    """w on a patch's ghosted cells from a ghosted base-grid state wb
+    (5, n0+2gb, n0+2gb): level-0 patches take their window of wb; level-1
+    patches the linear refinement of the ghost fill,
+        c + fx*(c[i+1]-c[i-1])/2 + fy*(c[j+1]-c[j-1])/2,
+    (fx, fy) the fine centre's offset from its coarse centre in coarse
+    widths.  Ghost values outside the domain are placeholders (the first
+    ghost fill replaces them)."""
+    n0 = h.n0
+    gb = (wb.shape[-1] - n0) // 2
+    i0, i1, j0, j1 = patch.box
+    g = patch.g
+    if patch.level == 0:
+        return np.ascontiguousarray(wb[:, j0 - g + gb:j1 + g + 1 + gb, i0 - g + gb:i1 + g + 1 + gb])
+    if patch.level != 1:
+        raise ValueError("refine_from_base covers levels 0 and 1")
+    r = h.ratio
+    I = np.arange(i0 - g, i1 + g + 1)
+    J = np.arange(j0 - g, j1 + g + 1)
+    ic = np.clip(np.floor_divide(I, r), -gb + 1, n0 + gb - 2)
+    jc = np.clip(np.floor_divide(J, r), -gb + 1, n0 + gb - 2)
+    fx = (I + 0.5) / r - (ic + 0.5)
+    fy = (J + 0.5) / r - (jc + 0.5)
+    C = lambda dj, di: wb[:, (jc + dj + gb)[:, None], (ic + di + gb)[None, :]]
+    out = C(0, 0) + fx[None, None, :] * (0.5 * (C(0, 1) - C(0, -1))) + fy[None, :, None] * (0.5 * (C(1, 0) - C(-1, 0)))
+    return np.ascontiguousarray(out)
+
+
 class DeviceHierarchy:
     def __init__(self, h, gas, reflect=(1, 1, 1, 1)):
         self.h, self.gas, self.reflect = h, gas, tuple(reflect)
@@ -74,8 +102,41 @@ class DeviceHierarchy:
     def _reflect(self):
         return (ctypes.c_int * 4)(*self.reflect)
 
-    def set_tiles(self, bx, by):
-        _lib.check(self.lib.enl_batch_set_tiles(self.handle, bx, by))
+    TILE_SHAPES = ((32, 8), (16, 16), (16, 8), (8, 8))
+
+    def set_tiles(self, bx, by, level=None):
+        if level is None:
+            _lib.check(self.lib.enl_batch_set_tiles(self.handle, bx, by))
+        else:
+            _lib.check(self.lib.enl_batch_set_level_tiles(self.handle, level, bx, by))
+
+    def autotune(self, dt, states, wl, steps=2, variant="hybrid_high_llf", rk=RK3, timer=None):
+        """Tile shape per level by coordinate descent over measured graph
+        steps: each level in turn tries every shape with the others fixed
+        and keeps the fastest.  Reloads `states` before each trial; returns
+        {level: (bx, by)} and the per-trial times (ms per step)."""
+        import time
+
+        def measure():
+            self.load_initial(wl, states)
+            self.advance(1, dt, mode="graph", variant=variant, rk=rk)  # capture
+            self.load_initial(wl, states)
+            self.sync()
+            t0 = time.perf_counter()
+            self.advance(steps, dt, mode="graph", variant=variant, rk=rk)
+            self.sync()
+            return (time.perf_counter() - t0) * 1e3 / steps
+
+        best, trials = {}, {}
+        for lev in range(self.h.nlevels):
+            times = {}
+            for shape in self.TILE_SHAPES:
+                self.set_tiles(*shape, level=lev)
+                times[shape] = measure()
+                trials["L%d:%dx%d" % ((lev,) + shape)] = times[shape]
+            best[lev] = min(times, key=times.get)
+            self.set_tiles(*best[lev], level=lev)
+        return best, trials
 
     def load_initial(self, wl, states=None):
         """Upload the analytic initial state (or given per-patch w arrays),
@@ -110,6 +171,9 @@ class DeviceHierarchy:
             _lib.check(self.lib.enl_batch_copy_state(self.handle, k, 0, _p(w), _p(pr), _p(T), None))
             out.append((w, pr, T))
         return out
+
+    def sync(self):
+        _lib.check(self.lib.enl_sync())
 
     def close(self):
         if self.handle:

@@ -38,6 +38,8 @@ class Workload:
     center: tuple = (0.5, 0.5)
     amp_range: tuple = (0.05, 0.8)
     sigma_range: tuple = (0.005, 0.03)
+    # hierarchy workloads: base-grid steps taken before the tags are formed
+    presteps: int = 0
 
     @property
     def dx(self):
@@ -116,6 +118,8 @@ def initial_fields(wl, gas):
 
 
 # The batched-hierarchy workload: base 1024^2 on [0,1]^2 m in 64^2 patches,
-# one finer level (ratio 4) from tag_cells, eight seeded pulses as in
-# PULSES_4096 (seed 2).
-AMR_1024 = Workload("amr_1024_r4_8pulses", n=1024, steps=100, seed=2, npulses=8)
+# eight seeded pulses as in PULSES_4096 (seed 2), advanced 500 steps on the
+# uniform base grid so that they have spread into interacting rings; then
+# one finer level (ratio 4) over tag_cells' tags (565 patches of 8..64 cells
+# a side, 1.2 M fine cells), initialised by linear refinement.
+AMR_1024 = Workload("amr_1024_r4_8pulses", n=1024, steps=100, seed=2, npulses=8, presteps=500)

@@ -203,8 +203,10 @@ int enl_batch_set_fill(void *handle, int level, int ncopy, const int *c_dp, cons
 /* Fine-to-coarse average after each step: (coarse patch, coarse local,
  * fine patch, fine local of the lower-left fine cell), ratio^2 cells. */
 int enl_batch_set_coarsen(void *handle, int n, const int *cp, const int *cl, const int *fp, const int *fl);
-/* Tile shape of the persistent stage kernel: 32x8, 16x16, 16x8 or 8x8. */
+/* Tile shape of the stage kernels on every level: 32x8, 16x16, 16x8 or 8x8. */
 int enl_batch_set_tiles(void *handle, int bx, int by);
+/* Tile shape of the stage kernels on one level (same choices). */
+int enl_batch_set_level_tiles(void *handle, int level, int bx, int by);
 /* Device pointers of a patch's current state and ghosted source. */
 int enl_batch_patch_state(void *handle, int pid, double **w_n, double **pres, double **temp, double **source);
 /* Copies a patch's current state (any argument may be NULL) between host

@@ -98,7 +98,7 @@ else:
         def advance(self, nsteps, dt, mode="graph", variant="hybrid_high_llf", rk=RK3):
             super().advance(nsteps, dt, rk, variant=variant)
 
-        def set_tiles(self, bx, by):
+        def set_tiles(self, bx, by, level=None):
             pass
 
         def close(self):

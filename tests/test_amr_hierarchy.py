@@ -13,7 +13,7 @@ import pytest
 import oracle
 from enlightning_amd import amr
 from enlightning_amd.gas import Gas, RK3
-from enlightning_amd.hierarchy import level_tags, patch_initial_state
+from enlightning_amd.hierarchy import level_tags, patch_initial_state, refine_from_base
 from enlightning_amd.workloads import Workload
 from oracle.amr import HostHierarchy
 
@@ -165,3 +165,32 @@ def test_twin_steps_stay_finite_and_fine_level_is_weno(hier):
     hh.advance(2, dt, RK3)
     for w, p, T in hh.states():
         assert np.isfinite(w).all() and np.isfinite(p).all()
+
+
+def test_refine_from_base_linear_and_windows(hier):
+    """Initial state of an evolved hierarchy: level-0 patches take their
+    window of the base grid; level-1 interiors are the linear refinement,
+    exact for a linear base field."""
+    h = hier
+    n0, g = h.n0, 4
+    x = (np.arange(-g, n0 + g) + 0.5) * h.dx(0)
+    X, Y = np.meshgrid(x, x)
+    a = np.array([1.2, 0.3, -0.2, 0.05, 350.0])
+    bx = np.array([0.1, 2.0, 0.5, 0.01, 3.0])
+    by = np.array([-0.05, 1.0, -1.5, 0.02, -2.0])
+    wb = np.stack([a[k] + bx[k] * X + by[k] * Y for k in range(5)])
+    for p in h.patches:
+        w = refine_from_base(h, p, wb)
+        assert w.shape == (5, p.Ny, p.Nx)
+        if p.level == 0:
+            i0, i1, j0, j1 = p.box
+            assert np.array_equal(w, wb[:, j0:j1 + 2 * g + 1, i0:i1 + 2 * g + 1])
+        else:
+            dx = h.dx(1)
+            xc = (np.arange(p.box[0], p.box[1] + 1) + 0.5) * dx
+            yc = (np.arange(p.box[2], p.box[3] + 1) + 0.5) * dx
+            XX, YY = np.meshgrid(xc, yc)
+            for k in range(5):
+                exact = a[k] + bx[k] * XX + by[k] * YY
+                got = w[k, g:-g, g:-g]
+                assert np.abs(got - exact).max() <= 1e-12 * np.abs(exact).max(), (p.pid, k)
