@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--no-amr", action="store_true")
     ap.add_argument("--amr-steps", type=int, default=40)
     ap.add_argument("--amr-plain-steps", type=int, default=4)
+    ap.add_argument("--precision-steps", type=int, default=2000)
     return ap.parse_args()
 
 
@@ -168,6 +169,100 @@ def amr_bench(args):
         "speedup_graph_vs_plain": (t_plain / args.amr_plain_steps) / (t_graph / args.amr_steps),
         "graph_bit_identical_to_plain": same,
         "unit": "cell-updates/s",
+    }, tags
+
+
+def precision_bench(args, tags):
+    """The FP32-perturbation form on the same hierarchy (the boxes of the
+    AMR workload, on a 2 km x 2 km domain): 16 weak isentropic pulses on an
+    exponentially stratified ambient (scale height 8 km), FP64 batched graph
+    against FP32 batched graph from the same state, per-field error."""
+    import numpy as np
+    import torch
+    from enlightning_amd import amr
+    from enlightning_amd.gas import Gas
+    from enlightning_amd.hierarchy import DeviceHierarchy
+    from enlightning_amd.workloads import WEAK_2KM, ambient_rows, stratified_state
+
+    gas = Gas()
+    wl = WEAK_2KM
+    h = amr.build(wl.n, 4, 4, wl.L, tags, tile0=64)
+    states, ambs = [], []
+    for p in h.patches:
+        dx = h.dx(p.level)
+        x = (np.arange(p.box[0] - p.g, p.box[1] + p.g + 1) + 0.5) * dx
+        y = (np.arange(p.box[2] - p.g, p.box[3] + p.g + 1) + 0.5) * dx
+        states.append(stratified_state(wl, gas, x, y))
+        rho, rhos, rhotn = ambient_rows(wl, gas, y)
+        ambs.append((rho, rhos, rhotn))
+
+    def ambient(lev):
+        n, g = h.ncells(lev), h.g
+        y = (np.arange(-g, n + g) + 0.5) * h.dx(lev)
+        return (-g,) + ambient_rows(wl, gas, y)
+
+    steps = args.precision_steps
+    dt = 0.5 * h.dx(h.nlevels - 1) / gas.c
+    cells = h.interior_cells()
+    d = DeviceHierarchy(h, gas)
+    d.set_tiles(16, 8)
+    d.load_initial(wl, states)
+    d.advance(1, dt, mode="graph")  # capture
+    d.load_initial(wl, states)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    d.advance(steps, dt, mode="graph")
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
+    s64 = d.states()
+    d.load_initial(wl, states)
+    d.set_ambient(ambient)
+    d.fp32_prepare()
+    d.advance(1, dt, mode="fp32_graph")  # capture
+    d.fp32_load()
+    d.load_initial(wl, states)
+    d.fp32_load()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    d.advance(steps, dt, mode="fp32_graph")
+    torch.cuda.synchronize()
+    t32 = time.perf_counter() - t0
+    d.fp32_store()
+    s32 = d.states()
+    d.close()
+    names = ("rho", "rho_u", "rho_v", "rho_s", "rho_Tn", "p", "T")
+    err = {}
+    for f_i, name in enumerate(names):
+        num = den = dev = linf = amax = 0.0
+        for p, a, b, am in zip(h.patches, s64, s32, ambs):
+            g = p.g
+            pick = (lambda st: st[0][f_i]) if f_i < 5 else (lambda st: st[f_i - 4])
+            x, y = pick(a)[g:-g, g:-g], pick(b)[g:-g, g:-g]
+            if f_i in (0, 3, 4):
+                ref = x - am[(0, None, None, 1, 2)[f_i]][g:-g, None]
+            elif f_i in (1, 2):
+                ref = x
+            elif f_i == 5:
+                ref = x - gas.p0
+            else:
+                ref = None
+            num += float(np.sum((x - y) ** 2))
+            den += float(np.sum(x ** 2))
+            if ref is not None:
+                dev += float(np.sum(ref ** 2))
+            linf = max(linf, float(np.abs(x - y).max()))
+            amax = max(amax, float(np.abs(x).max()))
+        err[name] = {"rel_l2": float(np.sqrt(num / den)), "rel_linf": linf / amax,
+                     "rel_l2_of_perturbation": float(np.sqrt(num / dev)) if dev > 0 else None}
+    return {
+        "workload": wl.name,
+        "hierarchy": "the amr workload's boxes on a %g m domain" % wl.L,
+        "steps": steps,
+        "fp64_graph": {"value": cells * steps / t64, "ms_per_step": t64 * 1e3 / steps},
+        "fp32_perturbation_graph": {"value": cells * steps / t32, "ms_per_step": t32 * 1e3 / steps},
+        "speedup_fp32_vs_fp64": t64 / t32,
+        "error_fp32_vs_fp64": err,
+        "unit": "cell-updates/s",
     }
 
 
@@ -283,7 +378,10 @@ def main():
     del plain, fused
     torch.cuda.empty_cache()
     if not args.no_amr:
-        out["amr"] = amr_bench(args)
+        out["amr"], tags = amr_bench(args)
+        torch.cuda.empty_cache()
+        if args.precision_steps > 0:
+            out["precision"] = precision_bench(args, tags)
         torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, gas, args.cpu_steps)

@@ -67,6 +67,10 @@ PROTOTYPES = {
     "enl_batch_set_coarsen": ([_P, _I] + [_P] * 4, _I),
     "enl_batch_set_tiles": ([_P, _I, _I], _I),
     "enl_batch_set_level_tiles": ([_P, _I, _I, _I], _I),
+    "enl_batch_set_ambient": ([_P, _I, _I, _I, _P, _P, _P, _P], _I),
+    "enl_batch_fp32_prepare": ([_P, _I, _P, _P], _I),
+    "enl_batch_fp32_load": ([_P], _I),
+    "enl_batch_fp32_store": ([_P], _I),
     "enl_batch_patch_state": ([_P, _I] + [_P] * 4, _I),
     "enl_batch_copy_state": ([_P, _I, _I] + [_P] * 4, _I),
     "enl_batch_prepare": ([_P, _P], _I),

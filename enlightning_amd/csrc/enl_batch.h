@@ -1,0 +1,106 @@
+// enl_batch.h -- the batched hierarchy's host-side state, shared by the FP64
+// form (enl_batch.hip) and the FP32-perturbation form (enl_batch32.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "enl_kernels.h"
+#include "enlightning_hip.h"
+
+namespace enl {
+
+constexpr int kMaxStages = 8;
+constexpr int kMaxLevels = 8;
+
+// One patch in device memory.  Offsets are in doubles from the pool base.
+struct PatchDev {
+  int nx, ny, g, level;
+  long sy, sk;      // ghosted row / component strides
+  long state[3];    // three ghosted state blocks [w_n(5), pres, temp], 7*sk each
+  long src;         // ghosted source
+  long src_int;     // interior copy of the source (the drop-in's layout)
+  long rhs;         // interior rhs slots, 5*nx*ny
+  double dx, dy, ylo;
+};
+
+// Device copies of one level's fill schedule.
+struct LevelFill {
+  int ncopy = 0, ninterp = 0, nedge = 0, ncorner = 0;
+  int *c_dp = nullptr, *c_dl = nullptr, *c_sp = nullptr, *c_sl = nullptr;
+  int *i_dp = nullptr, *i_dl = nullptr, *i_sp = nullptr, *i_sl = nullptr;
+  double *i_fx = nullptr, *i_fy = nullptr;
+  int *e_p = nullptr, *e_d = nullptr, *e_s = nullptr, *e_n = nullptr;
+  int *k_p = nullptr, *k_d = nullptr, *k_s = nullptr, *k_n = nullptr;
+};
+
+struct Fp32Form;  // enl_batch32.hip
+
+struct Batch {
+  int npatch = 0, g = 0, ratio = 0, nlevels = 0;
+  std::vector<PatchDev> hp;
+  std::vector<enl_patch_desc_t> desc;
+  double* pool = nullptr;
+  size_t pool_doubles = 0;
+  PatchDev* dp = nullptr;
+  int4* tiles = nullptr;
+  int ntiles = 0;
+  int tile_bx[8], tile_by[8];     // tile shape per level
+  int level_tile_begin[9] = {0};  // tiles sorted by level
+  bool all_pow2 = true;
+  LevelFill fill[8];
+  int ncoarsen = 0;
+  int *co_cp = nullptr, *co_cl = nullptr, *co_fp = nullptr, *co_fl = nullptr;
+  unsigned long long* list = nullptr;
+  unsigned* count = nullptr;
+  long cells = 0;
+  long level_cells[8] = {0};
+  RhsArgs* d_args = nullptr;  // kMaxStages x npatch argument blocks
+  std::vector<RhsArgs> h_args;
+  std::vector<void*> allocations;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<unsigned char> graph_key;
+  Fp32Form* f32 = nullptr;  // the FP32-perturbation form's buffers, once set up
+};
+
+inline unsigned grid_for(long n, int block = 256, long cap = 4096) {
+  long b = (n + block - 1) / block;
+  if (b < 1) b = 1;
+  return (unsigned)(b > cap ? cap : b);
+}
+
+template <typename T>
+inline int upload(Batch* b, T** dst, const T* src, long n) {
+  *dst = nullptr;
+  if (n <= 0) return 0;
+  hipError_t e = hipMalloc((void**)dst, sizeof(T) * (size_t)n);
+  if (e != hipSuccess) return check_hip(e, "hipMalloc(batch table)");
+  b->allocations.push_back(*dst);
+  return check_hip(hipMemcpy(*dst, src, sizeof(T) * (size_t)n, hipMemcpyHostToDevice), "upload batch table");
+}
+
+// State blocks of RK stage st of an order-stage step: stage 0 reads block 0
+// (the step's start state), stages alternate blocks 1 and 2, the last stage
+// writes block 0.
+inline int stage_out(int st, int order) { return (st == order - 1 && order > 1) ? 0 : (st % 2 == 0 ? 1 : 2); }
+inline int stage_in(int st, int order) { return st == 0 ? 0 : stage_out(st - 1, order); }
+
+// Ghost fill of one level of FP64 state block `buf` (-1: the source) from the
+// level's schedule; the coarse-to-fine refine reads block `cbuf` of the
+// coarser level.
+void fill_level(Batch* b, int level, int buf, int cbuf, int ncomp, const int* reflect, hipStream_t s);
+
+// The FP64 rhs of every patch of state block `in` into the w_n components of
+// block `out` (ghosts of `in` filled first).
+int batch_eval_rhs64(Batch* b, int variant, const enl_gas_t* gas, const int* reflect, int in, int out,
+                     hipStream_t s);
+
+// Steps of the FP32-perturbation form (modes ENL_BATCH_FP32 / _FP32_GRAPH).
+int fp32_advance(Batch* b, int mode, const enl_gas_t* gas, const int* reflect, int order, const double* a1,
+                 const double* a2, const double* be, double dt, int nsteps, hipStream_t s);
+
+// Frees the FP32 form's buffers (no-op when it was never set up).
+void fp32_release(Batch* b);
+
+}  // namespace enl

@@ -31,6 +31,7 @@
 #include <cstring>
 #include <vector>
 
+#include "enl_batch.h"
 #include "enl_rhs_body.h"
 #include "enlightning_hip.h"
 
@@ -39,18 +40,6 @@
 namespace enl {
 namespace {
 
-// One patch in device memory.  Offsets are in doubles from the pool base.
-constexpr int kMaxStages = 8;
-
-struct PatchDev {
-  int nx, ny, g, level;
-  long sy, sk;      // ghosted row / component strides
-  long state[3];    // three ghosted state blocks [w_n(5), pres, temp], 7*sk each
-  long src;         // ghosted source
-  long src_int;     // interior copy of the source (the drop-in's layout)
-  long rhs;         // interior rhs slots, 5*nx*ny
-  double dx, dy, ylo;
-};
 
 // Per stage and patch the kernels read a prebuilt argument block (pointers
 // into the pool, grid reciprocals, gas constants, RK coefficients) through a
@@ -106,7 +95,6 @@ __global__ void batch_reset_kernel(unsigned* count, int n) {
   if ((int)threadIdx.x < n) count[threadIdx.x] = 0;
 }
 
-constexpr int kMaxLevels = 8;
 
 // ---- ghost fill and coarsen -------------------------------------------------
 
@@ -200,88 +188,13 @@ __global__ void copy_interior_block_kernel(double* dst, const double* src, Patch
   }
 }
 
-unsigned grid_for(long n, int block = 256, long cap = 4096) {
-  long b = (n + block - 1) / block;
-  if (b < 1) b = 1;
-  return (unsigned)(b > cap ? cap : b);
-}
-
 bool pow2(double x) {
   if (!(x > 0) || !std::isfinite(x)) return false;
   int e;
   return std::frexp(x, &e) == 0.5;
 }
 
-// Device copies of one level's fill schedule.
-struct LevelFill {
-  int ncopy = 0, ninterp = 0, nedge = 0, ncorner = 0;
-  int *c_dp = nullptr, *c_dl = nullptr, *c_sp = nullptr, *c_sl = nullptr;
-  int *i_dp = nullptr, *i_dl = nullptr, *i_sp = nullptr, *i_sl = nullptr;
-  double *i_fx = nullptr, *i_fy = nullptr;
-  int *e_p = nullptr, *e_d = nullptr, *e_s = nullptr, *e_n = nullptr;
-  int *k_p = nullptr, *k_d = nullptr, *k_s = nullptr, *k_n = nullptr;
-};
-
 }  // namespace
-
-struct Batch {
-  int npatch = 0, g = 0, ratio = 0, nlevels = 0;
-  std::vector<PatchDev> hp;
-  std::vector<enl_patch_desc_t> desc;
-  double* pool = nullptr;
-  size_t pool_doubles = 0;
-  PatchDev* dp = nullptr;
-  int4* tiles = nullptr;
-  int ntiles = 0;
-  int tile_bx[8], tile_by[8];     // tile shape per level
-  int level_tile_begin[9] = {0};  // tiles sorted by level
-  bool all_pow2 = true;
-  LevelFill fill[8];
-  int ncoarsen = 0;
-  int *co_cp = nullptr, *co_cl = nullptr, *co_fp = nullptr, *co_fl = nullptr;
-  unsigned long long* list = nullptr;
-  unsigned* count = nullptr;
-  long cells = 0;
-  long level_cells[8] = {0};
-  RhsArgs* d_args = nullptr;  // kMaxStages x npatch argument blocks
-  std::vector<RhsArgs> h_args;
-  std::vector<void*> allocations;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
-  std::vector<unsigned char> graph_key;
-};
-
-namespace {
-
-template <typename T>
-int upload(Batch* b, T** dst, const T* src, long n) {
-  *dst = nullptr;
-  if (n <= 0) return 0;
-  hipError_t e = hipMalloc((void**)dst, sizeof(T) * (size_t)n);
-  if (e != hipSuccess) return check_hip(e, "hipMalloc(batch table)");
-  b->allocations.push_back(*dst);
-  return check_hip(hipMemcpy(*dst, src, sizeof(T) * (size_t)n, hipMemcpyHostToDevice), "upload batch table");
-}
-
-int make_tiles(Batch* b) {
-  std::vector<int4> t;
-  for (int lev = 0; lev < kMaxLevels; lev++) {
-    b->level_tile_begin[lev] = (int)t.size();
-    const int bx = b->tile_bx[lev], by = b->tile_by[lev];
-    for (int p = 0; p < b->npatch; p++)
-      if (b->hp[p].level == lev)
-        for (int j0 = 0; j0 < b->hp[p].ny; j0 += by)
-          for (int i0 = 0; i0 < b->hp[p].nx; i0 += bx) t.push_back(make_int4(p, i0, j0, 0));
-  }
-  b->level_tile_begin[kMaxLevels] = (int)t.size();
-  if (b->tiles) {
-    (void)hipFree(b->tiles);
-    for (auto& a : b->allocations)
-      if (a == b->tiles) a = nullptr;
-  }
-  b->ntiles = (int)t.size();
-  return upload(b, &b->tiles, t.data(), (long)t.size());
-}
 
 void fill_level(Batch* b, int level, int buf, int cbuf, int ncomp, const int* reflect, hipStream_t s) {
   FillArgs f;
@@ -306,21 +219,37 @@ void fill_level(Batch* b, int level, int buf, int cbuf, int ncomp, const int* re
                        L.k_s, L.k_n);
 }
 
+
+namespace {
+
+
+int make_tiles(Batch* b) {
+  std::vector<int4> t;
+  for (int lev = 0; lev < kMaxLevels; lev++) {
+    b->level_tile_begin[lev] = (int)t.size();
+    const int bx = b->tile_bx[lev], by = b->tile_by[lev];
+    for (int p = 0; p < b->npatch; p++)
+      if (b->hp[p].level == lev)
+        for (int j0 = 0; j0 < b->hp[p].ny; j0 += by)
+          for (int i0 = 0; i0 < b->hp[p].nx; i0 += bx) t.push_back(make_int4(p, i0, j0, 0));
+  }
+  b->level_tile_begin[kMaxLevels] = (int)t.size();
+  if (b->tiles) {
+    (void)hipFree(b->tiles);
+    for (auto& a : b->allocations)
+      if (a == b->tiles) a = nullptr;
+  }
+  b->ntiles = (int)t.size();
+  return upload(b, &b->tiles, t.data(), (long)t.size());
+}
+
+
 void fill_all(Batch* b, int buf, const int* reflect, hipStream_t s) {
   for (int lev = 0; lev < b->nlevels; lev++) fill_level(b, lev, buf, buf, 7, reflect, s);
 }
 
-// State blocks of RK stage st of an order-stage step: stage 0 reads block 0
-// (the step's start state, also ww), stages alternate blocks 1 and 2, the
-// last stage writes block 0.
-int stage_out(int st, int order) { return (st == order - 1 && order > 1) ? 0 : (st % 2 == 0 ? 1 : 2); }
-int stage_in(int st, int order) { return st == 0 ? 0 : stage_out(st - 1, order); }
-
-// Argument blocks of every (stage, patch), built on the host (the same
-// fill_rhs_constants as the single-patch launch) and uploaded when they
-// change; the kernels of stage st read b->d_args + st*npatch.
-int prepare_args(Batch* b, const enl_gas_t* gas, int order, const double* a1, const double* a2, const double* be,
-                 double dt, hipStream_t s) {
+// Arguments common to every patch: gas constants and the WENO list.
+RhsArgs base_args(Batch* b, const enl_gas_t* gas) {
   RhsArgs base;
   memset(&base, 0, sizeof base);
   base.alpha = gas->weno_alpha;
@@ -341,41 +270,55 @@ int prepare_args(Batch* b, const enl_gas_t* gas, int order, const double* a1, co
   base.c_v = gas->c_v;
   base.list = b->list;
   base.switch_map = nullptr;
-  std::vector<RhsArgs> t((size_t)order * b->npatch, base);
+  return base;
+}
+
+// One patch's arguments: read state block `in`, write block `out`, ww in
+// block `ww`, WENO-list counter slot `slot`.
+void patch_args(Batch* b, int p, int in, int out, int ww, int slot, RhsArgs& a) {
+  const PatchDev& pd = b->hp[p];
+  a.geom.nx = pd.nx;
+  a.geom.ny = pd.ny;
+  a.geom.g0 = pd.g;
+  a.geom.g1 = pd.g;
+  a.geom.sy = pd.sy;
+  a.geom.sk = pd.sk;
+  const double* w = b->pool + pd.state[in];
+  a.w = w;
+  a.p = w + 5 * pd.sk;
+  a.T = w + 6 * pd.sk;
+  a.H0 = b->pool + pd.src + pd.g + pd.g * pd.sy;
+  a.h0_sy = pd.sy;
+  a.KK = b->pool + pd.rhs;
+  a.dx = pd.dx;
+  a.dy = pd.dy;
+  a.ylo = pd.ylo;
+  a.level = pd.level;
+  a.count = b->count + slot * kMaxLevels + pd.level;
+  fill_rhs_constants(a);
+  a.ww = b->pool + pd.state[ww];
+  a.ww_off = pd.g + pd.g * pd.sy;
+  a.ww_sy = pd.sy;
+  a.ww_sk = pd.sk;
+  double* o = b->pool + pd.state[out];
+  a.w_out = o;
+  a.p_out = o + 5 * pd.sk;
+  a.T_out = o + 6 * pd.sk;
+}
+
+// Argument blocks of every (stage, patch), built on the host (the same
+// fill_rhs_constants as the single-patch launch) and uploaded when they
+// change; the kernels of stage st read b->d_args + st*npatch.
+int prepare_args(Batch* b, const enl_gas_t* gas, int order, const double* a1, const double* a2, const double* be,
+                 double dt, hipStream_t s) {
+  std::vector<RhsArgs> t((size_t)order * b->npatch, base_args(b, gas));
   for (int st = 0; st < order; st++) {
     for (int p = 0; p < b->npatch; p++) {
-      const PatchDev& pd = b->hp[p];
       RhsArgs& a = t[(size_t)st * b->npatch + p];
       a.alpha_1 = a1[st];
       a.alpha_2 = a2[st];
       a.bdt = be[st] * dt;
-      a.geom.nx = pd.nx;
-      a.geom.ny = pd.ny;
-      a.geom.g0 = pd.g;
-      a.geom.g1 = pd.g;
-      a.geom.sy = pd.sy;
-      a.geom.sk = pd.sk;
-      const double* in = b->pool + pd.state[stage_in(st, order)];
-      a.w = in;
-      a.p = in + 5 * pd.sk;
-      a.T = in + 6 * pd.sk;
-      a.H0 = b->pool + pd.src + pd.g + pd.g * pd.sy;
-      a.h0_sy = pd.sy;
-      a.KK = b->pool + pd.rhs;
-      a.dx = pd.dx;
-      a.dy = pd.dy;
-      a.ylo = pd.ylo;
-      a.level = pd.level;
-      a.count = b->count + st * kMaxLevels + pd.level;
-      fill_rhs_constants(a);
-      a.ww = b->pool + pd.state[0];
-      a.ww_off = pd.g + pd.g * pd.sy;
-      a.ww_sy = pd.sy;
-      a.ww_sk = pd.sk;
-      double* out = b->pool + pd.state[stage_out(st, order)];
-      a.w_out = out;
-      a.p_out = out + 5 * pd.sk;
-      a.T_out = out + 6 * pd.sk;
+      patch_args(b, p, stage_in(st, order), stage_out(st, order), 0, st, a);
     }
   }
   const size_t bytes = sizeof(RhsArgs) * t.size();
@@ -466,6 +409,41 @@ int enqueue_batched(Batch* b, int variant, const int* reflect, int order, int ns
   }
   return check_hip(hipGetLastError(), "batched stage launch");
 }
+
+}  // namespace
+
+// The rhs of every patch of state block `in`, stored into the w_n
+// components of block `out`: the batched stage kernels with alpha_1 =
+// alpha_2 = 0 and beta*dt = 1 (0*ww + 0*w + 1*K).  Ghosts of `in` are
+// filled first, level by level.  Uses WENO-list counter slot kMaxStages-1
+// and a temporary argument table.
+int batch_eval_rhs64(Batch* b, int variant, const enl_gas_t* gas, const int* reflect, int in, int out,
+                     hipStream_t s) {
+  std::vector<RhsArgs> t((size_t)b->npatch, base_args(b, gas));
+  for (int p = 0; p < b->npatch; p++) {
+    t[p].alpha_1 = 0;
+    t[p].alpha_2 = 0;
+    t[p].bdt = 1;
+    patch_args(b, p, in, out, 0, kMaxStages - 1, t[p]);
+  }
+  RhsArgs* d = nullptr;
+  hipError_t e = hipMalloc((void**)&d, sizeof(RhsArgs) * t.size());
+  if (e != hipSuccess) return check_hip(e, "hipMalloc(eval arguments)");
+  int rc = check_hip(hipMemcpy(d, t.data(), sizeof(RhsArgs) * t.size(), hipMemcpyHostToDevice), "eval arguments");
+  if (!rc) {
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, b->count, kMaxStages * kMaxLevels);
+    for (int lev = 0; lev < b->nlevels && !rc; lev++) {
+      fill_level(b, lev, in, in, 7, reflect, s);
+      rc = b->all_pow2 ? stage_variant<true>(b, variant, d, kMaxStages - 1, lev, s)
+                       : stage_variant<false>(b, variant, d, kMaxStages - 1, lev, s);
+    }
+  }
+  if (!rc) rc = check_hip(hipStreamSynchronize(s), "batch_eval_rhs64");
+  (void)hipFree(d);
+  return rc;
+}
+
+namespace {
 
 // The reference's way: the drop-in entry points called patch by patch
 // (SAMRAI's copy current -> scratch, then per stage the ghost fill and
@@ -784,6 +762,8 @@ int enl_batch_advance(void* handle, int mode, int variant, const enl_gas_t* gas,
   if (order < 1 || order > kMaxStages) return set_error(-2, "RK order out of range");
   hipStream_t s = lib_stream();
   if (mode == ENL_BATCH_PLAIN) return enqueue_plain(b, variant, gas, reflect, order, alpha_1, alpha_2, beta, dt, nsteps, s);
+  if (mode == ENL_BATCH_FP32 || mode == ENL_BATCH_FP32_GRAPH)
+    return fp32_advance(b, mode, gas, reflect, order, alpha_1, alpha_2, beta, dt, nsteps, s);
   if (mode != ENL_BATCH_FUSED && mode != ENL_BATCH_GRAPH) return set_error(-2, "unknown batch mode");
   int prc = prepare_args(b, gas, order, alpha_1, alpha_2, beta, dt, s);
   if (prc) return prc;
@@ -851,6 +831,7 @@ int enl_batch_destroy(void* handle) {
   (void)hipDeviceSynchronize();
   if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  fp32_release(b);
   for (void* a : b->allocations)
     if (a) (void)hipFree(a);
   delete b;

@@ -9,7 +9,7 @@ from . import _lib
 from .gas import RK3
 from .workloads import pulses_at, state_from_overpressure
 
-MODES = {"plain": 0, "fused": 1, "graph": 2}
+MODES = {"plain": 0, "fused": 1, "graph": 2, "fp32": 3, "fp32_graph": 4}
 
 
 def _p(a):
@@ -171,6 +171,29 @@ class DeviceHierarchy:
             _lib.check(self.lib.enl_batch_copy_state(self.handle, k, 0, _p(w), _p(pr), _p(T), None))
             out.append((w, pr, T))
         return out
+
+    # ---- FP32-perturbation form ------------------------------------------
+    def set_ambient(self, ambient):
+        """ambient(level) -> (row_lo, rho, rho*s, rho*T_n) arrays over level
+        rows row_lo .. row_lo+len-1 covering every patch's ghosted rows."""
+        gs = self.gas.to_struct()
+        for lev in range(self.h.nlevels):
+            row_lo, rho, rhos, rhotn = ambient(lev)
+            arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, rhos, rhotn)]
+            _lib.check(self.lib.enl_batch_set_ambient(self.handle, lev, int(row_lo), len(arrs[0]), *[_p(a) for a in arrs],
+                                                      ctypes.byref(gs)))
+
+    def fp32_prepare(self, variant="hybrid_high_llf"):
+        """Ambient correction and conversion of the current FP64 state."""
+        vid = _lib.VARIANT_ID[variant] if isinstance(variant, str) else int(variant)
+        gs = self.gas.to_struct()
+        _lib.check(self.lib.enl_batch_fp32_prepare(self.handle, vid, ctypes.byref(gs), self._reflect()))
+
+    def fp32_load(self):
+        _lib.check(self.lib.enl_batch_fp32_load(self.handle))
+
+    def fp32_store(self):
+        _lib.check(self.lib.enl_batch_fp32_store(self.handle))
 
     def sync(self):
         _lib.check(self.lib.enl_sync())

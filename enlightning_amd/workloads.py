@@ -123,3 +123,55 @@ def initial_fields(wl, gas):
 # one finer level (ratio 4) over tag_cells' tags (565 patches of 8..64 cells
 # a side, 1.2 M fine cells), initialised by linear refinement.
 AMR_1024 = Workload("amr_1024_r4_8pulses", n=1024, steps=100, seed=2, npulses=8, presteps=500)
+
+
+# ---- the precision form's weak-acoustic workload -----------------------------
+# An exponentially stratified ambient at rest and uniform pressure p0:
+#   rho_a(y) = rho0 exp(-y / H),  rho*s_a = -c_p ((rho_a - rho0) + a (rho_a - rho0)^2)
+# (the EOS of update_state.m4:90-94 then gives p = p0 at c_lr = 0, so the
+# ambient is in mechanical equilibrium; there is no gravity in the model),
+# rho*T_n in equilibrium with T (update_state.m4:103-105).  Pulses are
+# isentropic (specific entropy of the ambient row kept) with dp/p0 drawn in
+# [amp_range]; rho from the EOS quadratic at the cell's s.
+
+
+@dataclass
+class StratifiedWorkload(Workload):
+    scale_height: float = 8000.0
+
+
+WEAK_2KM = StratifiedWorkload("weak_acoustic_2km_16pulses", n=1024, L=2000.0, steps=2000, seed=3, npulses=16,
+                              amp_range=(1e-5, 1e-3), sigma_range=(0.005, 0.03), scale_height=8000.0)
+
+
+def ambient_rows(wl, gas, y):
+    """(rho, rho*s, rho*T_n) of the stratified ambient at heights y."""
+    y = np.asarray(y, dtype=np.float64)
+    a = (gas.gamma1 - 1.0) / (2.0 * gas.rho0)
+    rho = gas.rho0 * np.exp(-y / wl.scale_height)
+    D = rho - gas.rho0
+    rhos = -gas.c_p * (D + a * D * D)
+    s = rhos / rho
+    T = gas.T0 * np.exp((s - gas.R * np.log(gas.rho0 / rho)) / gas.c_v)
+    return rho, rhos, rho * T
+
+
+def stratified_state(wl, gas, x, y):
+    """w (5, len(y), len(x)) on cell centres: ambient + isentropic pulses."""
+    rho_a, rhos_a, _ = ambient_rows(wl, gas, y)
+    s = (rhos_a / rho_a)[:, None] * np.ones((1, len(x)))
+    dp = pulses_at(wl, x, y) * gas.p0
+    a = (gas.gamma1 - 1.0) / (2.0 * gas.rho0)
+    # c^2 (D + a D^2 + (rho0 + D) s / c_p) = dp, D = rho - rho0 (c_lr = 0)
+    b = 1.0 + s / gas.c_p
+    cc = gas.rho0 * s / gas.c_p - dp / gas.c ** 2
+    D = (-b + np.sqrt(b * b - 4.0 * a * cc)) / (2.0 * a)
+    # the ambient rows' own root, so that dp = 0 reproduces them exactly
+    D = np.where(dp == 0.0, (rho_a - gas.rho0)[:, None], D)
+    rho = gas.rho0 + D
+    T = gas.T0 * np.exp((s - gas.R * np.log(gas.rho0 / rho)) / gas.c_v)
+    w = np.zeros((5,) + rho.shape)
+    w[0] = rho
+    w[3] = rho * s
+    w[4] = rho * T
+    return w

@@ -180,7 +180,7 @@ typedef struct enl_patch_desc {
   double dx[2];
 } enl_patch_desc_t;
 
-enum { ENL_BATCH_PLAIN = 0, ENL_BATCH_FUSED = 1, ENL_BATCH_GRAPH = 2 };
+enum { ENL_BATCH_PLAIN = 0, ENL_BATCH_FUSED = 1, ENL_BATCH_GRAPH = 2, ENL_BATCH_FP32 = 3, ENL_BATCH_FP32_GRAPH = 4 };
 
 /* Creates a batch: one device pool holding, per patch, three ghosted states
  * [w_n(5), pres, temp] (SAMRAI CellData layout, 7 components), a ghosted
@@ -227,6 +227,27 @@ int enl_batch_update_state(void *handle, const enl_gas_t *gas);
 int enl_batch_advance(void *handle, int mode, int variant, const enl_gas_t *gas, const int *reflect, int order,
                       const double *alpha_1, const double *alpha_2, const double *beta, double dt, int nsteps);
 int enl_batch_destroy(void *handle);
+
+/* ---- FP32-perturbation form of the batched hierarchy (no reference
+ * counterpart; the precision form of the hot path) -------------------------
+ * The state is stored as FP32 deviations from an FP64 ambient that depends
+ * on height only, with FP64 RK accumulation of the carried state.  The rhs is
+ * rhs_hybrid_high_llf's, evaluated in FP32 on the perturbations.  Steps run
+ * through enl_batch_advance with mode ENL_BATCH_FP32 or ENL_BATCH_FP32_GRAPH
+ * (variant rhs_hybrid_high_llf, RK order >= 2). */
+/* Ambient of one level: rho, rho*s, rho*T_n on level rows row_lo ..
+ * row_lo+nrows-1 (covering every patch's ghosted rows); p and T follow from
+ * update_state's equation of state. */
+int enl_batch_set_ambient(void *handle, int level, int row_lo, int nrows, const double *rho, const double *rhos,
+                          const double *rhoTn, const enl_gas_t *gas);
+/* Sets up the FP32 form: computes the ambient correction (FP64 minus FP32
+ * rhs of the ambient) and converts the current FP64 state into the FP32 form.
+ * Uses FP64 state blocks 1 and 2 as scratch. */
+int enl_batch_fp32_prepare(void *handle, int variant, const enl_gas_t *gas, const int *reflect);
+/* Current FP64 state -> FP32 form (after prepare). */
+int enl_batch_fp32_load(void *handle);
+/* FP32 form -> the current FP64 state (ambient + perturbation). */
+int enl_batch_fp32_store(void *handle);
 
 /* ---- measurement hooks (no reference counterpart) ---------------------- */
 

@@ -1,0 +1,149 @@
+"""FP32-perturbation form of the batched hierarchy against its FP64 form.
+
+The FP64 batched form is pinned to the oracle (tests/test_gpu_batch.py); the
+FP32 form stores the state as FP32 deviations from an FP64 stratified ambient
+and accumulates the RK stages in FP64.  Checked here:
+* the ambient alone follows the FP64 form's own (tiny) ambient tendency;
+* weak isentropic pulses stay within a per-field relative L2 error of the
+  FP64 result, measured on the perturbation (the field minus the ambient);
+* eager and graph-replayed FP32 steps are bit-identical.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from enlightning_amd import amr
+from enlightning_amd.gas import Gas
+from enlightning_amd.hierarchy import level_tags
+from enlightning_amd.workloads import StratifiedWorkload, Workload, ambient_rows, stratified_state
+from tests import devops as D
+
+pytestmark = pytest.mark.gpu
+
+TAGS_WL = Workload("amr_test_64", n=64, seed=7, npulses=4, amp_range=(0.3, 0.8), sigma_range=(0.03, 0.06))
+# 100 m at 64^2 keeps dt (0.5 dx_fine / c) below the N2 relaxation time, as
+# the 2 km / 1024^2 workload does; the scale height keeps its stratification
+# per domain (L/H = 1/4).
+WEAK = StratifiedWorkload("weak_test_64", n=64, L=100.0, seed=5, npulses=4, amp_range=(1e-4, 1e-3),
+                          sigma_range=(0.05, 0.1), scale_height=400.0)
+FIELDS = ("rho", "rho_u", "rho_v", "rho_s", "rho_Tn", "p", "T")
+
+
+def hierarchy():
+    gas = Gas()
+    tags = level_tags(gas, TAGS_WL, oracle.tag_cells)
+    h = amr.build(64, 4, 4, WEAK.L, tags, tile0=16)
+    assert h.nlevels == 2
+    return gas, h
+
+
+def patch_states(h, gas, wl, pulses=True):
+    out = []
+    for p in h.patches:
+        dx = h.dx(p.level)
+        x = (np.arange(p.box[0] - p.g, p.box[1] + p.g + 1) + 0.5) * dx
+        y = (np.arange(p.box[2] - p.g, p.box[3] + p.g + 1) + 0.5) * dx
+        if pulses:
+            out.append(stratified_state(wl, gas, x, y))
+        else:
+            rho, rhos, rhotn = ambient_rows(wl, gas, y)
+            w = np.zeros((5, len(y), len(x)))
+            w[0], w[3], w[4] = rho[:, None], rhos[:, None], rhotn[:, None]
+            out.append(w)
+    return out
+
+
+def ambient_fn(h, gas, wl):
+    def amb(lev):
+        n, g = h.ncells(lev), h.g
+        y = (np.arange(-g, n + g) + 0.5) * h.dx(lev)
+        rho, rhos, rhotn = ambient_rows(wl, gas, y)
+        return -g, rho, rhos, rhotn
+    return amb
+
+
+def run_both(steps, pulses=True, mode32="fp32_graph"):
+    gas, h = hierarchy()
+    states = patch_states(h, gas, WEAK, pulses)
+    dt = 0.5 * h.dx(1) / gas.c
+    d = D.Hierarchy(h, gas)
+    if not hasattr(d, "fp32_prepare"):
+        pytest.skip("the FP32 form has no CPU stand-in")
+    d.load_initial(WEAK, states)
+    d.advance(steps, dt, mode="graph")
+    s64 = d.states()
+    d.load_initial(WEAK, states)
+    d.set_ambient(ambient_fn(h, gas, WEAK))
+    d.fp32_prepare()
+    d.advance(steps, dt, mode=mode32)
+    d.fp32_store()
+    s32 = d.states()
+    d.close()
+    amb = patch_states(h, gas, WEAK, pulses=False)
+    return h, gas, s64, s32, amb
+
+
+def interior(a, p):
+    g = p.g
+    return a[..., g:-g, g:-g]
+
+
+def fields(state, p):
+    w, pr, T = state
+    return [interior(w[k], p) for k in range(5)] + [interior(pr, p), interior(T, p)]
+
+
+def test_ambient_alone_follows_fp64():
+    """The ambient alone is not steady next to the walls and the coarse-fine
+    boundaries (ghost copies and linear refinement of the stratified profile
+    differ from the profile); the FP32 form follows that FP64 evolution to 1 %
+    of its size, plus 1e-9 of each field's natural scale (momenta: rho0*c)."""
+    h, gas, s64, s32, amb = run_both(6, pulses=False)
+    scales = (gas.rho0, gas.rho0 * gas.c, gas.rho0 * gas.c, None, gas.rho0 * gas.T0, gas.p0, gas.T0)
+    ambient = {0: 0, 3: 3, 4: 4}
+    for name_i, name in enumerate(FIELDS):
+        dev = err = 0.0
+        for p, a, b, wa in zip(h.patches, s64, s32, amb):
+            x, y = fields(a, p)[name_i], fields(b, p)[name_i]
+            if name_i in ambient:
+                ref = interior(wa[name_i], p)
+            elif name_i in (1, 2):
+                ref = 0.0
+            elif name_i == 5:
+                ref = gas.p0
+            else:
+                ref = x.mean()
+            dev = max(dev, float(np.abs(x - ref).max()))
+            err = max(err, float(np.abs(x - y).max()))
+        sc = scales[name_i] if scales[name_i] is not None else 100.0
+        assert err <= 1e-2 * dev + 1e-9 * sc, (name, err, dev)
+
+
+def test_weak_pulses_relative_error():
+    h, gas, s64, s32, amb = run_both(20)
+    worst = {}
+    for f_i, name in enumerate(FIELDS):
+        num = den = 0.0
+        for p, a, b, wa in zip(h.patches, s64, s32, amb):
+            x, y = fields(a, p)[f_i], fields(b, p)[f_i]
+            if f_i < 5:
+                ref = x - interior(wa[f_i], p)
+            elif f_i == 5:
+                ref = x - gas.p0
+            else:
+                ref = None
+            num += float(np.sum((x - y) ** 2))
+            den += float(np.sum(ref ** 2)) if ref is not None else 0.0
+        if den > 0:
+            worst[name] = np.sqrt(num / den)
+    print("rel-L2 vs the perturbation, FP32 form after 20 steps:", worst)
+    for name, e in worst.items():
+        assert e <= 1e-4, (name, e)
+
+
+def test_fp32_eager_equals_graph():
+    _, _, _, a, _ = run_both(3, mode32="fp32")
+    _, _, _, b, _ = run_both(3, mode32="fp32_graph")
+    for x, y in zip(a, b):
+        for k in range(3):
+            assert np.array_equal(x[k], y[k])
