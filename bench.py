@@ -205,7 +205,7 @@ def precision_bench(args, tags):
     dt = 0.5 * h.dx(h.nlevels - 1) / gas.c
     cells = h.interior_cells()
     d = DeviceHierarchy(h, gas)
-    d.set_tiles(16, 8)
+    tiles64, _ = d.autotune(dt, states, wl, steps=3)
     d.load_initial(wl, states)
     d.advance(1, dt, mode="graph")  # capture
     d.load_initial(wl, states)
@@ -218,8 +218,10 @@ def precision_bench(args, tags):
     d.load_initial(wl, states)
     d.set_ambient(ambient)
     d.fp32_prepare()
-    d.advance(1, dt, mode="fp32_graph")  # capture
+    tiles32, _ = d.autotune(dt, states, wl, steps=3, mode="fp32_graph")
+    d.load_initial(wl, states)
     d.fp32_load()
+    d.advance(1, dt, mode="fp32_graph")  # capture
     d.load_initial(wl, states)
     d.fp32_load()
     torch.cuda.synchronize()
@@ -258,8 +260,10 @@ def precision_bench(args, tags):
         "workload": wl.name,
         "hierarchy": "the amr workload's boxes on a %g m domain" % wl.L,
         "steps": steps,
-        "fp64_graph": {"value": cells * steps / t64, "ms_per_step": t64 * 1e3 / steps},
-        "fp32_perturbation_graph": {"value": cells * steps / t32, "ms_per_step": t32 * 1e3 / steps},
+        "fp64_graph": {"value": cells * steps / t64, "ms_per_step": t64 * 1e3 / steps,
+                       "tiles_per_level": ["%dx%d" % tiles64[l] for l in sorted(tiles64)]},
+        "fp32_perturbation_graph": {"value": cells * steps / t32, "ms_per_step": t32 * 1e3 / steps,
+                                    "tiles_per_level": ["%dx%d" % tiles32[l] for l in sorted(tiles32)]},
         "speedup_fp32_vs_fp64": t64 / t32,
         "error_fp32_vs_fp64": err,
         "unit": "cell-updates/s",

@@ -53,6 +53,17 @@ struct Patch32 {
   long sy, sk;
 };
 
+// Per level row, what the stage kernel needs of the ambient: the FP64
+// values of the components whose ambient varies along y, and FP32 values
+// and FP64 differences rounded once.
+struct RowF {
+  double aW[3];  // rho, rho*s, rho*T_n
+  float aRho, aS, aTn, aT, aD;  // rho, s, T_n, T, rho - rho0
+  float cy2, TmTn, pA;          // (c + c_lr*y)^2, T - T_n, p - p0
+  float d1Rho, d1T, d2T, pad;   // rho(r+1)-rho(r), T(r+1)-T(r-1), T(r-1)-2T(r)+T(r+1)
+};
+static_assert(sizeof(RowF) % 4 == 0, "RowF is copied word by word");
+
 // Ambient value of stored component k (w_n 0..4, p, T) from a row record.
 __host__ __device__ inline double amb_of(const double* rec, int k) {
   switch (k) {
@@ -73,6 +84,7 @@ struct Args32 {
   float* kout;       // eval mode: K32 stored here
   const double* h0;  // source, ghosted
   const double* amb; // record of ghosted row 0
+  const RowF* rows;  // row table entry of ghosted row 0
   int nx, ny, g, last;
   long sy, sk;
   float ylo, dy;
@@ -129,6 +141,51 @@ __device__ inline float weno_dir32(const float* F, const float* W, float alpha) 
   return p1 + m1 - p2 - m2;
 }
 
+// FP64 WENO5 with reciprocals refined by one Newton step, for the
+// components whose ambient varies along the stencil (rho, rho*s, rho*T_n
+// along y): there the LLF term alpha*w carries the stratified ambient, much
+// larger than the perturbation, so its reconstruction is kept in FP64.
+__device__ inline double rcpd(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  return r * (2.0 - x * r);
+}
+__device__ inline double wenom5d(double fgm2, double fgm1, double fg0, double fgp1, double fgp2) {
+  const double b0 = kC13_12 * sq(fg0 - 2 * fgp1 + fgp2) + 0.25 * sq(3 * fg0 - 4 * fgp1 + fgp2);
+  const double b1 = kC13_12 * sq(fgm1 - 2 * fg0 + fgp1) + 0.25 * sq(fgm1 - fgp1);
+  const double b2 = kC13_12 * sq(fgm2 - 2 * fgm1 + fg0) + 0.25 * sq(fgm2 - 4 * fgm1 + 3 * fg0);
+  const double z0 = kW01 * rcpd(sq(kEpsWeno + b0));
+  const double z1 = kW06 * rcpd(sq(kEpsWeno + b1));
+  const double z2 = kW03 * rcpd(sq(kEpsWeno + b2));
+  const double is = rcpd(z0 + z1 + z2);
+  const double f0 = kC1_6 * (11 * fg0 - 7 * fgp1 + 2 * fgp2);
+  const double f1 = kC1_6 * (2 * fgm1 + 5 * fg0 - fgp1);
+  const double f2 = kC1_6 * (-fgm2 + 5 * fgm1 + 2 * fg0);
+  return (z0 * is) * f0 + (z1 * is) * f1 + (z2 * is) * f2;
+}
+__device__ inline double wenop5d(double fgm2, double fgm1, double fg0, double fgp1, double fgp2) {
+  const double b0 = kC13_12 * sq(fg0 - 2 * fgp1 + fgp2) + 0.25 * sq(3 * fg0 - 4 * fgp1 + fgp2);
+  const double b1 = kC13_12 * sq(fgm1 - 2 * fg0 + fgp1) + 0.25 * sq(fgm1 - fgp1);
+  const double b2 = kC13_12 * sq(fgm2 - 2 * fgm1 + fg0) + 0.25 * sq(fgm2 - 4 * fgm1 + 3 * fg0);
+  const double z0 = kW03 * rcpd(sq(kEpsWeno + b0));
+  const double z1 = kW06 * rcpd(sq(kEpsWeno + b1));
+  const double z2 = kW01 * rcpd(sq(kEpsWeno + b2));
+  const double is = rcpd(z0 + z1 + z2);
+  const double f0 = kC1_6 * (2 * fg0 + 5 * fgp1 - fgp2);
+  const double f1 = kC1_6 * (-fgm1 + 5 * fg0 + 2 * fgp1);
+  const double f2 = kC1_6 * (2 * fgm2 - 7 * fgm1 + 11 * fg0);
+  return (z0 * is) * f0 + (z1 * is) * f1 + (z2 * is) * f2;
+}
+__device__ inline double weno_dir64(const double* F, const double* W, double alpha) {
+  double fp[7], fm[7];
+#pragma unroll
+  for (int m = 0; m < 7; m++) {
+    fp[m] = 0.5 * (F[m] + alpha * W[m]);
+    fm[m] = 0.5 * (F[m] - alpha * W[m]);
+  }
+  return wenop5d(fp[0], fp[1], fp[2], fp[3], fp[4]) + wenom5d(fm[1], fm[2], fm[3], fm[4], fm[5]) -
+         wenop5d(fp[1], fp[2], fp[3], fp[4], fp[5]) - wenom5d(fm[2], fm[3], fm[4], fm[5], fm[6]);
+}
+
 // LDS of one tile: raw perturbations, derived quantities, per-row ambient.
 template <int BX, int BY>
 struct Lds32 {
@@ -136,13 +193,7 @@ struct Lds32 {
   float w[5][TN];  // rho', rho*u, rho*v, (rho*s)', (rho*T_n)'
   float p[TN], T[TN];
   float rho[TN], u[TN], v[TN], s[TN], tn[TN];  // full rho, u, v; s', T_n'
-  // per staged row: ambient values, and FP64 differences rounded once
-  float aS[TH], aTn[TH], aT[TH], aD[TH], aRho[TH], cy2[TH], TmTn[TH];
-  float dA[3][TH];  // rho, rho*s, rho*T_n relative to the tile's first staged row
-  float dP[TH];     // p relative to the first staged row
-  float d1Rho[TH];  // rho(r+1) - rho(r)
-  float d1T[TH];    // T(r+1) - T(r-1)
-  float d2T[TH];    // T(r-1) - 2 T(r) + T(r+1)
+  RowF row[TH];  // per staged row
 };
 
 template <int BX, int BY>
@@ -152,8 +203,8 @@ __device__ inline float flux_x(const Lds32<BX, BY>& L, int e, int r, int k) {
     case 0: return ru;
     case 1: return ru * L.u[e] + L.p[e];
     case 2: return ru * L.v[e];
-    case 3: return ru * (L.aS[r] + L.s[e]);
-    default: return ru * (L.aTn[r] + L.tn[e]);
+    case 3: return ru * (L.row[r].aS + L.s[e]);
+    default: return ru * (L.row[r].aTn + L.tn[e]);
   }
 }
 template <int BX, int BY>
@@ -162,9 +213,9 @@ __device__ inline float flux_y(const Lds32<BX, BY>& L, int e, int r, int k) {
   switch (k) {
     case 0: return rv;
     case 1: return rv * L.u[e];
-    case 2: return rv * L.v[e] + (L.p[e] + L.dP[r]);
-    case 3: return rv * (L.aS[r] + L.s[e]);
-    default: return rv * (L.aTn[r] + L.tn[e]);
+    case 2: return rv * L.v[e] + (L.p[e] + L.row[r].pA);
+    case 3: return rv * (L.row[r].aS + L.s[e]);
+    default: return rv * (L.row[r].aTn + L.tn[e]);
   }
 }
 
@@ -182,28 +233,13 @@ __global__ void __launch_bounds__(BX* BY) stage32_kernel(const int4* __restrict_
   const int i0 = tl.y, j0 = tl.z;
   const int tid = threadIdx.x + threadIdx.y * BX;
   const int jr = j0 - 3;  // local row of staged row 0 (>= -g since g >= 3)
-  // per-row ambient, FP64 differences rounded once
-  for (int r = tid; r < TH; r += BX * BY) {
-    const int j = jr + r;
-    if (j >= -a.g && j < a.ny + a.g) {
-      const double* rec = a.amb + (long)(j + a.g) * kRec;
-      const double* ref = a.amb + (long)(jr + a.g) * kRec;
-      L.aRho[r] = (float)rec[0];
-      L.aS[r] = (float)rec[5];
-      L.aTn[r] = (float)rec[6];
-      L.aT[r] = (float)rec[4];
-      L.aD[r] = (float)(rec[0] - a.rho0);
-      L.cy2[r] = (float)rec[7];
-      L.TmTn[r] = (float)(rec[4] - rec[6]);
-      L.dA[0][r] = (float)(rec[0] - ref[0]);
-      L.dA[1][r] = (float)(rec[1] - ref[1]);
-      L.dA[2][r] = (float)(rec[2] - ref[2]);
-      L.dP[r] = (float)(rec[3] - ref[3]);
-      const bool up = j + 1 < a.ny + a.g, dn = j - 1 >= -a.g;
-      L.d1Rho[r] = up ? (float)(rec[kRec + 0] - rec[0]) : 0.0f;
-      L.d1T[r] = (up && dn) ? (float)(rec[kRec + 4] - rec[-kRec + 4]) : 0.0f;
-      L.d2T[r] = (up && dn) ? (float)((rec[-kRec + 4] - 2 * rec[4]) + rec[kRec + 4]) : 0.0f;
-    }
+  // per-row ambient of the staged rows inside the ghosted array
+  {
+    constexpr int W = sizeof(RowF) / 4;
+    const int r_hi = min(TH, a.ny + a.g - jr);
+    const unsigned* src = (const unsigned*)(a.rows + (jr + a.g));
+    unsigned* dst = (unsigned*)L.row;
+    for (int q = tid; q < r_hi * W; q += BX * BY) dst[q] = src[q];
   }
   __syncthreads();
   for (int e = tid; e < TN; e += BX * BY) {
@@ -216,13 +252,13 @@ __global__ void __launch_bounds__(BX* BY) stage32_kernel(const int4* __restrict_
       for (int k = 0; k < 5; k++) L.w[k][e] = w[k] = a.in[gx + k * a.sk];
       L.p[e] = a.in[gx + 5 * a.sk];
       L.T[e] = a.in[gx + 6 * a.sk];
-      const float rho = L.aRho[r] + w[0];
+      const float rho = L.row[r].aRho + w[0];
       const float ir = rcpf(rho);
       L.rho[e] = rho;
       L.u[e] = w[1] * ir;
       L.v[e] = w[2] * ir;
-      L.s[e] = (w[3] - L.aS[r] * w[0]) * ir;
-      L.tn[e] = (w[4] - L.aTn[r] * w[0]) * ir;
+      L.s[e] = (w[3] - L.row[r].aS * w[0]) * ir;
+      L.tn[e] = (w[4] - L.row[r].aTn * w[0]) * ir;
     }
   }
   __syncthreads();
@@ -255,13 +291,13 @@ __global__ void __launch_bounds__(BX* BY) stage32_kernel(const int4* __restrict_
     const float v_prime_y = (V[e + S] - V[e - S]) * a.inv_2dy;
     const float v_prime_x = (V[e + 1] - V[e - 1]) * a.inv_2dx;
     const float t_prime_x = (Tp[e + 1] - Tp[e - 1]) * a.inv_2dx;
-    const float t_prime_y = ((Tp[e + S] - Tp[e - S]) + L.d1T[r]) * a.inv_2dy;
+    const float t_prime_y = ((Tp[e + S] - Tp[e - S]) + L.row[r].d1T) * a.inv_2dy;
     const float t_dprime_x = (Tp[e - 1] - 2 * Tp[e] + Tp[e + 1]) * a.inv_dx2;
-    const float t_dprime_y = ((Tp[e - S] - 2 * Tp[e] + Tp[e + S]) + L.d2T[r]) * a.inv_dy2;
+    const float t_dprime_y = ((Tp[e - S] - 2 * Tp[e] + Tp[e + S]) + L.row[r].d2T) * a.inv_dy2;
     const float w0 = L.rho[e];
-    const float temp = L.aT[r] + Tp[e];
-    const float T_n = L.aTn[r] + L.tn[e];
-    const float dTn = (Tp[e] - L.tn[e]) + L.TmTn[r];  // T - T_n
+    const float temp = L.row[r].aT + Tp[e];
+    const float T_n = L.row[r].aTn + L.tn[e];
+    const float dTn = (Tp[e] - L.tn[e]) + L.row[r].TmTn;  // T - T_n
     const float itemp = rcpf(temp), iTn = rcpf(T_n);
     const float Mn = dTn * a.inv_tau_n;
     const float tmp1 = a.T_star_n * iTn;
@@ -289,8 +325,8 @@ __global__ void __launch_bounds__(BX* BY) stage32_kernel(const int4* __restrict_
       float t1 = R0[e + q] - R0[e - 1 + q], t2 = R0[e + 1 + q] - R0[e + q];
       float num = 2 * fabsf(t1 * t2) + a.ep, den = sqf(t1) + sqf(t2) + a.ep;
       if (num / den < 0.9999f) weno = true;
-      t1 = (R0[e + q * S] - R0[e + (q - 1) * S]) + L.d1Rho[r + q - 1];
-      t2 = (R0[e + (q + 1) * S] - R0[e + q * S]) + L.d1Rho[r + q];
+      t1 = (R0[e + q * S] - R0[e + (q - 1) * S]) + L.row[r + q - 1].d1Rho;
+      t2 = (R0[e + (q + 1) * S] - R0[e + q * S]) + L.row[r + q].d1Rho;
       num = 2 * fabsf(t1 * t2) + a.ep;
       den = sqf(t1) + sqf(t2) + a.ep;
       if (num / den < 0.9999f) weno = true;
@@ -329,14 +365,26 @@ __global__ void __launch_bounds__(BX* BY) stage32_kernel(const int4* __restrict_
         W[m] = L.w[k][e + m - 3];
       }
       const float x = weno_dir32(F, W, af);
+      float y;
+      if (k == 1 || k == 2) {
+        // momenta: no ambient, FP32
 #pragma unroll
-      for (int m = 0; m < 7; m++) {
-        const int rm = r + m - 3;
-        F[m] = flux_y(L, e + (m - 3) * S, rm, k);
-        const float dA = k == 0 ? L.dA[0][rm] : k == 3 ? L.dA[1][rm] : k == 4 ? L.dA[2][rm] : 0.0f;
-        W[m] = L.w[k][e + (m - 3) * S] + dA;
+        for (int m = 0; m < 7; m++) {
+          F[m] = flux_y(L, e + (m - 3) * S, r + m - 3, k);
+          W[m] = L.w[k][e + (m - 3) * S];
+        }
+        y = weno_dir32(F, W, ag);
+      } else {
+        // full values: FP64 ambient row + FP32 perturbation
+        const int c = k == 0 ? 0 : k - 2;
+        double Fd[7], Wd[7];
+#pragma unroll
+        for (int m = 0; m < 7; m++) {
+          Fd[m] = (double)flux_y(L, e + (m - 3) * S, r + m - 3, k);
+          Wd[m] = L.row[r + m - 3].aW[c] + (double)L.w[k][e + (m - 3) * S];
+        }
+        y = (float)weno_dir64(Fd, Wd, (double)ag);
       }
-      const float y = weno_dir32(F, W, ag);
       const float kk = x * a.inv_dx + y * a.inv_dy + H[k];
       K[0] = k == 0 ? kk : K[0];
       K[1] = k == 1 ? kk : K[1];
@@ -362,10 +410,10 @@ __global__ void __launch_bounds__(BX* BY) stage32_kernel(const int4* __restrict_
     }
     // ---- update_state (update_state.m4:87-107) in perturbation form
     const float rp = (float)wn[0], rsp = (float)wn[3];
-    const float pp = L.cy2[r] * (rp * (1.0f + a.a_eos * (2.0f * L.aD[r] + rp)) + rsp * a.inv_cp);
-    const float rho = L.aRho[r] + rp;
-    const float sp = (rsp - L.aS[r] * rp) * rcpf(rho);
-    const float Tpn = L.aT[r] * expm1f((sp + a.R * log1pf(rp * rcpf(L.aRho[r]))) * a.inv_cv);
+    const float pp = L.row[r].cy2 * (rp * (1.0f + a.a_eos * (2.0f * L.row[r].aD + rp)) + rsp * a.inv_cp);
+    const float rho = L.row[r].aRho + rp;
+    const float sp = (rsp - L.row[r].aS * rp) * rcpf(rho);
+    const float Tpn = L.row[r].aT * expm1f((sp + a.R * log1pf(rp * rcpf(L.row[r].aRho))) * a.inv_cv);
     a.out[gx + 5 * a.sk] = pp;
     a.out[gx + 6 * a.sk] = Tpn;
   }
@@ -528,6 +576,8 @@ struct Fp32Form {
   float* pool32 = nullptr;
   double* wc = nullptr;
   double* amb = nullptr;
+  RowF* rows = nullptr;
+  double rho0 = 0, p0 = 0;  // of the gas the ambient was set with
   Patch32* dp = nullptr;
   Args32* d_args = nullptr;  // kMaxStages x npatch
   Args32* d_eval = nullptr;  // npatch
@@ -604,6 +654,7 @@ Args32 base32(Batch* b, int p, const enl_gas_t* gas) {
   memset(&a, 0, sizeof a);
   a.h0 = b->pool + pd.src;
   a.amb = F->amb + q.amb * kRec;
+  a.rows = F->rows + q.amb;
   a.wc = F->wc + q.wc;
   a.dk = F->pool32 + q.dk;
   a.nx = pd.nx;
@@ -697,6 +748,7 @@ int setup32(Batch* b) {
   int rc = alloc(F, &F->pool32, (size_t)off32, "hipMalloc(FP32 pool)");
   if (!rc) rc = alloc(F, &F->wc, (size_t)off64, "hipMalloc(FP32 carried state)");
   if (!rc) rc = alloc(F, &F->amb, (size_t)amb_total * kRec, "hipMalloc(ambient)");
+  if (!rc) rc = alloc(F, &F->rows, (size_t)amb_total, "hipMalloc(ambient rows)");
   if (!rc) rc = alloc(F, &F->dp, (size_t)b->npatch, "hipMalloc(FP32 patch table)");
   if (!rc) rc = alloc(F, &F->d_args, (size_t)kMaxStages * b->npatch, "hipMalloc(FP32 arguments)");
   if (!rc) rc = alloc(F, &F->d_eval, (size_t)b->npatch, "hipMalloc(FP32 eval arguments)");
@@ -707,6 +759,34 @@ int setup32(Batch* b) {
               : check_hip(hipMemcpy(F->amb + amb_base[l] * kRec, F->amb_h[l].data(),
                                     sizeof(double) * F->amb_h[l].size(), hipMemcpyHostToDevice),
                           "ambient upload");
+  if (!rc) {
+    std::vector<RowF> rows((size_t)amb_total);
+    for (int l = 0; l < kMaxLevels; l++) {
+      const long nrec = (long)F->amb_h[l].size() / kRec;
+      for (long q = 0; q < nrec; q++) {
+        const double* rec = &F->amb_h[l][(size_t)q * kRec];
+        const bool up = q + 1 < nrec, dn = q > 0;
+        RowF& R = rows[(size_t)(amb_base[l] + q)];
+        memset(&R, 0, sizeof R);
+        R.aW[0] = rec[0];
+        R.aW[1] = rec[1];
+        R.aW[2] = rec[2];
+        R.aRho = (float)rec[0];
+        R.aS = (float)rec[5];
+        R.aTn = (float)rec[6];
+        R.aT = (float)rec[4];
+        R.aD = (float)(rec[0] - F->rho0);
+        R.cy2 = (float)rec[7];
+        R.TmTn = (float)(rec[4] - rec[6]);
+        R.pA = (float)(rec[3] - F->p0);
+        R.d1Rho = up ? (float)(rec[kRec + 0] - rec[0]) : 0.0f;
+        R.d1T = (up && dn) ? (float)(rec[kRec + 4] - rec[-kRec + 4]) : 0.0f;
+        R.d2T = (up && dn) ? (float)((rec[-kRec + 4] - 2 * rec[4]) + rec[kRec + 4]) : 0.0f;
+      }
+    }
+    rc = check_hip(hipMemcpy(F->rows, rows.data(), sizeof(RowF) * rows.size(), hipMemcpyHostToDevice),
+                   "ambient row upload");
+  }
   if (!rc)
     rc = check_hip(hipMemcpy(F->dp, F->hp.data(), sizeof(Patch32) * F->hp.size(), hipMemcpyHostToDevice),
                    "FP32 patch table upload");
@@ -856,6 +936,8 @@ int enl_batch_set_ambient(void* handle, int level, int row_lo, int nrows, const 
   const enl_patch_desc_t& d = b->desc[ref];
   Fp32Form* F = form(b);
   F->ready = false;
+  F->rho0 = gas->rho0;
+  F->p0 = gas->p0;
   F->amb_lo[level] = row_lo;
   std::vector<double>& A = F->amb_h[level];
   A.assign((size_t)nrows * kRec, 0.0);

@@ -110,20 +110,26 @@ class DeviceHierarchy:
         else:
             _lib.check(self.lib.enl_batch_set_level_tiles(self.handle, level, bx, by))
 
-    def autotune(self, dt, states, wl, steps=2, variant="hybrid_high_llf", rk=RK3, timer=None):
-        """Tile shape per level by coordinate descent over measured graph
-        steps: each level in turn tries every shape with the others fixed
-        and keeps the fastest.  Reloads `states` before each trial; returns
-        {level: (bx, by)} and the per-trial times (ms per step)."""
+    def autotune(self, dt, states, wl, steps=2, variant="hybrid_high_llf", rk=RK3, mode="graph"):
+        """Tile shape per level by coordinate descent over measured steps of
+        `mode` ("graph" or "fp32_graph"): each level in turn tries every shape
+        with the others fixed and keeps the fastest.  Reloads `states` before
+        each trial; returns {level: (bx, by)} and the per-trial times (ms per
+        step)."""
         import time
 
+        def reload():
+            self.load_initial(wl, states)
+            if mode.startswith("fp32"):
+                self.fp32_load()
+
         def measure():
-            self.load_initial(wl, states)
-            self.advance(1, dt, mode="graph", variant=variant, rk=rk)  # capture
-            self.load_initial(wl, states)
+            reload()
+            self.advance(1, dt, mode=mode, variant=variant, rk=rk)  # capture
+            reload()
             self.sync()
             t0 = time.perf_counter()
-            self.advance(steps, dt, mode="graph", variant=variant, rk=rk)
+            self.advance(steps, dt, mode=mode, variant=variant, rk=rk)
             self.sync()
             return (time.perf_counter() - t0) * 1e3 / steps
 
