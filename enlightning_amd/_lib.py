@@ -67,8 +67,21 @@ PROTOTYPES = {
     "enl_batch_set_coarsen": ([_P, _I] + [_P] * 4, _I),
     "enl_batch_set_tiles": ([_P, _I, _I], _I),
     "enl_batch_set_level_tiles": ([_P, _I, _I, _I], _I),
-    "enl_batch_set_ambient": ([_P, _I, _I, _I, _P, _P, _P, _P], _I),
-    "enl_batch_fp32_prepare": ([_P, _I, _P, _P], _I),
+    "enl_switch_create": ([_P], _I),
+    "enl_switch_destroy": ([_P], _I),
+    "enl_switch_set_params": ([_P, _P], _I),
+    "enl_switch_get_params": ([_P, _P], _I),
+    "enl_switch_features": ([_I, _I, _I, _I, _I, _P, _P, ctypes.POINTER(EnlGas), _P, _I, _P], _I),
+    "enl_switch_forward": ([_P, _P, _I, _P], _I),
+    "enl_switch_gradients": ([_P, _P, _P, _P, _I, ctypes.c_float, _P, _P], _I),
+    "enl_switch_train_step": ([_P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float, _P], _I),
+    "enl_switch_decide": ([_P, _I, _I, _I, _I, _I, _P, _P, ctypes.POINTER(EnlGas), ctypes.c_float, ctypes.c_float, _P,
+                           _P], _I),
+    "enl_advance_patch_learned": ([_P, ctypes.c_float, ctypes.c_float, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P,
+                                   ctypes.POINTER(EnlGas), _P, _I, _P, _P, _P, ctypes.c_double, _I, _P, _P, _P, _P,
+                                   _P, ctypes.c_size_t], _I),
+    "enl_batch_set_ambient": ([_P, _I, _I, _I, _P, _P, _P, ctypes.POINTER(EnlGas)], _I),
+    "enl_batch_fp32_prepare": ([_P, _I, ctypes.POINTER(EnlGas), _P], _I),
     "enl_batch_fp32_load": ([_P], _I),
     "enl_batch_fp32_store": ([_P], _I),
     "enl_batch_patch_state": ([_P, _I] + [_P] * 4, _I),

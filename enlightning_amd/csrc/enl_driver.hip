@@ -207,11 +207,58 @@ extern "C" size_t enl_fused_scratch_bytes(int ifirst0, int ilast0, int ifirst1, 
   return sizeof(double) * (size_t)(2 * 7 * G.sk + 5L * G.nx * G.ny);
 }
 
+namespace {
+// The learned switch of a fused advance, when one is given.
+struct LearnedSwitch {
+  Switch* sw = nullptr;
+  float lo = 0, hi = 1;
+  unsigned char* decide = nullptr;
+  unsigned long long* totals = nullptr;
+};
+
+int advance_fused(const LearnedSwitch& L, int variant, int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw,
+                  const double* xlo, const double* dx, const enl_gas_t* gas, const int* reflect, int order,
+                  const double* alpha_1, const double* alpha_2, const double* beta, double dt, int nsteps, int level,
+                  double* w_n, double* pres, double* temp, double* source, void* scratch, size_t scratch_bytes);
+}  // namespace
+
 extern "C" int enl_advance_patch_fused(int variant, int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw,
                                        const double* xlo, const double* dx, const enl_gas_t* gas, const int* reflect,
                                        int order, const double* alpha_1, const double* alpha_2, const double* beta,
                                        double dt, int nsteps, int level, double* w_n, double* pres, double* temp,
                                        double* source, void* scratch, size_t scratch_bytes) {
+  return advance_fused(LearnedSwitch(), variant, ifirst0, ilast0, ifirst1, ilast1, gcw, xlo, dx, gas, reflect, order,
+                       alpha_1, alpha_2, beta, dt, nsteps, level, w_n, pres, temp, source, scratch, scratch_bytes);
+}
+
+extern "C" int enl_advance_patch_learned(void* sw, float lo, float hi, unsigned char* decide,
+                                         unsigned long long* totals, int variant, int ifirst0, int ilast0,
+                                         int ifirst1, int ilast1, int gcw, const double* xlo, const double* dx,
+                                         const enl_gas_t* gas, const int* reflect, int order, const double* alpha_1,
+                                         const double* alpha_2, const double* beta, double dt, int nsteps,
+                                         double* w_n, double* pres, double* temp, double* source, void* scratch,
+                                         size_t scratch_bytes) {
+  if (!sw || !decide) return set_error(-2, "enl_advance_patch_learned: no switch or decision map");
+  if (!(variant == kHybridHighLlf || variant == kHybridHigh || variant == kHybridLowLlf || variant == kHybridLow))
+    return set_error(-1, "the learned switch replaces the sensor of the hybrid variants only");
+  if (!(lo <= hi)) return set_error(-2, "enl_advance_patch_learned: lo > hi");
+  if (!is_device_ptr(decide) || (totals && !is_device_ptr(totals)))
+    return set_error(-4, "enl_advance_patch_learned needs device decision and count arrays");
+  LearnedSwitch L;
+  L.sw = (Switch*)sw;
+  L.lo = lo;
+  L.hi = hi;
+  L.decide = decide;
+  L.totals = totals;
+  return advance_fused(L, variant, ifirst0, ilast0, ifirst1, ilast1, gcw, xlo, dx, gas, reflect, order, alpha_1,
+                       alpha_2, beta, dt, nsteps, 0, w_n, pres, temp, source, scratch, scratch_bytes);
+}
+
+namespace {
+int advance_fused(const LearnedSwitch& L, int variant, int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw,
+                  const double* xlo, const double* dx, const enl_gas_t* gas, const int* reflect, int order,
+                  const double* alpha_1, const double* alpha_2, const double* beta, double dt, int nsteps, int level,
+                  double* w_n, double* pres, double* temp, double* source, void* scratch, size_t scratch_bytes) {
   if (variant < 0 || variant >= kNumVariants) return set_error(-1, "unknown rhs variant");
   int need = (variant == kWenoLowLlf || variant == kWenoLow) ? 2 : 3;
   if (gcw < need) return set_error(-2, "ghost width below the stencil reach");
@@ -305,6 +352,13 @@ extern "C" int enl_advance_patch_fused(int variant, int ifirst0, int ilast0, int
       ra.alpha_1 = alpha_1[stage];
       ra.alpha_2 = alpha_2[stage];
       ra.bdt = beta[stage] * dt;
+      if (L.sw) {
+        // the learned switch decides level 0's cells from the stage input
+        int rc = switch_decide(L.sw, G, in.w, in.p, gas, L.lo, L.hi, L.decide, s);
+        if (!rc && L.totals) rc = switch_accumulate(L.sw, L.totals, s);
+        if (rc) return rc;
+        ra.decide = L.decide;
+      }
       t = g_timer.begin(ENL_KERNEL_STAGE, s);
       e = launch_stage(variant, ra, s);
       g_timer.end(t, s);
@@ -319,3 +373,4 @@ extern "C" int enl_advance_patch_fused(int variant, int ifirst0, int ilast0, int
   }
   return e == hipSuccess ? 0 : check_hip(e, "fused patch loop launch");
 }
+}  // namespace

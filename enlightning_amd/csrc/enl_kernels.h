@@ -2,6 +2,8 @@
 // update path (internal C++ API; the exported C ABI sits in enl_abi.hip).
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include "enlightning_hip.h"
 #include <stdint.h>
 
 namespace enl {
@@ -34,6 +36,9 @@ struct RhsArgs {
   long h0_sy;        // nx for an interior array, sy for a ghosted one
   double* KK;        // interior, 5 components
   int* switch_map;   // optional: interior int map of the DRP(0)/WENO(1) choice
+  // optional, level 0 of the hybrid variants: interior map of a switch
+  // decided elsewhere (0 DRP, 1 WENO, 2 undecided: the sensor decides)
+  const unsigned char* decide = nullptr;
   Geom geom;
   double dx, dy, ylo;
   double alpha, kappa, mu, muB, R, T_star_n, tau_n, c, c_lr;
@@ -106,6 +111,13 @@ hipError_t launch_copy_interior(double* ww, const double* w, const Geom& g, hipS
 
 // Library runtime state shared by the exported entry points (enl_abi.hip).
 hipStream_t lib_stream();
+
+// The learned switch (enl_switch.hip): decisions of one patch state (0 DRP,
+// 1 WENO, 2 undecided) and the running counts of its decisions.
+struct Switch;
+int switch_decide(Switch* sw, const Geom& G, const double* w_n, const double* pres, const enl_gas_t* gas, float lo,
+                  float hi, unsigned char* decide, hipStream_t s);
+int switch_accumulate(Switch* sw, unsigned long long* totals, hipStream_t s);
 int set_error(int code, const char* fmt, ...);
 int check_hip(hipError_t e, const char* what);
 bool is_device_ptr(const void* p);

@@ -367,9 +367,10 @@ __device__ inline void split_body(const RhsArgs& a, int i0, int j0, unsigned pid
     const long ik = (long)G.nx * G.ny;
     double H[5];
     cell_source<Sh::TW, P>(t, e, a.H0[(long)i + (long)j * a.h0_sy], a, H);
-    if constexpr (Sc::hybrid)
-      tag = cell_sensor<Sh::TW>(t.w0, e, a.ep, a.level);
-    else
+    if constexpr (Sc::hybrid) {
+      const int d = a.decide ? (int)a.decide[icell] : 2;
+      tag = d == 2 ? cell_sensor<Sh::TW>(t.w0, e, a.ep, a.level) : d;
+    } else
       tag = Sc::drp ? 0 : 1;
     if (tag == 0) {
       double out[5];

@@ -1,0 +1,599 @@
+// enl_switch.hip -- a learned stand-in for the hybrid scheme's density
+// sensor (rhs_hybrid_high_llf.m4:239-256), which picks DRP or WENO per cell
+// on level 0.
+//
+// Per RK stage and patch:
+//   1. a prefilter marks as DRP every cell whose sensor pairs all satisfy
+//      t1^2 + t2^2 < 5e-5*eps: there 1 - r <= (t1^2+t2^2)/(t1^2+t2^2+eps)
+//      < 5e-5, so r > 0.9999f and the sensor itself says DRP.  The other
+//      cells (candidates) are compacted into a list;
+//   2. the network runs on the candidates only: a 5x5 window of four
+//      normalised differences (x and y differences of rho and p, each
+//      asinh(d / scale) with scale sqrt(eps) for rho and c^2 sqrt(eps) for
+//      p) -> MLP 100-64-32-1 (ReLU, sigmoid), BF16 inputs and weights, FP32
+//      accumulation.  p > hi -> WENO, p < lo -> DRP, otherwise the plain
+//      sensor decides (the split kernel evaluates it for those cells);
+//   3. the split kernel reads the decisions and compacts the WENO cells as
+//      before.
+// Training runs on the device too: forward/backward per sample, weight
+// gradients summed per parameter in a fixed order, Adam on FP32 master
+// weights, BF16 copies for the forward pass.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "enl_device.h"
+#include "enl_kernels.h"
+#include "enlightning_hip.h"
+
+// The switch is not part of the reference arithmetic: multiply-adds fuse.
+#pragma clang fp contract(fast)
+
+namespace enl {
+namespace {
+
+constexpr int kIn = 100, kH1 = 64, kH2 = 32;
+constexpr int oW1 = 0;                   // [kH1][kIn]
+constexpr int oB1 = oW1 + kH1 * kIn;     // 6400
+constexpr int oW2 = oB1 + kH1;           // [kH2][kH1], 6464
+constexpr int oB2 = oW2 + kH2 * kH1;     // 8512
+constexpr int oW3 = oB2 + kH2;           // [kH2], 8544
+constexpr int oB3 = oW3 + kH2;           // 8576
+constexpr int kNParam = oB3 + 1;         // 8577
+static_assert(kNParam == ENL_SWITCH_NPARAM, "parameter count");
+
+// The forward pass reads the parameters from LDS in this order:
+// W1 transposed [kIn][kH1], b1, W2 transposed [kH1][kH2], b2, w3, b3.
+constexpr int lW1 = 0, lB1 = kIn * kH1, lW2 = lB1 + kH1, lB2 = lW2 + kH1 * kH2, lW3 = lB2 + kH2, lB3 = lW3 + kH2;
+
+__host__ __device__ inline unsigned short f2bf(float f) {
+  unsigned u;
+  memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+  return (unsigned short)(u >> 16);
+}
+__host__ __device__ inline float bf2f(unsigned short h) {
+  unsigned u = (unsigned)h << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+// LDS image of the BF16 parameters (as FP32), transposed for broadcast reads.
+__device__ inline void load_params(float* lp, const unsigned short* pb) {
+  for (int q = threadIdx.x; q < kNParam; q += blockDim.x) {
+    float v = bf2f(pb[q]);
+    int dst;
+    if (q < oB1) {
+      const int n = q / kIn, k = q % kIn;
+      dst = lW1 + k * kH1 + n;
+    } else if (q < oW2) {
+      dst = lB1 + (q - oB1);
+    } else if (q < oB2) {
+      const int n = (q - oW2) / kH1, m = (q - oW2) % kH1;
+      dst = lW2 + m * kH2 + n;
+    } else if (q < oW3) {
+      dst = lB2 + (q - oB2);
+    } else if (q < oB3) {
+      dst = lW3 + (q - oW3);
+    } else {
+      dst = lB3;
+    }
+    lp[dst] = v;
+  }
+}
+
+struct Feat {
+  const double* r;  // rho: ghosted w_n component 0
+  const double* p;  // ghosted pressure
+  long sy;
+  float inv_r, inv_p;  // 1/sqrt(eps), 1/(c^2 sqrt(eps))
+};
+
+// The four features of window point (i+di, j+dj): x/y differences of rho and
+// p, asinh-normalised and rounded to BF16.  g is the ghosted index of (i, j).
+__device__ inline void point_features(const Feat& F, long g, int di, int dj, float x[4]) {
+  const long q = g + di + (long)dj * F.sy;
+  const double r0 = F.r[q], p0 = F.p[q];
+  x[0] = bf2f(f2bf(asinhf((float)(F.r[q + 1] - r0) * F.inv_r)));
+  x[1] = bf2f(f2bf(asinhf((float)(F.r[q + F.sy] - r0) * F.inv_r)));
+  x[2] = bf2f(f2bf(asinhf((float)(F.p[q + 1] - p0) * F.inv_p)));
+  x[3] = bf2f(f2bf(asinhf((float)(F.p[q + F.sy] - p0) * F.inv_p)));
+}
+
+// MLP on one sample whose BF16 features are at xb (LDS or global); h1 is
+// the lane's LDS slot for the first hidden layer.  Layer 1 runs in four
+// chunks of 16 units over a rolled input loop and layer 2 over a rolled
+// loop on h1, so the live registers stay low.  Returns the logit; h2
+// receives the second hidden layer.
+constexpr int kLanes = 64;      // one wave per block
+constexpr int kH1Pad = kH1 + 1;  // LDS row of h1 (odd stride: no bank conflicts)
+
+__device__ inline float mlp_forward(const float* lp, const unsigned short* xb, float* h1, float h2[kH2]) {
+#pragma unroll
+  for (int c = 0; c < kH1 / 16; c++) {
+    float a[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) a[j] = lp[lB1 + c * 16 + j];
+#pragma unroll 2
+    for (int k = 0; k < kIn; k++) {
+      const float xk = bf2f(xb[k]);
+      const float4* w = (const float4*)(lp + lW1 + k * kH1 + c * 16);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const float4 wq = w[q];
+        a[4 * q + 0] += wq.x * xk;
+        a[4 * q + 1] += wq.y * xk;
+        a[4 * q + 2] += wq.z * xk;
+        a[4 * q + 3] += wq.w * xk;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j++) h1[c * 16 + j] = a[j] > 0.0f ? a[j] : 0.0f;
+  }
+  float a2[kH2];
+#pragma unroll
+  for (int n = 0; n < kH2; n++) a2[n] = lp[lB2 + n];
+#pragma unroll 2
+  for (int m = 0; m < kH1; m++) {
+    const float h = h1[m];
+    const float4* w = (const float4*)(lp + lW2 + m * kH2);
+#pragma unroll
+    for (int q = 0; q < kH2 / 4; q++) {
+      const float4 wq = w[q];
+      a2[4 * q + 0] += wq.x * h;
+      a2[4 * q + 1] += wq.y * h;
+      a2[4 * q + 2] += wq.z * h;
+      a2[4 * q + 3] += wq.w * h;
+    }
+  }
+  float z = lp[lB3];
+#pragma unroll
+  for (int n = 0; n < kH2; n++) {
+    h2[n] = a2[n] > 0.0f ? a2[n] : 0.0f;
+    z += lp[lW3 + n] * h2[n];
+  }
+  return z;
+}
+
+__device__ inline float sigmoid(float z) { return 1.0f / (1.0f + __expf(-z)); }
+
+// ---- per-stage decisions -----------------------------------------------------
+
+struct DecideArgs {
+  Feat F;
+  int nx, ny, g;
+  double thr;  // 5e-5 * eps
+  unsigned char* decide;
+  unsigned* cand;    // candidate cells (interior index)
+  unsigned* ncand;   // [0] candidates, [1] WENO, [2] undecided
+  float lo, hi;
+};
+
+__global__ void decide_reset_kernel(unsigned* n) {
+  if (threadIdx.x < 3) n[threadIdx.x] = 0;
+}
+
+// Prefilter: DRP where the sensor provably says DRP, else a candidate.
+__global__ void __launch_bounds__(256) prefilter_kernel(DecideArgs a) {
+  const long ncell = (long)a.nx * a.ny;
+  for (long c0 = (long)blockIdx.x * blockDim.x; c0 < ncell; c0 += (long)gridDim.x * blockDim.x) {
+    const long c = c0 + threadIdx.x;
+    bool cand = false;
+    if (c < ncell) {
+      const int i = (int)(c % a.nx), j = (int)(c / a.nx);
+      const long g = (long)(i + a.g) + (long)(j + a.g) * a.F.sy;
+      const double* r = a.F.r;
+      const long S = a.F.sy;
+#pragma unroll
+      for (int q = -1; q <= 1; q++) {
+        double t1 = r[g + q] - r[g - 1 + q], t2 = r[g + 1 + q] - r[g + q];
+        if (t1 * t1 + t2 * t2 >= a.thr) cand = true;
+        t1 = r[g + q * S] - r[g + (q - 1) * S];
+        t2 = r[g + (q + 1) * S] - r[g + q * S];
+        if (t1 * t1 + t2 * t2 >= a.thr) cand = true;
+      }
+      if (!cand) a.decide[c] = 0;
+    }
+    const unsigned long long m = __ballot(cand);
+    if (m) {
+      const int lead = __ffsll((long long)m) - 1;
+      unsigned base = 0;
+      if ((int)(threadIdx.x & 63) == lead) base = atomicAdd(a.ncand, (unsigned)__popcll(m));
+      base = __shfl(base, lead);
+      if (cand) a.cand[base + __popcll(m & __lanemask_lt())] = (unsigned)c;
+    }
+  }
+}
+
+// One wave per block: the wave's features sit in LDS (BF16) next to the
+// parameters.
+constexpr int kInferLanes = kLanes;
+
+__global__ void __launch_bounds__(kInferLanes) infer_kernel(DecideArgs a, const unsigned short* params) {
+  __shared__ float lp[kNParam + 3];
+  __shared__ unsigned short xs[kInferLanes][kIn];
+  __shared__ float h1s[kInferLanes][kH1Pad];
+  load_params(lp, params);
+  __syncthreads();
+  const unsigned n = a.ncand[0];
+  unsigned nweno = 0, nfall = 0;
+  for (unsigned q0 = blockIdx.x * kInferLanes; q0 < n; q0 += gridDim.x * kInferLanes) {
+    const unsigned q = q0 + threadIdx.x;
+    if (q < n) {
+      const unsigned c = a.cand[q];
+      const int i = (int)(c % (unsigned)a.nx), j = (int)(c / (unsigned)a.nx);
+      const long g = (long)(i + a.g) + (long)(j + a.g) * a.F.sy;
+      unsigned short* xb = xs[threadIdx.x];
+      for (int pt = 0; pt < 25; pt++) {
+        float f[4];
+        point_features(a.F, g, pt % 5 - 2, pt / 5 - 2, f);
+        for (int k = 0; k < 4; k++) xb[pt * 4 + k] = f2bf(f[k]);
+      }
+      float h2[kH2];
+      const float p = sigmoid(mlp_forward(lp, xb, h1s[threadIdx.x], h2));
+      const unsigned char d = p > a.hi ? 1 : (p < a.lo ? 0 : 2);
+      a.decide[c] = d;
+      nweno += d == 1;
+      nfall += d == 2;
+    }
+  }
+  // per-wave sums, one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) {
+    nweno += __shfl_down(nweno, off);
+    nfall += __shfl_down(nfall, off);
+  }
+  if (threadIdx.x == 0) {
+    if (nweno) atomicAdd(a.ncand + 1, nweno);
+    if (nfall) atomicAdd(a.ncand + 2, nfall);
+  }
+}
+
+// ---- training ---------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) features_kernel(Feat F, int nx, int g, const int* cells, int n,
+                                                       unsigned short* X) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const int c = cells[q];
+    const int i = c % nx, j = c / nx;
+    const long gi = (long)(i + g) + (long)(j + g) * F.sy;
+    for (int pt = 0; pt < 25; pt++) {
+      float xs[4];
+      point_features(F, gi, pt % 5 - 2, pt / 5 - 2, xs);
+      for (int k = 0; k < 4; k++) X[(long)q * kIn + pt * 4 + k] = f2bf(xs[k]);
+    }
+  }
+}
+
+struct TrainBuf {
+  float *h1, *h2, *dh1, *dh2, *dz, *p;
+};
+
+// Forward and backward of one sample per lane: activations and their
+// gradients for the per-parameter sums; dz = wt*(p - y)/sum(wt) is the
+// gradient of the weighted mean binary cross-entropy with respect to the
+// logit.
+__global__ void __launch_bounds__(kLanes) fwd_bwd_kernel(const unsigned short* params, const unsigned short* X,
+                                                         const float* y, const float* wt, int n, float inv_wsum,
+                                                         TrainBuf t, float* loss) {
+  __shared__ float lp[kNParam + 3];
+  __shared__ float h1s[kLanes][kH1Pad];
+  load_params(lp, params);
+  __syncthreads();
+  float* h1 = h1s[threadIdx.x];
+  float lsum = 0.0f;
+  for (int b0 = blockIdx.x * kLanes; b0 < n; b0 += gridDim.x * kLanes) {
+    const int b = b0 + threadIdx.x;
+    if (b < n) {
+      float h2[kH2];
+      const float z = mlp_forward(lp, X + (long)b * kIn, h1, h2);
+      const float p = sigmoid(z);
+      t.p[b] = p;
+      const float yb = y[b], wb = wt[b] * inv_wsum;
+      // -[y log p + (1-y) log(1-p)] from the logit, stable for large |z|
+      lsum += wb * (fmaxf(z, 0.0f) - z * yb + log1pf(__expf(-fabsf(z))));
+      const float dz = wb * (p - yb);
+      t.dz[b] = dz;
+      float dh2[kH2];
+#pragma unroll
+      for (int m = 0; m < kH2; m++) {
+        dh2[m] = h2[m] > 0.0f ? dz * lp[lW3 + m] : 0.0f;
+        t.h2[(long)b * kH2 + m] = h2[m];
+        t.dh2[(long)b * kH2 + m] = dh2[m];
+      }
+#pragma unroll 2
+      for (int k = 0; k < kH1; k++) {
+        const float4* w = (const float4*)(lp + lW2 + k * kH2);
+        float s = 0.0f;
+#pragma unroll
+        for (int q = 0; q < kH2 / 4; q++) {
+          const float4 wq = w[q];
+          s += wq.x * dh2[4 * q] + wq.y * dh2[4 * q + 1] + wq.z * dh2[4 * q + 2] + wq.w * dh2[4 * q + 3];
+        }
+        t.h1[(long)b * kH1 + k] = h1[k];
+        t.dh1[(long)b * kH1 + k] = h1[k] > 0.0f ? s : 0.0f;
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) lsum += __shfl_down(lsum, off);
+  if (threadIdx.x == 0 && loss) atomicAdd(loss, lsum);
+}
+
+// One lane per parameter: the batch sum of its gradient, in sample order.
+__global__ void __launch_bounds__(256) grad_kernel(const unsigned short* X, int n, TrainBuf t, float* grad) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= kNParam) return;
+  float s = 0.0f;
+  if (q < oB1) {
+    const int nn = q / kIn, k = q % kIn;
+    for (int b = 0; b < n; b++) s += t.dh1[(long)b * kH1 + nn] * bf2f(X[(long)b * kIn + k]);
+  } else if (q < oW2) {
+    for (int b = 0; b < n; b++) s += t.dh1[(long)b * kH1 + (q - oB1)];
+  } else if (q < oB2) {
+    const int nn = (q - oW2) / kH1, m = (q - oW2) % kH1;
+    for (int b = 0; b < n; b++) s += t.dh2[(long)b * kH2 + nn] * t.h1[(long)b * kH1 + m];
+  } else if (q < oW3) {
+    for (int b = 0; b < n; b++) s += t.dh2[(long)b * kH2 + (q - oB2)];
+  } else if (q < oB3) {
+    for (int b = 0; b < n; b++) s += t.dz[b] * t.h2[(long)b * kH2 + (q - oW3)];
+  } else {
+    for (int b = 0; b < n; b++) s += t.dz[b];
+  }
+  grad[q] = s;
+}
+
+__global__ void __launch_bounds__(256) adam_kernel(float* w, float* m, float* v, const float* g, unsigned short* wb,
+                                                   float lr, float b1, float b2, float c1, float c2, float eps) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= kNParam) return;
+  const float gq = g[q];
+  const float mq = b1 * m[q] + (1.0f - b1) * gq;
+  const float vq = b2 * v[q] + (1.0f - b2) * gq * gq;
+  m[q] = mq;
+  v[q] = vq;
+  const float wq = w[q] - lr * (mq * c1) / (sqrtf(vq * c2) + eps);
+  w[q] = wq;
+  wb[q] = f2bf(wq);
+}
+
+__global__ void __launch_bounds__(256) to_bf16_kernel(const float* w, unsigned short* wb) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < kNParam) wb[q] = f2bf(w[q]);
+}
+
+__global__ void __launch_bounds__(kLanes) forward_kernel(const unsigned short* params, const unsigned short* X,
+                                                         int n, float* p) {
+  __shared__ float lp[kNParam + 3];
+  __shared__ float h1s[kLanes][kH1Pad];
+  load_params(lp, params);
+  __syncthreads();
+  for (int b = blockIdx.x * kLanes + threadIdx.x; b < n; b += gridDim.x * kLanes) {
+    float h2[kH2];
+    p[b] = sigmoid(mlp_forward(lp, X + (long)b * kIn, h1s[threadIdx.x], h2));
+  }
+}
+
+__global__ void accumulate_kernel(const unsigned* n, unsigned long long* totals) {
+  if (threadIdx.x < 3) totals[threadIdx.x] += n[threadIdx.x];
+}
+
+unsigned blocks_for(long n, long cap = 2048) {
+  long b = (n + 255) / 256;
+  return (unsigned)(b < 1 ? 1 : (b > cap ? cap : b));
+}
+
+}  // namespace
+
+struct Switch {
+  float* w = nullptr;            // FP32 master parameters
+  unsigned short* wb = nullptr;  // BF16 copy
+  float *m = nullptr, *v = nullptr, *g = nullptr, *loss = nullptr;
+  long step = 0;
+  TrainBuf t{};
+  int cap = 0;               // samples the training buffers hold
+  unsigned* cand = nullptr;  // candidate list of the decisions
+  unsigned* ncand = nullptr;
+  long cand_cap = 0;
+};
+
+namespace {
+
+int grow_train(Switch* s, int n) {
+  if (n <= s->cap) return 0;
+  float* bufs[6] = {s->t.h1, s->t.h2, s->t.dh1, s->t.dh2, s->t.dz, s->t.p};
+  for (float* b : bufs)
+    if (b) (void)hipFree(b);
+  const size_t per[6] = {kH1, kH2, kH1, kH2, 1, 1};
+  float** dst[6] = {&s->t.h1, &s->t.h2, &s->t.dh1, &s->t.dh2, &s->t.dz, &s->t.p};
+  for (int k = 0; k < 6; k++) {
+    hipError_t e = hipMalloc((void**)dst[k], sizeof(float) * per[k] * (size_t)n);
+    if (e != hipSuccess) return check_hip(e, "hipMalloc(switch training buffers)");
+  }
+  s->cap = n;
+  return 0;
+}
+
+Feat make_feat(const Geom& G, const double* w_n, const double* pres, const enl_gas_t* gas) {
+  Feat F;
+  F.r = w_n;
+  F.p = pres;
+  F.sy = G.sy;
+  const double ep = sensor_eps(gas->tag_tolerance[2]);
+  F.inv_r = (float)(1.0 / std::sqrt(ep));
+  F.inv_p = (float)(1.0 / (gas->c * gas->c * std::sqrt(ep)));
+  return F;
+}
+
+}  // namespace
+
+// The decisions of one patch state into `decide` (nx*ny, device), on stream s.
+int switch_decide(Switch* sw, const Geom& G, const double* w_n, const double* pres, const enl_gas_t* gas, float lo,
+                  float hi, unsigned char* decide, hipStream_t s) {
+  const long ncell = (long)G.nx * G.ny;
+  if (ncell > sw->cand_cap) {
+    if (sw->cand) (void)hipFree(sw->cand);
+    hipError_t e = hipMalloc((void**)&sw->cand, sizeof(unsigned) * (size_t)ncell);
+    if (e != hipSuccess) return check_hip(e, "hipMalloc(switch candidates)");
+    sw->cand_cap = ncell;
+  }
+  DecideArgs a;
+  a.F = make_feat(G, w_n, pres, gas);
+  a.nx = G.nx;
+  a.ny = G.ny;
+  a.g = G.g0;
+  a.thr = 5e-5 * sensor_eps(gas->tag_tolerance[2]);
+  a.decide = decide;
+  a.cand = sw->cand;
+  a.ncand = sw->ncand;
+  a.lo = lo;
+  a.hi = hi;
+  hipLaunchKernelGGL(decide_reset_kernel, dim3(1), dim3(64), 0, s, sw->ncand);
+  hipLaunchKernelGGL(prefilter_kernel, dim3(blocks_for(ncell, 8192)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(infer_kernel, dim3(2048), dim3(kInferLanes), 0, s, a, sw->wb);
+  return check_hip(hipGetLastError(), "switch decisions");
+}
+
+// Adds the last decisions' counts (candidates, WENO, undecided) to totals.
+int switch_accumulate(Switch* sw, unsigned long long* totals, hipStream_t s) {
+  hipLaunchKernelGGL(accumulate_kernel, dim3(1), dim3(64), 0, s, sw->ncand, totals);
+  return check_hip(hipGetLastError(), "switch counts");
+}
+
+}  // namespace enl
+
+using namespace enl;
+
+extern "C" {
+
+int enl_switch_create(void** handle) {
+  *handle = nullptr;
+  Switch* s = new Switch();
+  hipError_t e = hipMalloc((void**)&s->w, sizeof(float) * kNParam);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->wb, sizeof(unsigned short) * kNParam);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->m, sizeof(float) * kNParam);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->v, sizeof(float) * kNParam);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->g, sizeof(float) * kNParam);
+  if (e == hipSuccess) e = hipMalloc((void**)&s->loss, sizeof(float));
+  if (e == hipSuccess) e = hipMalloc((void**)&s->ncand, sizeof(unsigned) * 4);
+  if (e == hipSuccess) e = hipMemset(s->m, 0, sizeof(float) * kNParam);
+  if (e == hipSuccess) e = hipMemset(s->v, 0, sizeof(float) * kNParam);
+  if (e == hipSuccess) e = hipMemset(s->w, 0, sizeof(float) * kNParam);
+  if (e == hipSuccess) e = hipMemset(s->wb, 0, sizeof(unsigned short) * kNParam);
+  if (e != hipSuccess) {
+    enl_switch_destroy(s);
+    return check_hip(e, "enl_switch_create");
+  }
+  *handle = s;
+  return 0;
+}
+
+int enl_switch_destroy(void* handle) {
+  Switch* s = (Switch*)handle;
+  if (!s) return 0;
+  (void)hipDeviceSynchronize();
+  void* ptrs[] = {s->w, s->wb, s->m, s->v, s->g, s->loss, s->ncand, s->cand,
+                  s->t.h1, s->t.h2, s->t.dh1, s->t.dh2, s->t.dz, s->t.p};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete s;
+  return 0;
+}
+
+int enl_switch_set_params(void* handle, const float* params) {
+  Switch* s = (Switch*)handle;
+  if (!s || !params) return set_error(-2, "enl_switch_set_params: bad arguments");
+  hipStream_t st = lib_stream();
+  hipError_t e = hipMemcpyAsync(s->w, params, sizeof(float) * kNParam, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemsetAsync(s->m, 0, sizeof(float) * kNParam, st);
+  if (e == hipSuccess) e = hipMemsetAsync(s->v, 0, sizeof(float) * kNParam, st);
+  if (e != hipSuccess) return check_hip(e, "enl_switch_set_params");
+  s->step = 0;
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(kNParam)), dim3(256), 0, st, s->w, s->wb);
+  return check_hip(hipStreamSynchronize(st), "enl_switch_set_params sync");
+}
+
+int enl_switch_get_params(void* handle, float* params) {
+  Switch* s = (Switch*)handle;
+  if (!s || !params) return set_error(-2, "enl_switch_get_params: bad arguments");
+  hipStream_t st = lib_stream();
+  hipError_t e = hipMemcpyAsync(params, s->w, sizeof(float) * kNParam, hipMemcpyDeviceToHost, st);
+  if (e != hipSuccess) return check_hip(e, "enl_switch_get_params");
+  return check_hip(hipStreamSynchronize(st), "enl_switch_get_params sync");
+}
+
+int enl_switch_features(int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw, const double* w_n,
+                        const double* pres, const enl_gas_t* gas, const int* cells, int n, unsigned short* X) {
+  if (gcw < 3 || ilast0 < ifirst0 || ilast1 < ifirst1 || n < 0) return set_error(-2, "enl_switch_features: bad box");
+  if (n == 0) return 0;
+  const void* arrays[4] = {w_n, pres, cells, X};
+  for (const void* a : arrays)
+    if (!is_device_ptr(a)) return set_error(-4, "enl_switch_features needs device arrays");
+  Geom G = make_geom(ifirst0, ilast0, ifirst1, ilast1, gcw, gcw);
+  hipLaunchKernelGGL(features_kernel, dim3(blocks_for(n)), dim3(256), 0, lib_stream(), make_feat(G, w_n, pres, gas),
+                     G.nx, gcw, cells, n, X);
+  return check_hip(hipGetLastError(), "enl_switch_features");
+}
+
+int enl_switch_forward(void* handle, const unsigned short* X, int n, float* p) {
+  Switch* s = (Switch*)handle;
+  if (!s || n < 0) return set_error(-2, "enl_switch_forward: bad arguments");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(forward_kernel, dim3(blocks_for(n * 4L)), dim3(kLanes), 0, lib_stream(), s->wb, X, n, p);
+  return check_hip(hipGetLastError(), "enl_switch_forward");
+}
+
+// Loss and gradient of a batch: grad (device, ENL_SWITCH_NPARAM), loss (host).
+int enl_switch_gradients(void* handle, const unsigned short* X, const float* y, const float* wt, int n, float wsum,
+                         float* grad, float* loss) {
+  Switch* s = (Switch*)handle;
+  if (!s || n <= 0 || !(wsum > 0)) return set_error(-2, "enl_switch_gradients: bad arguments");
+  int rc = grow_train(s, n);
+  if (rc) return rc;
+  hipStream_t st = lib_stream();
+  hipError_t e = hipMemsetAsync(s->loss, 0, sizeof(float), st);
+  if (e != hipSuccess) return check_hip(e, "loss reset");
+  hipLaunchKernelGGL(fwd_bwd_kernel, dim3(blocks_for(n * 4L)), dim3(kLanes), 0, st, s->wb, X, y, wt, n, 1.0f / wsum,
+                     s->t, s->loss);
+  hipLaunchKernelGGL(grad_kernel, dim3(blocks_for(kNParam)), dim3(256), 0, st, X, n, s->t, grad ? grad : s->g);
+  if (loss) {
+    e = hipMemcpyAsync(loss, s->loss, sizeof(float), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return check_hip(e, "loss copy");
+    return check_hip(hipStreamSynchronize(st), "enl_switch_gradients sync");
+  }
+  return check_hip(hipGetLastError(), "enl_switch_gradients");
+}
+
+int enl_switch_train_step(void* handle, const unsigned short* X, const float* y, const float* wt, int n, float wsum,
+                          float lr, float* loss) {
+  Switch* s = (Switch*)handle;
+  int rc = enl_switch_gradients(handle, X, y, wt, n, wsum, s ? s->g : nullptr, loss);
+  if (rc) return rc;
+  s->step++;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = (float)(1.0 / (1.0 - std::pow((double)b1, (double)s->step)));
+  const float c2 = (float)(1.0 / (1.0 - std::pow((double)b2, (double)s->step)));
+  hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(kNParam)), dim3(256), 0, lib_stream(), s->w, s->m, s->v, s->g,
+                     s->wb, lr, b1, b2, c1, c2, 1e-8f);
+  return check_hip(hipGetLastError(), "enl_switch_train_step");
+}
+
+int enl_switch_decide(void* handle, int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw, const double* w_n,
+                      const double* pres, const enl_gas_t* gas, float lo, float hi, unsigned char* decide,
+                      unsigned* counts) {
+  Switch* s = (Switch*)handle;
+  if (!s || gcw < 3 || ilast0 < ifirst0 || ilast1 < ifirst1) return set_error(-2, "enl_switch_decide: bad arguments");
+  Geom G = make_geom(ifirst0, ilast0, ifirst1, ilast1, gcw, gcw);
+  hipStream_t st = lib_stream();
+  int rc = switch_decide(s, G, w_n, pres, gas, lo, hi, decide, st);
+  if (rc) return rc;
+  if (counts) {
+    hipError_t e = hipMemcpyAsync(counts, s->ncand, sizeof(unsigned) * 3, hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return check_hip(e, "switch counts");
+    return check_hip(hipStreamSynchronize(st), "enl_switch_decide sync");
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -11,6 +11,8 @@ c_lr = 0; the remaining constants are those of input.txt.
 import math
 from dataclasses import dataclass, field, asdict
 
+import numpy as np
+
 from ._lib import EnlGas
 
 
@@ -80,3 +82,11 @@ class Gas:
 # w_n = alpha_1*w + alpha_2*w_n + beta*dt*K (input.txt:130-133 lists them to
 # 10 digits; the exact thirds of matlab/hybrid.m:132-145 are used here).
 RK3 = ((1.0, 0.75, 1.0 / 3.0), (0.0, 0.25, 2.0 / 3.0), (1.0, 0.25, 2.0 / 3.0))
+
+
+def sensor_eps(tol):
+    """The hybrid sensor's epsilon from tag_tolerance(2): 0.9f * tol * xi^2 /
+    (1 - 0.9f * tol^2) with xi = 1 (rhs_hybrid_high_llf.m4:114-115; the
+    literal 0.9 is single precision in the Fortran)."""
+    c09 = float(np.float32(0.9))
+    return c09 * tol / (1.0 - c09 * tol * tol)

@@ -249,6 +249,46 @@ int enl_batch_fp32_load(void *handle);
 /* FP32 form -> the current FP64 state (ambient + perturbation). */
 int enl_batch_fp32_store(void *handle);
 
+/* ---- learned WENO/DRP switch (no reference counterpart; stands in for the
+ * hybrid variants' density sensor, rhs_hybrid_high_llf.m4:239-256) --------
+ * A prefilter keeps the cells where the sensor provably says DRP; on the
+ * other cells an MLP 100-64-32-1 (BF16 weights and inputs, FP32 accumulation)
+ * over a 5x5 window of normalised rho and p differences gives p(WENO):
+ * p > hi -> WENO, p < lo -> DRP, otherwise the sensor decides.  Parameters
+ * are packed FP32: W1[64][100], b1[64], W2[32][64], b2[32], w3[32], b3. */
+#define ENL_SWITCH_NPARAM 8577
+int enl_switch_create(void **handle);
+int enl_switch_destroy(void *handle);
+int enl_switch_set_params(void *handle, const float *params); /* host array; resets the optimiser */
+int enl_switch_get_params(void *handle, float *params);       /* host array */
+/* Features (BF16 bits, n x 100) of listed interior cells (i + j*nx) of a
+ * patch state; device arrays. */
+int enl_switch_features(int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw, const double *w_n,
+                        const double *pres, const enl_gas_t *gas, const int *cells, int n, unsigned short *X);
+/* p(WENO) of n feature rows (device arrays). */
+int enl_switch_forward(void *handle, const unsigned short *X, int n, float *p);
+/* Weighted mean binary cross-entropy of a batch (labels y, weights wt with
+ * sum wsum; device arrays) and its gradient (device, ENL_SWITCH_NPARAM). */
+int enl_switch_gradients(void *handle, const unsigned short *X, const float *y, const float *wt, int n, float wsum,
+                         float *grad, float *loss);
+/* One Adam step on a batch. */
+int enl_switch_train_step(void *handle, const unsigned short *X, const float *y, const float *wt, int n, float wsum,
+                          float lr, float *loss);
+/* Decisions (0 DRP, 1 WENO, 2 sensor) of a patch state into decide (nx*ny,
+ * device); counts (host, optional): candidates, WENO, undecided. */
+int enl_switch_decide(void *handle, int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw, const double *w_n,
+                      const double *pres, const enl_gas_t *gas, float lo, float hi, unsigned char *decide,
+                      unsigned *counts);
+/* enl_advance_patch_fused on level 0 of a hybrid variant with the learned
+ * switch deciding every stage; totals (device, 3 x u64, optional) accumulate
+ * the decision counts. */
+int enl_advance_patch_learned(void *sw, float lo, float hi, unsigned char *decide, unsigned long long *totals,
+                              int variant, int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw,
+                              const double *xlo, const double *dx, const enl_gas_t *gas, const int *reflect, int order,
+                              const double *alpha_1, const double *alpha_2, const double *beta, double dt, int nsteps,
+                              double *w_n, double *pres, double *temp, double *source, void *scratch,
+                              size_t scratch_bytes);
+
 /* ---- measurement hooks (no reference counterpart) ---------------------- */
 
 /* Kernel classes timed by the patch-loop entry points. */
