@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--amr-steps", type=int, default=40)
     ap.add_argument("--amr-plain-steps", type=int, default=4)
     ap.add_argument("--precision-steps", type=int, default=2000)
+    ap.add_argument("--switch-steps", type=int, default=100)
     return ap.parse_args()
 
 
@@ -270,6 +271,96 @@ def precision_bench(args, tags):
     }
 
 
+def switch_bench(args):
+    """The learned switch on the tortuous line source (4096^2 base at 0.25 m,
+    held-out channel seed): the plain hybrid (density sensor) against the
+    hybrid with the trained network deciding level 0's cells (p > 0.8 WENO,
+    p < 0.2 DRP, else the sensor), both fused FP64 from the same state; the
+    plain solution judges the learned one's error, and the network's calls are
+    compared with the sensor's on the same states (missed shocks: sensor WENO,
+    network confident DRP)."""
+    import numpy as np
+    import torch
+    from enlightning_amd import switch as S
+    from enlightning_amd.gas import Gas
+
+    params = S.load_params()
+    if params is None:
+        return {"skipped": "no trained parameters (python -m enlightning_amd.train_switch)"}
+    gas = Gas()
+    wl = S.CHANNEL_4096
+    dt = wl.dt(gas)
+    steps = args.switch_steps
+    sw = S.Switch(params)
+    lo, hi = 0.2, 0.8
+    from enlightning_amd import patch as P
+    from enlightning_amd.workloads import state_from_overpressure
+    N = wl.n + 2 * wl.g
+    w0 = state_from_overpressure(S.channel_overpressure(wl, wl.seed), gas)
+    ic = (w0, np.full((N, N), gas.p0), np.full((N, N), gas.T0), np.zeros((N, N)))
+    plain = P.DevicePatch(wl, gas)
+    learned = P.DevicePatch(wl, gas)
+    plain.init_state(*ic)
+    learned.init_state(*ic)
+    plain.advance_fused(2, dt)  # warm-up
+    sw.advance(learned, 2, dt, lo, hi)
+    plain.init_state(*ic)
+    learned.init_state(*ic)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain.advance_fused(steps, dt)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    cand, weno, undecided = sw.advance(learned, steps, dt, lo, hi)
+    torch.cuda.synchronize()
+    t_learned = time.perf_counter() - t0
+    # without a switch: WENO on every cell (rhs_weno_high_llf), same steps
+    allw = P.DevicePatch(wl, gas)
+    allw.init_state(*ic)
+    allw.advance_fused(1, dt, variant="weno_high_llf")
+    allw.init_state(*ic)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    allw.advance_fused(steps, dt, variant="weno_high_llf")
+    torch.cuda.synchronize()
+    t_allweno = time.perf_counter() - t0
+    del allw
+    # decisions against the sensor on the final learned state
+    S.P.fill_physical_bc(learned.box, learned.g, (1, 1, 1, 1), learned.reflect, learned.w, learned.p, learned.T)
+    sens = S.sensor_map(learned)
+    dec, _ = sw.decide(learned, lo, hi)
+    missed = int(((sens == 1) & (dec == 0)).sum().item())
+    false_weno = int(((sens == 0) & (dec == 1)).sum().item())
+    sensor_weno = int((sens == 1).sum().item())
+    g = wl.g
+    err = {}
+    names = ("rho", "rho_u", "rho_v", "rho_s", "rho_Tn")
+    for k, name in enumerate(names):
+        x = plain.w[k, g:-g, g:-g]
+        y = learned.w[k, g:-g, g:-g]
+        ref = x - x.mean() if k in (0, 3, 4) else x
+        err[name] = {"rel_l2": float(torch.linalg.norm(x - y) / torch.linalg.norm(ref)),
+                     "linf": float((x - y).abs().max()), "ref_linf": float(ref.abs().max())}
+    cells = wl.n * wl.n
+    out = {
+        "workload": wl.name, "steps": steps, "cfl": wl.cfl, "thresholds": [lo, hi],
+        "plain_sensor": {"value": cells * steps / t_plain, "ms_per_step": t_plain * 1e3 / steps},
+        "learned_switch": {"value": cells * steps / t_learned, "ms_per_step": t_learned * 1e3 / steps},
+        "weno_everywhere_no_switch": {"value": cells * steps / t_allweno, "ms_per_step": t_allweno * 1e3 / steps},
+        "speedup_learned_vs_plain": t_plain / t_learned,
+        "speedup_learned_vs_no_switch": t_allweno / t_learned,
+        "decisions_summed_over_stages": {"candidates": cand, "network_weno": weno, "to_sensor": undecided,
+                                         "network_drp": cand - weno - undecided,
+                                         "prefilter_drp": 3 * steps * cells - cand},
+        "final_state_vs_sensor": {"sensor_weno": sensor_weno, "missed_shocks": missed, "false_weno": false_weno},
+        "error_vs_plain_hybrid": err,
+        "unit": "cell-updates/s",
+    }
+    sw.close()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -386,6 +477,10 @@ def main():
         torch.cuda.empty_cache()
         if args.precision_steps > 0:
             out["precision"] = precision_bench(args, tags)
+            torch.cuda.empty_cache()
+        if args.switch_steps > 0:
+            out["learned_switch"] = switch_bench(args)
+            torch.cuda.empty_cache()
         torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, gas, args.cpu_steps)

@@ -104,7 +104,8 @@ __device__ inline void point_features(const Feat& F, long g, int di, int dj, flo
 }
 
 // MLP on one sample whose BF16 features are at xb (LDS or global); h1 is
-// the lane's LDS slot for the first hidden layer.  Layer 1 runs in four
+// the lane's LDS slot for the first hidden layer, kept in BF16 as the
+// matrix-core pass feeds it to layer 2.  Layer 1 runs in four
 // chunks of 16 units over a rolled input loop and layer 2 over a rolled
 // loop on h1, so the live registers stay low.  Returns the logit; h2
 // receives the second hidden layer.
@@ -131,7 +132,7 @@ __device__ inline float mlp_forward(const float* lp, const unsigned short* xb, f
       }
     }
 #pragma unroll
-    for (int j = 0; j < 16; j++) h1[c * 16 + j] = a[j] > 0.0f ? a[j] : 0.0f;
+    for (int j = 0; j < 16; j++) h1[c * 16 + j] = bf2f(f2bf(a[j] > 0.0f ? a[j] : 0.0f));
   }
   float a2[kH2];
 #pragma unroll
@@ -208,44 +209,136 @@ __global__ void __launch_bounds__(256) prefilter_kernel(DecideArgs a) {
   }
 }
 
-// One wave per block: the wave's features sit in LDS (BF16) next to the
-// parameters.
-constexpr int kInferLanes = kLanes;
+// Inference on the matrix cores: each wave takes 16 candidates at a time
+// (M = 16) through v_mfma_f32_16x16x16_bf16.  Lane l of a 16x16x16 MFMA holds
+// A[l%16][4(l/16)+v], B[4(l/16)+v][l%16] and D[4(l/16)+v][l%16], v = 0..3,
+// so lane l computes the features of its cell l%16 at window points
+// 4s + l/16 (s = 0..6; feature k = 4*point + channel, K padded to 112), and
+// keeps the B operands of W1 (28 MFMA tiles) and W2 (8) in registers for
+// the whole kernel.  Layer 1's output goes through LDS (BF16) into layer 2's
+// A layout; layer 3 is a 16-lane reduction.
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(kInferLanes) infer_kernel(DecideArgs a, const unsigned short* params) {
-  __shared__ float lp[kNParam + 3];
-  __shared__ unsigned short xs[kInferLanes][kIn];
-  __shared__ float h1s[kInferLanes][kH1Pad];
-  load_params(lp, params);
-  __syncthreads();
+constexpr int kWaves = 4;
+constexpr int kH1Row = kH1 + 4;  // LDS row of a cell's layer-1 output (bf16)
+
+__global__ void __launch_bounds__(64 * kWaves) infer_kernel(DecideArgs a, const unsigned short* params) {
+  __shared__ unsigned short h1s[kWaves][16][kH1Row];
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int row = l & 15, quad = l >> 4;
+  // B operands: W1^T[k][n] = W1[n][k], W2^T[k][n] = W2[n][k]
+  bf16x4 bw1[7][4], bw2[4][2];
+#pragma unroll
+  for (int sk = 0; sk < 7; sk++)
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int k = 16 * sk + 4 * quad + v, n = 16 * t + row;
+        bw1[sk][t][v] = k < kIn ? (short)params[oW1 + n * kIn + k] : (short)0;
+      }
+#pragma unroll
+  for (int sk = 0; sk < 4; sk++)
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) bw2[sk][t][v] = (short)params[oW2 + (16 * t + row) * kH1 + 16 * sk + 4 * quad + v];
+  float b1v[4], b2v[2], w3v[2];
+#pragma unroll
+  for (int t = 0; t < 4; t++) b1v[t] = bf2f(params[oB1 + 16 * t + row]);
+#pragma unroll
+  for (int t = 0; t < 2; t++) {
+    b2v[t] = bf2f(params[oB2 + 16 * t + row]);
+    w3v[t] = bf2f(params[oW3 + 16 * t + row]);
+  }
+  const float b3 = bf2f(params[oB3]);
+
   const unsigned n = a.ncand[0];
+  const unsigned groups = (n + 15) / 16;
   unsigned nweno = 0, nfall = 0;
-  for (unsigned q0 = blockIdx.x * kInferLanes; q0 < n; q0 += gridDim.x * kInferLanes) {
-    const unsigned q = q0 + threadIdx.x;
-    if (q < n) {
-      const unsigned c = a.cand[q];
+  for (unsigned grp = blockIdx.x * kWaves + wv; grp < groups; grp += gridDim.x * kWaves) {
+    const unsigned q = grp * 16 + row;
+    const bool live = q < n;
+    unsigned c = 0;
+    long g = 0;
+    if (live) {
+      c = a.cand[q];
       const int i = (int)(c % (unsigned)a.nx), j = (int)(c / (unsigned)a.nx);
-      const long g = (long)(i + a.g) + (long)(j + a.g) * a.F.sy;
-      unsigned short* xb = xs[threadIdx.x];
-      for (int pt = 0; pt < 25; pt++) {
+      g = (long)(i + a.g) + (long)(j + a.g) * a.F.sy;
+    }
+    // layer 1: features of this lane's points straight into the A operands
+    f32x4 acc1[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) acc1[t] = f32x4{0, 0, 0, 0};
+#pragma unroll
+    for (int sk = 0; sk < 7; sk++) {
+      const int pt = 4 * sk + quad;
+      bf16x4 av = {0, 0, 0, 0};
+      if (live && pt < 25) {
         float f[4];
         point_features(a.F, g, pt % 5 - 2, pt / 5 - 2, f);
-        for (int k = 0; k < 4; k++) xb[pt * 4 + k] = f2bf(f[k]);
+#pragma unroll
+        for (int v = 0; v < 4; v++) av[v] = (short)f2bf(f[v]);
       }
-      float h2[kH2];
-      const float p = sigmoid(mlp_forward(lp, xb, h1s[threadIdx.x], h2));
-      const unsigned char d = p > a.hi ? 1 : (p < a.lo ? 0 : 2);
-      a.decide[c] = d;
-      nweno += d == 1;
-      nfall += d == 2;
+#pragma unroll
+      for (int t = 0; t < 4; t++) acc1[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bw1[sk][t], acc1[t], 0, 0, 0);
+    }
+    // bias, ReLU, BF16 into LDS as [cell][unit]
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const float h = acc1[t][v] + b1v[t];
+        h1s[wv][4 * quad + v][16 * t + row] = f2bf(h > 0.0f ? h : 0.0f);
+      }
+    // the buffer is the wave's own; LDS operations of a wave complete in order
+    __builtin_amdgcn_wave_barrier();
+    // layer 2
+    f32x4 acc2[2] = {f32x4{0, 0, 0, 0}, f32x4{0, 0, 0, 0}};
+#pragma unroll
+    for (int sk = 0; sk < 4; sk++) {
+      bf16x4 av;
+#pragma unroll
+      for (int v = 0; v < 4; v++) av[v] = (short)h1s[wv][row][16 * sk + 4 * quad + v];
+#pragma unroll
+      for (int t = 0; t < 2; t++) acc2[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bw2[sk][t], acc2[t], 0, 0, 0);
+    }
+    __builtin_amdgcn_wave_barrier();
+    // layer 3: this lane's share of z for cells 4*quad+v, summed over the 16
+    // lanes of its quad
+    float z[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      float s = 0.0f;
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        const float h = acc2[t][v] + b2v[t];
+        s += w3v[t] * (h > 0.0f ? h : 0.0f);
+      }
+#pragma unroll
+      for (int m = 1; m < 16; m <<= 1) s += __shfl_xor(s, m);
+      z[v] = s + b3;
+    }
+    if (row == 0) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const unsigned qq = grp * 16 + 4 * quad + v;
+        if (qq < n) {
+          const float p = sigmoid(z[v]);
+          const unsigned char d = p > a.hi ? 1 : (p < a.lo ? 0 : 2);
+          a.decide[a.cand[qq]] = d;
+          nweno += d == 1;
+          nfall += d == 2;
+        }
+      }
     }
   }
-  // per-wave sums, one atomic per wave
   for (int off = 32; off > 0; off >>= 1) {
     nweno += __shfl_down(nweno, off);
     nfall += __shfl_down(nfall, off);
   }
-  if (threadIdx.x == 0) {
+  if (l == 0) {
     if (nweno) atomicAdd(a.ncand + 1, nweno);
     if (nfall) atomicAdd(a.ncand + 2, nfall);
   }
@@ -451,7 +544,7 @@ int switch_decide(Switch* sw, const Geom& G, const double* w_n, const double* pr
   a.hi = hi;
   hipLaunchKernelGGL(decide_reset_kernel, dim3(1), dim3(64), 0, s, sw->ncand);
   hipLaunchKernelGGL(prefilter_kernel, dim3(blocks_for(ncell, 8192)), dim3(256), 0, s, a);
-  hipLaunchKernelGGL(infer_kernel, dim3(2048), dim3(kInferLanes), 0, s, a, sw->wb);
+  hipLaunchKernelGGL(infer_kernel, dim3(1024), dim3(64 * kWaves), 0, s, a, sw->wb);
   return check_hip(hipGetLastError(), "switch decisions");
 }
 

@@ -235,11 +235,14 @@ def candidates(dp):
     return cand
 
 
-def build_samples(sw, wl, gas, seeds, snapshots=(0, 20, 60), dt=None, per_snapshot=40000, rng_seed=0):
+def build_samples(sw, wl, gas, seeds, snapshots=(0, 20, 60), dt=None, per_snapshot=40000, rng_seed=0,
+                  weno_share=0.75):
     """Features, labels and weights from plain-sensor maps: for each channel
     seed and snapshot step, every sensor-WENO candidate (up to
     per_snapshot/2) and as many sensor-DRP candidates; labels are the
-    sensor's choice; classes weighted to equal total weight."""
+    sensor's choice.  The WENO class carries weno_share of the total weight,
+    so that calling a sensor-WENO cell DRP (a missed shock) costs more than
+    the opposite error."""
     gen = torch.Generator(device="cuda")
     gen.manual_seed(rng_seed)
     Xs, ys = [], []
@@ -269,7 +272,7 @@ def build_samples(sw, wl, gas, seeds, snapshots=(0, 20, 60), dt=None, per_snapsh
     y = torch.cat(ys).float()
     npos = float(y.sum().item())
     nneg = float(y.numel() - npos)
-    w = torch.where(y > 0.5, 0.5 / max(npos, 1.0), 0.5 / max(nneg, 1.0)).float() * y.numel()
+    w = torch.where(y > 0.5, weno_share / max(npos, 1.0), (1.0 - weno_share) / max(nneg, 1.0)).float() * y.numel()
     return X, y, w
 
 

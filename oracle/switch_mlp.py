@@ -6,7 +6,8 @@ kernels compute, in float64 numpy, so that the device forward pass, loss and
 gradients can be checked:
   features  asinh(d_rho / sqrt(eps)), asinh(d_p / (c^2 sqrt(eps))) of the x
             and y forward differences on a 5x5 window, rounded to BF16;
-  network   100-64-32-1, ReLU, sigmoid, parameters rounded to BF16;
+  network   100-64-32-1, ReLU, sigmoid, parameters and the first hidden
+            layer rounded to BF16;
   loss      weighted mean binary cross-entropy, sum(w_b * l_b) / sum(w_b).
 The gradients are pinned by central finite differences of this loss
 (tests/test_switch_oracle.py), independent of the backward formulas here.
@@ -41,19 +42,23 @@ def pack(W1, b1, W2, b2, w3, b3):
     return np.concatenate([W1.ravel(), b1, W2.ravel(), b2, w3, [b3]])
 
 
-def forward(params, X):
-    """Logits and hidden activations for feature rows X (already BF16 values)."""
+def forward(params, X, round_h1=True):
+    """Logits and hidden activations for feature rows X (already BF16 values);
+    the first hidden layer is rounded to BF16 as the device feeds it to the
+    second."""
     W1, b1, W2, b2, w3, b3 = unpack(bf16(params))
     a1 = X @ W1.T + b1
     h1 = np.maximum(a1, 0.0)
+    if round_h1:
+        h1 = bf16(h1)
     a2 = h1 @ W2.T + b2
     h2 = np.maximum(a2, 0.0)
     z = h2 @ w3 + b3
     return z, h1, h2
 
 
-def loss(params, X, y, w):
-    z = forward(params, X)[0]
+def loss(params, X, y, w, round_h1=True):
+    z = forward(params, X, round_h1)[0]
     l = np.maximum(z, 0.0) - z * y + np.log1p(np.exp(-np.abs(z)))
     return float(np.sum(w * l) / np.sum(w))
 
@@ -69,10 +74,11 @@ def loss_exact_params(pvals, X, y, w):
     return float(np.sum(w * l) / np.sum(w))
 
 
-def gradients(params, X, y, w):
-    """d loss / d params, evaluated at the BF16-rounded parameters."""
+def gradients(params, X, y, w, round_h1=True):
+    """d loss / d params, evaluated at the BF16-rounded parameters (the
+    rounding of the first hidden layer passes gradients straight through)."""
     W1, b1, W2, b2, w3, b3 = unpack(bf16(params))
-    z, h1, h2 = forward(params, X)
+    z, h1, h2 = forward(params, X, round_h1)
     p = 1.0 / (1.0 + np.exp(-z))
     dz = w * (p - y) / np.sum(w)
     gw3 = h2.T @ dz

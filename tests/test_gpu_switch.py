@@ -48,12 +48,17 @@ def test_forward_loss_gradients_match_oracle():
     w = rng.uniform(0.5, 2.0, size=n).astype(np.float32)
     p = sw.forward(X).cpu().numpy()
     z = M.forward(params, xf)[0]
-    assert np.abs(p - 1 / (1 + np.exp(-z))).max() <= 2e-5
+    # the first hidden layer is rounded to BF16: where FP32 and FP64 sums
+    # straddle a rounding boundary that unit moves by one BF16 ulp, so a few
+    # samples differ by up to ~1e-3; the rest agree to FP32 accuracy
+    e = np.abs(p - 1 / (1 + np.exp(-z)))
+    assert np.percentile(e, 99) <= 2e-5 and e.max() <= 5e-3
     g, loss = sw.gradients(X, torch.from_numpy(y).cuda(), torch.from_numpy(w).cuda())
     g = g.cpu().numpy()
     g_ref = M.gradients(params, xf, y.astype(np.float64), w.astype(np.float64))
-    assert abs(loss - M.loss(params, xf, y, w)) <= 2e-5 * max(1.0, abs(loss))
-    assert np.abs(g - g_ref).max() <= 2e-4 * np.abs(g_ref).max()
+    assert abs(loss - M.loss(params, xf, y, w)) <= 1e-4 * max(1.0, abs(loss))
+    ge = np.abs(g - g_ref) / np.abs(g_ref).max()
+    assert np.percentile(ge, 99) <= 2e-4 and ge.max() <= 1e-2
     sw.close()
 
 
@@ -127,3 +132,29 @@ def test_learned_hybrid_close_to_plain(trained):
         ref = x - x.mean() if k in (0, 3, 4) else x
         err = np.sqrt(np.sum((x - y) ** 2) / max(np.sum(ref ** 2), 1e-300))
         assert err <= 5e-2, (k, err)
+
+
+def test_matrix_core_decisions_match_forward(trained):
+    """The per-stage decisions (matrix-core pass over the compacted
+    candidates) against the plain forward pass on the same features: every
+    non-candidate is DRP, and candidates get the thresholded probability
+    except within 1e-3 of a threshold (summation order differs)."""
+    sw, _, _, _ = trained
+    gas = Gas()
+    dp = S.channel_patch(SMALL, gas, seed=96)
+    dp.advance_fused(8, SMALL.dt(gas))
+    S.P.fill_physical_bc(dp.box, dp.g, (1, 1, 1, 1), dp.reflect, dp.w, dp.p, dp.T)
+    lo, hi = 0.2, 0.8
+    d, counts = sw.decide(dp, lo, hi)
+    cand = S.candidates(dp).flatten()
+    d = d.flatten()
+    assert counts[0] == int(cand.sum().item())
+    assert int((d[~cand] != 0).sum().item()) == 0
+    cells = torch.nonzero(cand).flatten()
+    p = sw.forward(sw.features(dp, cells))
+    ref = torch.where(p > hi, 1, torch.where(p < lo, 0, 2)).to(torch.uint8)
+    got = d[cells]
+    bad = got != ref
+    near = ((p - lo).abs() < 1e-3) | ((p - hi).abs() < 1e-3)
+    assert int((bad & ~near).sum().item()) == 0
+    assert counts[1] == int((got == 1).sum().item()) and counts[2] == int((got == 2).sum().item())
