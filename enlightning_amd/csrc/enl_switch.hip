@@ -10,7 +10,7 @@
 //   2. the network runs on the candidates only: a 5x5 window of four
 //      normalised differences (x and y differences of rho and p, each
 //      asinh(d / scale) with scale sqrt(eps) for rho and c^2 sqrt(eps) for
-//      p) -> MLP 100-64-32-1 (ReLU, sigmoid), BF16 inputs and weights, FP32
+//      p, asinh_fast below) -> MLP 100-64-32-1 (ReLU, sigmoid), BF16 inputs and weights, FP32
 //      accumulation.  p > hi -> WENO, p < lo -> DRP, otherwise the plain
 //      sensor decides (the split kernel evaluates it for those cells);
 //   3. the split kernel reads the decisions and compacts the WENO cells as
@@ -85,6 +85,22 @@ __device__ inline void load_params(float* lp, const unsigned short* pb) {
   }
 }
 
+// asinh in a few instructions: a Taylor polynomial below 0.5 (relative
+// error < 1.3e-4) and log(|x| + sqrt(x^2 + 1)) with the hardware log and
+// square root above; the features are rounded to BF16 (relative step
+// 2^-8) afterwards, so both branches are well inside that.
+__device__ inline float asinh_fast(float x) {
+  const float a = fabsf(x);
+  float r;
+  if (a < 0.5f) {
+    const float a2 = a * a;
+    r = a * (1.0f + a2 * (-1.0f / 6.0f + a2 * (3.0f / 40.0f + a2 * (-15.0f / 336.0f))));
+  } else {
+    r = __logf(a + __fsqrt_rn(a * a + 1.0f));
+  }
+  return copysignf(r, x);
+}
+
 struct Feat {
   const double* r;  // rho: ghosted w_n component 0
   const double* p;  // ghosted pressure
@@ -97,10 +113,10 @@ struct Feat {
 __device__ inline void point_features(const Feat& F, long g, int di, int dj, float x[4]) {
   const long q = g + di + (long)dj * F.sy;
   const double r0 = F.r[q], p0 = F.p[q];
-  x[0] = bf2f(f2bf(asinhf((float)(F.r[q + 1] - r0) * F.inv_r)));
-  x[1] = bf2f(f2bf(asinhf((float)(F.r[q + F.sy] - r0) * F.inv_r)));
-  x[2] = bf2f(f2bf(asinhf((float)(F.p[q + 1] - p0) * F.inv_p)));
-  x[3] = bf2f(f2bf(asinhf((float)(F.p[q + F.sy] - p0) * F.inv_p)));
+  x[0] = bf2f(f2bf(asinh_fast((float)(F.r[q + 1] - r0) * F.inv_r)));
+  x[1] = bf2f(f2bf(asinh_fast((float)(F.r[q + F.sy] - r0) * F.inv_r)));
+  x[2] = bf2f(f2bf(asinh_fast((float)(F.p[q + 1] - p0) * F.inv_p)));
+  x[3] = bf2f(f2bf(asinh_fast((float)(F.p[q + F.sy] - p0) * F.inv_p)));
 }
 
 // MLP on one sample whose BF16 features are at xb (LDS or global); h1 is
@@ -177,35 +193,45 @@ __global__ void decide_reset_kernel(unsigned* n) {
   if (threadIdx.x < 3) n[threadIdx.x] = 0;
 }
 
-// Prefilter: DRP where the sensor provably says DRP, else a candidate.
-__global__ void __launch_bounds__(256) prefilter_kernel(DecideArgs a) {
-  const long ncell = (long)a.nx * a.ny;
-  for (long c0 = (long)blockIdx.x * blockDim.x; c0 < ncell; c0 += (long)gridDim.x * blockDim.x) {
-    const long c = c0 + threadIdx.x;
-    bool cand = false;
-    if (c < ncell) {
-      const int i = (int)(c % a.nx), j = (int)(c / a.nx);
-      const long g = (long)(i + a.g) + (long)(j + a.g) * a.F.sy;
-      const double* r = a.F.r;
-      const long S = a.F.sy;
+// Prefilter: DRP where the sensor provably says DRP, else a candidate.  A
+// 32 x 8 tile of cells with its 2-cell halo of rho is staged in LDS; the
+// candidates are compacted with one atomic per wave.
+constexpr int kPBX = 32, kPBY = 8, kPW = kPBX + 4, kPH = kPBY + 4;
+
+__global__ void __launch_bounds__(kPBX * kPBY) prefilter_kernel(DecideArgs a) {
+  __shared__ double rt[kPH * kPW];
+  const int nbx = (a.nx + kPBX - 1) / kPBX;
+  const int i0 = (blockIdx.x % nbx) * kPBX, j0 = (blockIdx.x / nbx) * kPBY;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < kPH * kPW; e += kPBX * kPBY) {
+    const int gi = i0 + e % kPW - 2, gj = j0 + e / kPW - 2;
+    if (gi >= -a.g && gi < a.nx + a.g && gj >= -a.g && gj < a.ny + a.g)
+      rt[e] = a.F.r[(long)(gi + a.g) + (long)(gj + a.g) * a.F.sy];
+  }
+  __syncthreads();
+  const int i = i0 + tid % kPBX, j = j0 + tid / kPBX;
+  bool cand = false;
+  long c = 0;
+  if (i < a.nx && j < a.ny) {
+    c = (long)i + (long)j * a.nx;
+    const int e = (tid / kPBX + 2) * kPW + tid % kPBX + 2;
 #pragma unroll
-      for (int q = -1; q <= 1; q++) {
-        double t1 = r[g + q] - r[g - 1 + q], t2 = r[g + 1 + q] - r[g + q];
-        if (t1 * t1 + t2 * t2 >= a.thr) cand = true;
-        t1 = r[g + q * S] - r[g + (q - 1) * S];
-        t2 = r[g + (q + 1) * S] - r[g + q * S];
-        if (t1 * t1 + t2 * t2 >= a.thr) cand = true;
-      }
-      if (!cand) a.decide[c] = 0;
+    for (int q = -1; q <= 1; q++) {
+      double t1 = rt[e + q] - rt[e - 1 + q], t2 = rt[e + 1 + q] - rt[e + q];
+      if (t1 * t1 + t2 * t2 >= a.thr) cand = true;
+      t1 = rt[e + q * kPW] - rt[e + (q - 1) * kPW];
+      t2 = rt[e + (q + 1) * kPW] - rt[e + q * kPW];
+      if (t1 * t1 + t2 * t2 >= a.thr) cand = true;
     }
-    const unsigned long long m = __ballot(cand);
-    if (m) {
-      const int lead = __ffsll((long long)m) - 1;
-      unsigned base = 0;
-      if ((int)(threadIdx.x & 63) == lead) base = atomicAdd(a.ncand, (unsigned)__popcll(m));
-      base = __shfl(base, lead);
-      if (cand) a.cand[base + __popcll(m & __lanemask_lt())] = (unsigned)c;
-    }
+    if (!cand) a.decide[c] = 0;
+  }
+  const unsigned long long m = __ballot(cand);
+  if (m) {
+    const int lead = __ffsll((long long)m) - 1;
+    unsigned base = 0;
+    if ((int)(threadIdx.x & 63) == lead) base = atomicAdd(a.ncand, (unsigned)__popcll(m));
+    base = __shfl(base, lead);
+    if (cand) a.cand[base + __popcll(m & __lanemask_lt())] = (unsigned)c;
   }
 }
 
@@ -543,7 +569,8 @@ int switch_decide(Switch* sw, const Geom& G, const double* w_n, const double* pr
   a.lo = lo;
   a.hi = hi;
   hipLaunchKernelGGL(decide_reset_kernel, dim3(1), dim3(64), 0, s, sw->ncand);
-  hipLaunchKernelGGL(prefilter_kernel, dim3(blocks_for(ncell, 8192)), dim3(256), 0, s, a);
+  const unsigned ntile = (unsigned)(((G.nx + kPBX - 1) / kPBX) * ((G.ny + kPBY - 1) / kPBY));
+  hipLaunchKernelGGL(prefilter_kernel, dim3(ntile), dim3(kPBX * kPBY), 0, s, a);
   hipLaunchKernelGGL(infer_kernel, dim3(1024), dim3(64 * kWaves), 0, s, a, sw->wb);
   return check_hip(hipGetLastError(), "switch decisions");
 }

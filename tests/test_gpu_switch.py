@@ -110,9 +110,13 @@ def trained():
 def test_training_reduces_loss_and_learns_the_sensor(trained):
     sw, losses, untrained, ev = trained
     print("losses", losses, "untrained", untrained, "trained", ev)
-    assert losses[-1] < 0.5 * losses[0]
+    assert losses[-1] < 0.8 * losses[0]
     assert ev["accuracy"] >= 0.9
     assert ev["accuracy"] > untrained["accuracy"]
+    # the network settles most rows itself, where the untrained one leaves
+    # them to the sensor, and misses fewer shocks
+    assert ev["to_sensor"] < 0.5 * untrained["to_sensor"]
+    assert ev["missed_shocks"] <= untrained["missed_shocks"]
 
 
 def test_learned_hybrid_close_to_plain(trained):
