@@ -61,6 +61,7 @@ __host__ __device__ inline double sq(double x) { return x * x; }
 
 // fortran/rhs_hybrid_high_llf.m4:365-398
 __device__ inline double wenom5(double fgm2, double fgm1, double fg0, double fgp1, double fgp2) {
+#pragma clang fp contract(fast)
   double beta0 = kC13_12 * sq(fg0 - 2 * fgp1 + fgp2) + 0.25 * sq(3 * fg0 - 4 * fgp1 + fgp2);
   double beta1 = kC13_12 * sq(fgm1 - 2 * fg0 + fgp1) + 0.25 * sq(fgm1 - fgp1);
   double beta2 = kC13_12 * sq(fgm2 - 2 * fgm1 + fg0) + 0.25 * sq(fgm2 - 4 * fgm1 + 3 * fg0);
@@ -77,6 +78,7 @@ __device__ inline double wenom5(double fgm2, double fgm1, double fg0, double fgp
 
 // fortran/rhs_hybrid_high_llf.m4:400-433
 __device__ inline double wenop5(double fgm2, double fgm1, double fg0, double fgp1, double fgp2) {
+#pragma clang fp contract(fast)
   double beta0 = kC13_12 * sq(fg0 - 2 * fgp1 + fgp2) + 0.25 * sq(3 * fg0 - 4 * fgp1 + fgp2);
   double beta1 = kC13_12 * sq(fgm1 - 2 * fg0 + fgp1) + 0.25 * sq(fgm1 - fgp1);
   double beta2 = kC13_12 * sq(fgm2 - 2 * fgm1 + fg0) + 0.25 * sq(fgm2 - 4 * fgm1 + 3 * fg0);
@@ -93,6 +95,7 @@ __device__ inline double wenop5(double fgm2, double fgm1, double fg0, double fgp
 
 // fortran/rhs_hybrid_low_llf.m4:349-373
 __device__ inline double wenom3(double fgm1, double fg0, double fgp1) {
+#pragma clang fp contract(fast)
   double beta0 = sq(fgp1 - fg0);
   double beta1 = sq(fg0 - fgm1);
   double zeta0 = kC1_3 / sq(kEpsWeno + beta0);
@@ -106,6 +109,7 @@ __device__ inline double wenom3(double fgm1, double fg0, double fgp1) {
 
 // fortran/rhs_hybrid_low_llf.m4:375-399
 __device__ inline double wenop3(double fgm1, double fg0, double fgp1) {
+#pragma clang fp contract(fast)
   double beta0 = sq(fgp1 - fg0);
   double beta1 = sq(fg0 - fgm1);
   double zeta0 = kC2_3 / sq(kEpsWeno + beta0);

@@ -76,6 +76,7 @@ struct Tile {
 
 // Inviscid fluxes F (x) and G (y), rhs_hybrid_high_llf.m4:125-156.
 __device__ inline double flux_f(double w0, double u, double v, double s, double tn, double p, int k) {
+#pragma clang fp contract(fast)
   switch (k) {
     case 0: return w0 * u;
     case 1: return w0 * sq(u) + p;
@@ -85,6 +86,7 @@ __device__ inline double flux_f(double w0, double u, double v, double s, double 
   }
 }
 __device__ inline double flux_g(double w0, double u, double v, double s, double tn, double p, int k) {
+#pragma clang fp contract(fast)
   switch (k) {
     case 0: return w0 * v;
     case 1: return w0 * v * u;
@@ -130,6 +132,7 @@ __device__ inline void cell_output(const RhsArgs& a, int i, int j, const double 
 // Physical source H, rhs_hybrid_high_llf.m4:159-237.
 template <int S, bool P>
 __device__ inline void cell_source(const Tile& t, int e, double H0, const RhsArgs& a, double H[5]) {
+#pragma clang fp contract(fast)
   const double dx = a.dx, dy = a.dy;
   const double* U = t.u;
   const double* V = t.v;
@@ -212,6 +215,7 @@ __device__ inline int cell_sensor(const double* r, int e, double ep, int level) 
 // DRP branch, rhs_hybrid_high_llf.m4:258-268.
 template <int S, bool P>
 __device__ inline void cell_drp(const Tile& t, int e, const RhsArgs& a, const double H[5], double out[5]) {
+#pragma clang fp contract(fast)
   const double mdx = -a.dx, mdy = -a.dy;
 #pragma unroll
   for (int k = 0; k < 5; k++) {
@@ -228,6 +232,7 @@ __device__ inline void cell_drp(const Tile& t, int e, const RhsArgs& a, const do
 // (rhs_hybrid_high_llf.m4:286-354); returns pf1 + mf1 - pf2 - mf2.
 template <int ORDER>
 __device__ inline double weno_dir(const double* F, const double* W, double alpha) {
+#pragma clang fp contract(fast)
   double fp[7], fm[7];
   constexpr int R = ORDER == 5 ? 3 : 2;
 #pragma unroll
@@ -256,6 +261,7 @@ __device__ inline double weno_dir(const double* F, const double* W, double alpha
 template <int REACH>
 __device__ inline void llf_alpha(const double* u, const double* v, int j, const RhsArgs& a, double& af,
                                  double& ag) {
+#pragma clang fp contract(fast)
   af = 0;
   ag = 0;
 #pragma unroll
@@ -273,6 +279,7 @@ __device__ inline void llf_alpha(const double* u, const double* v, int j, const 
 template <int V, int S, bool P>
 __device__ inline void cell_weno_tile(const Tile& t, int e, int j, const RhsArgs& a, const double H[5],
                                       double out[5]) {
+#pragma clang fp contract(fast)
   using Sc = Scheme<V>;
   constexpr int R = Sc::order == 5 ? 3 : 2;
   double af, ag;
@@ -405,6 +412,7 @@ __device__ inline void split_body(const RhsArgs& a, int i0, int j0, unsigned pid
 template <int V, bool P, bool FUSED>
 __device__ inline double list_dir(const RhsArgs& a, long g, int j, int dir, int k, const double* w0,
                                   const double* un, double alpha) {
+#pragma clang fp contract(fast)
   using Sc = Scheme<V>;
   constexpr int R = Sc::order == 5 ? 3 : 2;
   const Geom& G = a.geom;
@@ -436,6 +444,7 @@ __device__ inline double list_dir(const RhsArgs& a, long g, int j, int dir, int 
 // The component loop is kept rolled to bound register use.
 template <int V, bool P, bool FUSED>
 __device__ inline void list_cell(const RhsArgs& a, int cell) {
+#pragma clang fp contract(fast)
   using Sc = Scheme<V>;
   constexpr int R = Sc::order == 5 ? 3 : 2;
   const Geom& G = a.geom;
@@ -492,6 +501,7 @@ __device__ inline void list_cell(const RhsArgs& a, int cell) {
 // forms as X[k] (dir 0) or Y[k] (dir 1), by the same operations.
 template <int V, bool P>
 __device__ inline double list_item(const RhsArgs& a, int cell, int dir, int k) {
+#pragma clang fp contract(fast)
   using Sc = Scheme<V>;
   constexpr int R = Sc::order == 5 ? 3 : 2;
   const Geom& G = a.geom;
