@@ -66,7 +66,7 @@ namespace {
 // One tile per workgroup: sensor + DRP cells finish here, WENO cells go to
 // the list (split_body).  The fused epilogue is capped at 128 VGPRs.
 template <int V, bool P, int BX, int BY>
-__global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(5)))
 batch_split_kernel(const int4* __restrict__ tiles, int ntiles, const RhsArgs* __restrict__ args) {
   ENL_BATCH_LDS(BX, BY, L, false)
   const int4 tl = tiles[xcd_tile((int)blockIdx.x, ntiles)];
