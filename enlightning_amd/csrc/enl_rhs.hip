@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(Sh::threads) __attribute__((amdgpu_waves_per_e
 // The fused form carries the runge_kutta and update_state epilogue.  Both
 // split kernels are capped at 5 waves per SIMD (<= 102 VGPRs), what the LDS
 // of a tile allows anyway (5 tiles of 29.8 KB per CU); the cap also gives
-// the better schedule (0.756 vs 0.816 ms per fused 4096^2 stage).
+// the better schedule (0.74 vs 0.82 ms per fused 4096^2 stage).
 template <int V, bool P, bool FUSED>
 __global__ void __launch_bounds__(Sh::threads) __attribute__((amdgpu_waves_per_eu(5)))
 rhs_split_fused_kernel(RhsArgs a) {

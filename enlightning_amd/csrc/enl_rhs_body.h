@@ -329,10 +329,12 @@ __device__ inline void stage_tile(const RhsArgs& a, int i0, int j0, TileLds<BX, 
       s0[e] = w0;
       sp[e] = a.p[gidx];
       sT[e] = a.T[gidx];
-      su[e] = w1 / w0;
-      sv[e] = w2 / w0;
-      ss[e] = w3 / w0;
-      stn[e] = w4 / w0;
+      // one division per staged cell: the four quotients through 1/w0
+      const double iw0 = 1.0 / w0;
+      su[e] = w1 * iw0;
+      sv[e] = w2 * iw0;
+      ss[e] = w3 * iw0;
+      stn[e] = w4 * iw0;
       if constexpr (RAW) {
         L.w[1][e] = w1;
         L.w[2][e] = w2;
