@@ -210,7 +210,7 @@ int do_rhs(int variant, const int* ifirst0, const int* ilast0, const int* ifirst
   if ((rc = bind(args[3], 3, pres, ng, true, false))) return rc;
   if ((rc = bind(args[4], 4, temp, ng, true, false))) return rc;
   if (switch_map && (rc = bind(args[5], 5, switch_map, (size_t)G.nx * G.ny * sizeof(int), false, true))) return rc;
-  RhsArgs a;
+  RhsArgs a = {};
   a.w = (const double*)args[0].dev;
   a.H0 = (const double*)args[1].dev;
   a.h0_sy = G.nx;

@@ -250,7 +250,7 @@ void fill_all(Batch* b, int buf, const int* reflect, hipStream_t s) {
 
 // Arguments common to every patch: gas constants and the WENO list.
 RhsArgs base_args(Batch* b, const enl_gas_t* gas) {
-  RhsArgs base;
+  RhsArgs base = {};
   memset(&base, 0, sizeof base);
   base.alpha = gas->weno_alpha;
   base.kappa = gas->kappa;

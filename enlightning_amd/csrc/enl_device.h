@@ -133,12 +133,66 @@ __host__ __device__ inline double sensor_eps(double tol) {
   return kC09 * tol * sq(xi) / (1 - kC09 * sq(tol));
 }
 
+// FP64 division through a shared reciprocal.  hipcc expands n / d into
+// v_div_scale of both operands (range scaling), v_rcp_f64 and two Newton
+// steps on the reciprocal, the quotient n*r, its residual, the corrected
+// quotient (v_div_fmas) and v_div_fixup (special operands).  When neither
+// operand needs scaling, both v_div_scale return their input, v_div_fmas is
+// a plain fma and v_div_fixup returns the quotient: rcp_nr and div_nr below
+// are then that same sequence, so div_nr(n, d, rcp_nr(d)) == n / d bit for
+// bit, and one reciprocal serves every quotient by d.  No scaling happens
+// when |d| is in [2^-50, 2^50] (den_ok) and n is zero or |n| is in
+// [2^-960, 2^700] (num_ok): the exponents differ by less than 768, the
+// exponent of n is above the sequence's tiny-numerator bound (2^-969) and
+// 1/d, n/d are normal.  A zero numerator gives +0 (n / d gives -0 for
+// n = -0, d > 0; the two compare equal).
+__device__ inline double rcp_nr(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ inline double div_nr(double n, double d, double r) {
+  const double q = n * r;
+  const double e = fma(-d, q, n);
+  return fma(e, r, q);
+}
+__host__ __device__ inline bool den_ok(double d) {
+  const double m = fabs(d);
+  return m >= 0x1p-50 && m <= 0x1p50;
+}
+__host__ __device__ inline bool num_ok(double n) {
+  const double m = fabs(n);
+  return (m >= 0x1p-960 && m <= 0x1p700) || n == 0;
+}
+
 // Pressure and temperature of one cell, update_state.m4:88-105 (xc1 is the
 // cell-centre height xylo(1)+dxy(1)*(j-ifirst1+0.5)).
 __device__ inline void eos_p_T(double c, double c_lr, double rho0, double gamma1, double c_p, double p0, double T0,
                                double c_v, double R, double xc1, double w0, double w3, double& p, double& T) {
   p = sq(c + c_lr * xc1) * ((w0 - rho0) + ((gamma1 - 1) / (2 * rho0)) * sq(w0 - rho0) + (w0 / c_p) * (w3 / w0)) + p0;
   T = T0 * exp((w3 / w0 - R * log(rho0 / w0)) / c_v);
+}
+
+// eos_p_T with its quotients through rcp_nr / div_nr where the operands
+// allow it, else through eos_p_T; the same p and T.  k1 is
+// (gamma1 - 1) / (2 * rho0) formed on the host (IEEE, as the device forms
+// it); consts_ok says rho0 passes num_ok and c_p, c_v pass den_ok.
+__device__ inline void eos_p_T_nr(double c, double c_lr, double rho0, double gamma1, double c_p, double p0,
+                                  double T0, double c_v, double R, double k1, bool consts_ok, double xc1, double w0,
+                                  double w3, double& p, double& T) {
+  if (consts_ok && den_ok(w0) && num_ok(w3)) {
+    const double rw = rcp_nr(w0);
+    const double s = div_nr(w3, w0, rw);
+    const double x = s - R * log(div_nr(rho0, w0, rw));
+    if (num_ok(x)) {
+      p = sq(c + c_lr * xc1) * ((w0 - rho0) + k1 * sq(w0 - rho0) + div_nr(w0, c_p, rcp_nr(c_p)) * s) + p0;
+      T = T0 * exp(div_nr(x, c_v, rcp_nr(c_v)));
+      return;
+    }
+  }
+  eos_p_T(c, c_lr, rho0, gamma1, c_p, p0, T0, c_v, R, xc1, w0, w3, p, T);
 }
 
 }  // namespace enl

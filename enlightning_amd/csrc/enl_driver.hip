@@ -102,7 +102,7 @@ extern "C" int enl_advance_patch(int variant, int ifirst0, int ilast0, int ifirs
   hipStream_t s = lib_stream();
   Geom G = make_geom(ifirst0, ilast0, ifirst1, ilast1, gcw, gcw);
 
-  RhsArgs ra;
+  RhsArgs ra = {};
   ra.w = w_n;
   ra.p = pres;
   ra.T = temp;

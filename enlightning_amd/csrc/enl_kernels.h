@@ -62,6 +62,10 @@ struct RhsArgs {
   double* T_out;
   double alpha_1, alpha_2, bdt;  // bdt = beta*dt
   double rho0, gamma1, c_p, p0, T0, c_v;
+  // filled with the reciprocals: (gamma1 - 1) / (2 * rho0), and whether the
+  // per-run operands of the source / update_state quotients admit div_nr
+  double eos_k1;
+  bool fast_div, eos_ok;
 };
 
 struct RkArgs {

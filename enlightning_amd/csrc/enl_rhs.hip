@@ -58,12 +58,12 @@ using Sh = TileShape<kBX, kBY>;
   }
 
 // Tile origin of this workgroup: a 1-D grid of nbx * nby tiles, in XCD
-// chunks, row-major within a chunk.
+// chunks of vertical strips (strip_tile).
 #define ENL_TILE_ORIGIN(a, i0, j0)                                      \
   const int nbx_ = (a.geom.nx + kBX - 1) / kBX;                         \
   const int nby_ = (a.geom.ny + kBY - 1) / kBY;                         \
-  const int t_ = xcd_tile((int)blockIdx.x, nbx_ * nby_);                \
-  const int i0 = (t_ % nbx_) * kBX, j0 = (t_ / nbx_) * kBY;
+  const int2 t_ = strip_tile((int)blockIdx.x, nbx_, nby_);              \
+  const int i0 = t_.x * kBX, j0 = t_.y * kBY;
 
 template <int V, bool P, bool FUSED>
 __global__ void __launch_bounds__(Sh::threads) __attribute__((amdgpu_waves_per_eu(5))) rhs_split_kernel(RhsArgs a) {

@@ -31,6 +31,22 @@ __device__ inline int xcd_tile(int b, int n) {
   return x * q + (x < r ? x : r) + (b >> 3);
 }
 
+constexpr int kStrip = 32;
+// Tile (tx, ty) of workgroup b on an nbx x nby tile grid.  The linear tile
+// order runs through vertical strips kStrip tiles wide, row-major inside
+// a strip, and xcd_tile gives each XCD a contiguous range of it.  A tile's
+// upper neighbour then follows it kStrip tiles later on the same XCD,
+// while the halo rows they share are still in that XCD's L2; in row-major
+// order a whole row of tiles (128 at 4096^2) passes between them, more data
+// than one XCD's 4 MB L2 keeps.
+__device__ inline int2 strip_tile(int b, int nbx, int nby) {
+  const int t = xcd_tile(b, nbx * nby);
+  const int full = kStrip * nby;
+  const int s = t / full, r = t - s * full;
+  const int w = min(kStrip, nbx - s * kStrip);
+  return make_int2(s * kStrip + r % w, r / w);
+}
+
 // LDS of one staged tile.  w[1..4] (the raw momenta etc.) are only staged by
 // the dense kernel; w[0] is rho.
 template <int BX, int BY>
@@ -50,6 +66,11 @@ __host__ __device__ inline void fill_rhs_constants(RhsArgs& a) {
   a.inv_mdy = 1.0 / (-a.dy);
   a.inv_dx = 1.0 / a.dx;
   a.inv_dy = 1.0 / a.dy;
+  // operands of the source and update_state quotients that are the same
+  // for every cell (div_nr's range conditions)
+  a.fast_div = den_ok(a.tau_n) && num_ok(a.T_star_n) && num_ok(a.mu) && num_ok(a.muB) && num_ok(a.kappa);
+  a.eos_k1 = (a.gamma1 - 1) / (2 * a.rho0);
+  a.eos_ok = num_ok(a.rho0) && den_ok(a.c_p) && den_ok(a.c_v);
 }
 
 // a / c for a grid constant c; a * (1/c) when 1/c is exact (POW2).
@@ -123,15 +144,20 @@ __device__ inline void cell_output(const RhsArgs& a, int i, int j, const double 
     }
     const double xc1 = a.ylo + a.dy * ((double)(j) + 0.5);
     double p, T;
-    eos_p_T(a.c, a.c_lr, a.rho0, a.gamma1, a.c_p, a.p0, a.T0, a.c_v, a.R, xc1, wn[0], wn[3], p, T);
+    eos_p_T_nr(a.c, a.c_lr, a.rho0, a.gamma1, a.c_p, a.p0, a.T0, a.c_v, a.R, a.eos_k1, a.eos_ok, xc1, wn[0], wn[3], p,
+               T);
     a.p_out[g] = p;
     a.T_out[g] = T;
   }
 }
 
-// Physical source H, rhs_hybrid_high_llf.m4:159-237.
-template <int S, bool P>
-__device__ inline void cell_source(const Tile& t, int e, double H0, const RhsArgs& a, double H[5]) {
+// Physical source H, rhs_hybrid_high_llf.m4:159-237.  The thirteen
+// quotients by tau_n, T_n, temp, 2*temp and temp^2 are formed as n / d
+// (NR false) or through one reciprocal per denominator (NR true: rcp_nr /
+// div_nr, the same bits where every operand passes den_ok / num_ok); the
+// NR form returns false, its H to be discarded, where one does not.
+template <int S, bool P, bool NR>
+__device__ inline bool cell_source_q(const Tile& t, int e, double H0, const RhsArgs& a, double H[5]) {
 #pragma clang fp contract(fast)
   const double dx = a.dx, dy = a.dy;
   const double* U = t.u;
@@ -165,20 +191,41 @@ __device__ inline void cell_source(const Tile& t, int e, double H0, const RhsArg
   double w0 = t.w0[e];
   double temp = TT[e];
   double T_n = t.tn[e];
-  double Mn = (temp - T_n) / a.tau_n;
-  double tmp1 = a.T_star_n / (T_n);
+  const double temp2 = sq(temp);
+  const double nx1 = a.kappa * sq(t_prime_x), nx2 = a.kappa * t_dprime_x;
+  const double ny1 = a.kappa * sq(t_prime_y), ny2 = a.kappa * t_dprime_y;
+  bool ok = true;
+  double rtau = 0, rTn = 0, rT = 0, r2T = 0, rT2 = 0;
+  if constexpr (NR) {
+    ok = den_ok(temp2) && den_ok(T_n) && num_ok(w0) && num_ok(temp - T_n) && num_ok(nx1) && num_ok(nx2) &&
+         num_ok(ny1) && num_ok(ny2);
+    rtau = rcp_nr(a.tau_n);
+    rTn = rcp_nr(T_n);
+    rT = rcp_nr(temp);
+    r2T = rcp_nr(2 * temp);
+    rT2 = rcp_nr(temp2);
+  }
+  auto dv = [](double n, double d, double r) { return NR ? div_nr(n, d, r) : n / d; };
+  double Mn = dv(temp - T_n, a.tau_n, rtau);
+  double tmp1 = dv(a.T_star_n, T_n, rTn);
   double c_vn = kC078 * a.R * sq(tmp1) * exp(-tmp1);
-  double A_n = ((temp / T_n) - 1) * c_vn;
+  double A_n = (dv(temp, T_n, rTn) - 1) * c_vn;
   double phi_xx = kC4_3 * u_prime_x - kC2_3 * v_prime_y;
   double phi_xy_yx = u_prime_y + v_prime_x;
   double phi_yy = kC4_3 * v_prime_y - kC2_3 * u_prime_x;
-  double sigma = (a.muB / temp) * sq(u_prime_x + v_prime_y) +
-                 (a.mu / (2 * temp)) * (sq(phi_xx) + 2 * sq(phi_xy_yx) + sq(phi_yy)) +
-                 (a.kappa / sq(temp)) * (sq(t_prime_x) + sq(t_prime_y)) + (w0 / temp) * (A_n * Mn);
-  H[3] = sigma - (w0 / T_n) * c_vn * Mn +
-         (-((a.kappa * sq(t_prime_x)) / sq(temp)) + ((a.kappa * t_dprime_x) / temp) -
-          ((a.kappa * sq(t_prime_y)) / sq(temp)) + ((a.kappa * t_dprime_y) / temp));
-  H[4] = (w0 / a.tau_n) * (temp - T_n);
+  double sigma = dv(a.muB, temp, rT) * sq(u_prime_x + v_prime_y) +
+                 dv(a.mu, 2 * temp, r2T) * (sq(phi_xx) + 2 * sq(phi_xy_yx) + sq(phi_yy)) +
+                 dv(a.kappa, temp2, rT2) * (sq(t_prime_x) + sq(t_prime_y)) + dv(w0, temp, rT) * (A_n * Mn);
+  H[3] = sigma - dv(w0, T_n, rTn) * c_vn * Mn +
+         (-dv(nx1, temp2, rT2) + dv(nx2, temp, rT) - dv(ny1, temp2, rT2) + dv(ny2, temp, rT));
+  H[4] = dv(w0, a.tau_n, rtau) * (temp - T_n);
+  return ok;
+}
+
+template <int S, bool P>
+__device__ inline void cell_source(const Tile& t, int e, double H0, const RhsArgs& a, double H[5]) {
+  if (a.fast_div && cell_source_q<S, P, true>(t, e, H0, a, H)) return;
+  cell_source_q<S, P, false>(t, e, H0, a, H);
 }
 
 // r = (2|t1 t2| + ep) / (t1^2 + t2^2 + ep) < float(0.9999)?  The numerator
@@ -212,18 +259,26 @@ __device__ inline int cell_sensor(const double* r, int e, double ep, int level) 
   return tag;
 }
 
-// DRP branch, rhs_hybrid_high_llf.m4:258-268.
+// DRP branch, rhs_hybrid_high_llf.m4:258-268, without the source:
+// out[k] = (F_x)/(-dx) + (G_y)/(-dy); the caller adds H[k] (the reference
+// sums the three in that order).  The x stencil is finished before the y
+// stencil's loads are issued (sched_barrier), which keeps the twelve
+// points' values from being live at once.
 template <int S, bool P>
-__device__ inline void cell_drp(const Tile& t, int e, const RhsArgs& a, const double H[5], double out[5]) {
+__device__ inline void cell_drp(const Tile& t, int e, const RhsArgs& a, double out[5]) {
 #pragma clang fp contract(fast)
   const double mdx = -a.dx, mdy = -a.dy;
+  double fx[5];
+#pragma unroll
+  for (int k = 0; k < 5; k++)
+    fx[k] = -kA3 * tflux_f(t, e - 3, k) - kA2 * tflux_f(t, e - 2, k) - kA1 * tflux_f(t, e - 1, k) +
+            kA1 * tflux_f(t, e + 1, k) + kA2 * tflux_f(t, e + 2, k) + kA3 * tflux_f(t, e + 3, k);
+  __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int k = 0; k < 5; k++) {
-    double fx = -kA3 * tflux_f(t, e - 3, k) - kA2 * tflux_f(t, e - 2, k) - kA1 * tflux_f(t, e - 1, k) +
-                kA1 * tflux_f(t, e + 1, k) + kA2 * tflux_f(t, e + 2, k) + kA3 * tflux_f(t, e + 3, k);
     double gy = -kA3 * tflux_g(t, e - 3 * S, k) - kA2 * tflux_g(t, e - 2 * S, k) - kA1 * tflux_g(t, e - S, k) +
                 kA1 * tflux_g(t, e + S, k) + kA2 * tflux_g(t, e + 2 * S, k) + kA3 * tflux_g(t, e + 3 * S, k);
-    out[k] = divc<P>(fx, mdx, a.inv_mdx) + divc<P>(gy, mdy, a.inv_mdy) + H[k];
+    out[k] = divc<P>(fx[k], mdx, a.inv_mdx) + divc<P>(gy, mdy, a.inv_mdy);
   }
 }
 
@@ -374,16 +429,27 @@ __device__ inline void split_body(const RhsArgs& a, int i0, int j0, unsigned pid
     const int e = (threadIdx.y + Sh::halo) * Sh::TW + threadIdx.x + Sh::halo;
     const long icell = (long)i + (long)j * G.nx;
     const long ik = (long)G.nx * G.ny;
-    double H[5];
-    cell_source<Sh::TW, P>(t, e, a.H0[(long)i + (long)j * a.h0_sy], a, H);
     if constexpr (Sc::hybrid) {
       const int d = a.decide ? (int)a.decide[icell] : 2;
       tag = d == 2 ? cell_sensor<Sh::TW>(t.w0, e, a.ep, a.level) : d;
     } else
       tag = Sc::drp ? 0 : 1;
+    // Hybrid DRP cells: the stencil part first, so that H is not live
+    // across it (the DRP-only variant keeps source then stencil)
+    double out[5] = {0, 0, 0, 0, 0};
+    double H[5];
+    const double* h0 = &a.H0[(long)i + (long)j * a.h0_sy];
+    if constexpr (Sc::hybrid) {
+      if (tag == 0) cell_drp<Sh::TW, P>(t, e, a, out);
+      __builtin_amdgcn_sched_barrier(0);
+      cell_source<Sh::TW, P>(t, e, *h0, a, H);
+    } else {
+      cell_source<Sh::TW, P>(t, e, *h0, a, H);
+      if (tag == 0) cell_drp<Sh::TW, P>(t, e, a, out);
+    }
     if (tag == 0) {
-      double out[5];
-      cell_drp<Sh::TW, P>(t, e, a, H, out);
+#pragma unroll
+      for (int k = 0; k < 5; k++) out[k] = out[k] + H[k];
       cell_output<FUSED>(a, i, j, out);
     } else {
 #pragma unroll
