@@ -125,6 +125,18 @@ __device__ inline double tflux_g(const Tile& t, int e, int k) {
 
 // Result of one cell: store rhs (plain), or apply the runge_kutta update
 // and update_state and store the new state (fused stage).
+// The fused stage's operands of the runge_kutta update at cell (i, j):
+// ww (wa) and the stage input w (wb).
+__device__ inline void rk_operands(const RhsArgs& a, int i, int j, double wa[5], double wb[5]) {
+  const Geom& G = a.geom;
+  const long g = (long)(i + G.g0) + (long)(j + G.g1) * G.sy;
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    wa[k] = a.ww[a.ww_off + i + j * a.ww_sy + k * a.ww_sk];
+    wb[k] = a.w[g + k * G.sk];
+  }
+}
+
 template <bool FUSED>
 __device__ inline void cell_output(const RhsArgs& a, int i, int j, const double K[5]) {
   const Geom& G = a.geom;
@@ -135,11 +147,13 @@ __device__ inline void cell_output(const RhsArgs& a, int i, int j, const double 
     for (int k = 0; k < 5; k++) a.KK[c + k * ik] = K[k];
   } else {
     const long g = (long)(i + G.g0) + (long)(j + G.g1) * G.sy;
-    double wn[5];
+    // the ten loads before the first store, so that they are in flight
+    // together (w_out aliases neither, which the compiler cannot know)
+    double wa[5], wb[5], wn[5];
+    rk_operands(a, i, j, wa, wb);
 #pragma unroll
     for (int k = 0; k < 5; k++) {
-      wn[k] = a.alpha_1 * a.ww[a.ww_off + i + j * a.ww_sy + k * a.ww_sk] + a.alpha_2 * a.w[g + k * G.sk] +
-              a.bdt * K[k];
+      wn[k] = a.alpha_1 * wa[k] + a.alpha_2 * wb[k] + a.bdt * K[k];
       a.w_out[g + k * G.sk] = wn[k];
     }
     const double xc1 = a.ylo + a.dy * ((double)(j) + 0.5);
@@ -372,30 +386,45 @@ __device__ inline void cell_weno_tile(const Tile& t, int e, int j, const RhsArgs
 template <int BX, int BY, bool RAW>
 __device__ inline void stage_tile(const RhsArgs& a, int i0, int j0, TileLds<BX, BY>& L) {
   using Sh = TileShape<BX, BY>;
+  constexpr int kIt = (Sh::TN + Sh::threads - 1) / Sh::threads;
   const Geom& G = a.geom;
   const int tid = threadIdx.x + threadIdx.y * BX;
   double *s0 = L.w[0], *sp = L.p, *sT = L.T, *su = L.u, *sv = L.v, *ss = L.s, *stn = L.tn;
-  for (int e = tid; e < Sh::TN; e += Sh::threads) {
-    int gi = i0 + (e % Sh::TW) - Sh::halo, gj = j0 + (e / Sh::TW) - Sh::halo;
-    if (gi >= -G.g0 && gi < G.nx + G.g0 && gj >= -G.g1 && gj < G.ny + G.g1) {
-      long gidx = (long)(gi + G.g0) + (long)(gj + G.g1) * G.sy;
-      double w0 = a.w[gidx], w1 = a.w[gidx + G.sk], w2 = a.w[gidx + 2 * G.sk];
-      double w3 = a.w[gidx + 3 * G.sk], w4 = a.w[gidx + 4 * G.sk];
-      s0[e] = w0;
-      sp[e] = a.p[gidx];
-      sT[e] = a.T[gidx];
-      // one division per staged cell: the four quotients through 1/w0
-      const double iw0 = 1.0 / w0;
-      su[e] = w1 * iw0;
-      sv[e] = w2 * iw0;
-      ss[e] = w3 * iw0;
-      stn[e] = w4 * iw0;
-      if constexpr (RAW) {
-        L.w[1][e] = w1;
-        L.w[2][e] = w2;
-        L.w[3][e] = w3;
-        L.w[4][e] = w4;
-      }
+  // Every load of the thread's staged cells is issued before the first is
+  // used: a cell outside the tile or the ghost box reads element 0 (always
+  // in the arrays) and is not stored.
+  double v[kIt][7];
+  bool in[kIt];
+#pragma unroll
+  for (int it = 0; it < kIt; it++) {
+    const int e = tid + it * Sh::threads;
+    const int gi = i0 + (e % Sh::TW) - Sh::halo, gj = j0 + (e / Sh::TW) - Sh::halo;
+    in[it] = e < Sh::TN && gi >= -G.g0 && gi < G.nx + G.g0 && gj >= -G.g1 && gj < G.ny + G.g1;
+    const long gidx = in[it] ? (long)(gi + G.g0) + (long)(gj + G.g1) * G.sy : 0;
+#pragma unroll
+    for (int k = 0; k < 5; k++) v[it][k] = a.w[gidx + k * G.sk];
+    v[it][5] = a.p[gidx];
+    v[it][6] = a.T[gidx];
+  }
+#pragma unroll
+  for (int it = 0; it < kIt; it++) {
+    if (!in[it]) continue;
+    const int e = tid + it * Sh::threads;
+    const double w0 = v[it][0], w1 = v[it][1], w2 = v[it][2], w3 = v[it][3], w4 = v[it][4];
+    s0[e] = w0;
+    sp[e] = v[it][5];
+    sT[e] = v[it][6];
+    // one division per staged cell: the four quotients through 1/w0
+    const double iw0 = 1.0 / w0;
+    su[e] = w1 * iw0;
+    sv[e] = w2 * iw0;
+    ss[e] = w3 * iw0;
+    stn[e] = w4 * iw0;
+    if constexpr (RAW) {
+      L.w[1][e] = w1;
+      L.w[2][e] = w2;
+      L.w[3][e] = w3;
+      L.w[4][e] = w4;
     }
   }
 }
