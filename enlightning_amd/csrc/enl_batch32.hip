@@ -242,16 +242,31 @@ __global__ void __launch_bounds__(BX* BY) stage32_kernel(const int4* __restrict_
     for (int q = tid; q < r_hi * W; q += BX * BY) dst[q] = src[q];
   }
   __syncthreads();
-  for (int e = tid; e < TN; e += BX * BY) {
+  // every load of the thread's staged cells is issued before the first is
+  // used; a cell outside the tile or the ghost box reads element 0 and is
+  // not stored
+  constexpr int kIt = (TN + BX * BY - 1) / (BX * BY);
+  float v[kIt][7];
+  bool in[kIt];
+#pragma unroll
+  for (int it = 0; it < kIt; it++) {
+    const int e = tid + it * BX * BY;
     const int gi = i0 + (e % TW) - 3, gj = jr + e / TW;
-    if (gi >= -a.g && gi < a.nx + a.g && gj >= -a.g && gj < a.ny + a.g) {
-      const long gx = (long)(gi + a.g) + (long)(gj + a.g) * a.sy;
+    in[it] = e < TN && gi >= -a.g && gi < a.nx + a.g && gj >= -a.g && gj < a.ny + a.g;
+    const long gx = in[it] ? (long)(gi + a.g) + (long)(gj + a.g) * a.sy : 0;
+#pragma unroll
+    for (int k = 0; k < 7; k++) v[it][k] = a.in[gx + k * a.sk];
+  }
+#pragma unroll
+  for (int it = 0; it < kIt; it++) {
+    const int e = tid + it * BX * BY;
+    if (in[it]) {
       const int r = e / TW;
       float w[5];
 #pragma unroll
-      for (int k = 0; k < 5; k++) L.w[k][e] = w[k] = a.in[gx + k * a.sk];
-      L.p[e] = a.in[gx + 5 * a.sk];
-      L.T[e] = a.in[gx + 6 * a.sk];
+      for (int k = 0; k < 5; k++) L.w[k][e] = w[k] = v[it][k];
+      L.p[e] = v[it][5];
+      L.T[e] = v[it][6];
       const float rho = L.row[r].aRho + w[0];
       const float ir = rcpf(rho);
       L.rho[e] = rho;
@@ -400,11 +415,18 @@ __global__ void __launch_bounds__(BX* BY) stage32_kernel(const int4* __restrict_
     return;
   } else {
     // ---- FP64 RK accumulation (runge_kutta.m4:60-63) of the perturbation
-    double wn[5];
+    // the loads before the first store, so that they are in flight together
+    double wn[5], wcv[5];
+    float dkv[5];
 #pragma unroll
     for (int k = 0; k < 5; k++) {
-      const double kd = (double)K[k] + (double)a.dk[ic + k * nn];
-      wn[k] = a.alpha_1 * a.wc[ic + k * nn] + a.alpha_2 * (double)L.w[k][e] + a.bdt * kd;
+      dkv[k] = a.dk[ic + k * nn];
+      wcv[k] = a.wc[ic + k * nn];
+    }
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      const double kd = (double)K[k] + (double)dkv[k];
+      wn[k] = a.alpha_1 * wcv[k] + a.alpha_2 * (double)L.w[k][e] + a.bdt * kd;
       a.out[gx + k * a.sk] = (float)wn[k];
       if (a.last) a.wc[ic + k * nn] = wn[k];
     }

@@ -194,20 +194,31 @@ __global__ void decide_reset_kernel(unsigned* n) {
 }
 
 // Prefilter: DRP where the sensor provably says DRP, else a candidate.  A
-// 32 x 8 tile of cells with its 2-cell halo of rho is staged in LDS; the
-// candidates are compacted with one atomic per wave.
+// 32 x 8 tile of cells with its 2-cell halo of rho is staged in LDS (every
+// load of a thread issued before the first store); the candidates are
+// compacted with one atomic per workgroup.
 constexpr int kPBX = 32, kPBY = 8, kPW = kPBX + 4, kPH = kPBY + 4;
+constexpr int kPWaves = kPBX * kPBY / 64;
 
 __global__ void __launch_bounds__(kPBX * kPBY) prefilter_kernel(DecideArgs a) {
   __shared__ double rt[kPH * kPW];
+  __shared__ unsigned wcount[kPWaves], wbase[kPWaves];
   const int nbx = (a.nx + kPBX - 1) / kPBX;
   const int i0 = (blockIdx.x % nbx) * kPBX, j0 = (blockIdx.x / nbx) * kPBY;
   const int tid = threadIdx.x;
-  for (int e = tid; e < kPH * kPW; e += kPBX * kPBY) {
+  constexpr int kIt = (kPH * kPW + kPBX * kPBY - 1) / (kPBX * kPBY);
+  double v[kIt];
+  bool in[kIt];
+#pragma unroll
+  for (int it = 0; it < kIt; it++) {
+    const int e = tid + it * kPBX * kPBY;
     const int gi = i0 + e % kPW - 2, gj = j0 + e / kPW - 2;
-    if (gi >= -a.g && gi < a.nx + a.g && gj >= -a.g && gj < a.ny + a.g)
-      rt[e] = a.F.r[(long)(gi + a.g) + (long)(gj + a.g) * a.F.sy];
+    in[it] = e < kPH * kPW && gi >= -a.g && gi < a.nx + a.g && gj >= -a.g && gj < a.ny + a.g;
+    v[it] = a.F.r[in[it] ? (long)(gi + a.g) + (long)(gj + a.g) * a.F.sy : 0];
   }
+#pragma unroll
+  for (int it = 0; it < kIt; it++)
+    if (in[it]) rt[tid + it * kPBX * kPBY] = v[it];
   __syncthreads();
   const int i = i0 + tid % kPBX, j = j0 + tid / kPBX;
   bool cand = false;
@@ -226,13 +237,22 @@ __global__ void __launch_bounds__(kPBX * kPBY) prefilter_kernel(DecideArgs a) {
     if (!cand) a.decide[c] = 0;
   }
   const unsigned long long m = __ballot(cand);
-  if (m) {
-    const int lead = __ffsll((long long)m) - 1;
-    unsigned base = 0;
-    if ((int)(threadIdx.x & 63) == lead) base = atomicAdd(a.ncand, (unsigned)__popcll(m));
-    base = __shfl(base, lead);
-    if (cand) a.cand[base + __popcll(m & __lanemask_lt())] = (unsigned)c;
+  const int wave = tid >> 6;
+  if ((tid & 63) == 0) wcount[wave] = (unsigned)__popcll(m);
+  __syncthreads();
+  if (tid == 0) {
+    unsigned tot = 0;
+#pragma unroll
+    for (int w = 0; w < kPWaves; w++) tot += wcount[w];
+    unsigned b = tot ? atomicAdd(a.ncand, tot) : 0u;
+#pragma unroll
+    for (int w = 0; w < kPWaves; w++) {
+      wbase[w] = b;
+      b += wcount[w];
+    }
   }
+  __syncthreads();
+  if (cand) a.cand[wbase[wave] + __popcll(m & __lanemask_lt())] = (unsigned)c;
 }
 
 // Inference on the matrix cores: each wave takes 16 candidates at a time
