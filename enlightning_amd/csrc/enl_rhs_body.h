@@ -250,7 +250,9 @@ __device__ inline void cell_source(const Tile& t, int e, double H0, const RhsArg
 __device__ inline bool ratio_below(double tmp1, double tmp2, double ep) {
   double num = 2 * fabs(tmp1 * tmp2) + ep;
   double den = sq(tmp1) + sq(tmp2) + ep;
-  if (num > 1e-30 && num < 1e30 && den > 1e-30 && den < 1e30) {
+  // den >= num up to rounding (t1^2 + t2^2 >= 2|t1 t2|), so these two bounds
+  // keep both within a few ulps of [1e-30, 1e30], normal in single precision
+  if (num > 1e-30 && den < 1e30) {
     float q = (float)num * __builtin_amdgcn_rcpf((float)den);
     if (q < (float)(kThresh * (1 - 1e-6))) return true;
     if (q > (float)(kThresh * (1 + 1e-6))) return false;
@@ -395,11 +397,16 @@ __device__ inline void stage_tile(const RhsArgs& a, int i0, int j0, TileLds<BX, 
   // in the arrays) and is not stored.
   double v[kIt][7];
   bool in[kIt];
+  // the whole tile and halo inside the ghost box (every tile of a grid
+  // whose sides are multiples of the tile's, with g >= halo): no per-cell
+  // bounds test
+  const bool inside = i0 - Sh::halo >= -G.g0 && i0 + Sh::TW - Sh::halo <= G.nx + G.g0 &&
+                      j0 - Sh::halo >= -G.g1 && j0 + Sh::TH - Sh::halo <= G.ny + G.g1;
 #pragma unroll
   for (int it = 0; it < kIt; it++) {
     const int e = tid + it * Sh::threads;
     const int gi = i0 + (e % Sh::TW) - Sh::halo, gj = j0 + (e / Sh::TW) - Sh::halo;
-    in[it] = e < Sh::TN && gi >= -G.g0 && gi < G.nx + G.g0 && gj >= -G.g1 && gj < G.ny + G.g1;
+    in[it] = e < Sh::TN && (inside || (gi >= -G.g0 && gi < G.nx + G.g0 && gj >= -G.g1 && gj < G.ny + G.g1));
     const long gidx = in[it] ? (long)(gi + G.g0) + (long)(gj + G.g1) * G.sy : 0;
 #pragma unroll
     for (int k = 0; k < 5; k++) v[it][k] = a.w[gidx + k * G.sk];
