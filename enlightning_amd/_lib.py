@@ -61,7 +61,8 @@ PROTOTYPES = {
     "enl_profile_stop": ([_P, _P, _I], _I),
     "enl_fused_scratch_bytes": ([_I] * 5, ctypes.c_size_t),
     "enl_advance_patch_fused": ([_I] * 6 + [_P, _P, ctypes.POINTER(EnlGas), _P, _I, _P, _P, _P, _D, _I, _I]
-                                + [_P] * 5 + [ctypes.c_size_t], _I),
+                                + [_P] * 5 + [ctypes.c_size_t, _D], _I),
+    "enl_absorb": ([_I] * 6 + [_P] * 5 + [_D, ctypes.POINTER(EnlGas)] + [_P] * 3, _I),
     "enl_batch_create": ([_I, ctypes.POINTER(EnlPatchDesc), _I, _I, ctypes.POINTER(ctypes.c_void_p)], _I),
     "enl_batch_set_fill": ([_P, _I, _I] + [_P] * 4 + [_I] + [_P] * 6 + [_I] + [_P] * 4 + [_I] + [_P] * 4, _I),
     "enl_batch_set_coarsen": ([_P, _I] + [_P] * 4, _I),
@@ -79,7 +80,7 @@ PROTOTYPES = {
                            _P], _I),
     "enl_advance_patch_learned": ([_P, ctypes.c_float, ctypes.c_float, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P,
                                    ctypes.POINTER(EnlGas), _P, _I, _P, _P, _P, ctypes.c_double, _I, _P, _P, _P, _P,
-                                   _P, ctypes.c_size_t], _I),
+                                   _P, ctypes.c_size_t, _D], _I),
     "enl_batch_set_ambient": ([_P, _I, _I, _I, _P, _P, _P, ctypes.POINTER(EnlGas)], _I),
     "enl_batch_fp32_prepare": ([_P, _I, ctypes.POINTER(EnlGas), _P], _I),
     "enl_batch_fp32_load": ([_P], _I),
@@ -92,7 +93,7 @@ PROTOTYPES = {
     "enl_batch_advance": ([_P, _I, _I, ctypes.POINTER(EnlGas), _P, _I, _P, _P, _P, _D, _I], _I),
     "enl_batch_destroy": ([_P], _I),
     "enl_advance_patch": ([_I] * 6 + [_P, _P, ctypes.POINTER(EnlGas), _P, _I, _P, _P, _P, _D, _I, _I]
-                          + [_P] * 6, _I),
+                          + [_P] * 6 + [_D], _I),
 }
 
 _lib = None

@@ -86,11 +86,29 @@ extern "C" int enl_profile_stop(double* ms, int* count, int nkinds) {
   return g_timer.overflow ? set_error(-7, "profile capacity exceeded; later launches were not timed") : 0;
 }
 
+// Absorbing envelopes of a patch that covers the whole domain: the sides
+// whose reflect flag is 0 (ABSORB) when absorb_width > 0; the domain is
+// [xlo, xlo + n*dx] in each direction.
+static int whole_domain_absorb(const Geom& G, const double* xlo, const double* dx, const int* reflect, double absorb_width,
+                               AbsorbPlan* plan, const double** dtab, bool* on, hipStream_t s) {
+  int ab[4];
+  bool any = false;
+  for (int k = 0; k < 4; k++) {
+    ab[k] = reflect[k] == 0;
+    any = any || ab[k];
+  }
+  *on = any && absorb_width > 0;
+  if (!*on) return 0;
+  const double xhi[2] = {xlo[0] + dx[0] * (double)G.nx, xlo[1] + dx[1] * (double)G.ny};
+  int rc = absorb_plan(G, xlo, dx, xlo, xhi, ab, absorb_width, plan);
+  return rc ? rc : absorb_upload(*plan, dtab, s);
+}
+
 extern "C" int enl_advance_patch(int variant, int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw,
                                  const double* xlo, const double* dx, const enl_gas_t* gas, const int* reflect,
                                  int order, const double* alpha_1, const double* alpha_2, const double* beta,
                                  double dt, int nsteps, int level, double* w_n, double* pres, double* temp,
-                                 double* source, double* ww, double* KK) {
+                                 double* source, double* ww, double* KK, double absorb_width) {
   if (variant < 0 || variant >= kNumVariants) return set_error(-1, "unknown rhs variant");
   int need = (variant == kWenoLowLlf || variant == kWenoLow) ? 2 : 3;
   if (gcw < need) return set_error(-2, "ghost width below the stencil reach");
@@ -170,7 +188,11 @@ extern "C" int enl_advance_patch(int variant, int ifirst0, int ilast0, int ifirs
     bc.reflect[k] = reflect[k];
   }
 
-  int rc = 0;
+  AbsorbPlan plan;
+  const double* dtab = nullptr;
+  bool absorb_on = false;
+  int rc = whole_domain_absorb(G, xlo, dx, reflect, absorb_width, &plan, &dtab, &absorb_on, s);
+  if (rc) return rc;
   long t;
   hipError_t e = hipSuccess;
   for (int step = 0; step < nsteps && e == hipSuccess; step++) {
@@ -196,6 +218,7 @@ extern "C" int enl_advance_patch(int variant, int ifirst0, int ilast0, int ifirs
       t = g_timer.begin(ENL_KERNEL_STATE, s);
       e = launch_update_state(st, s);
       g_timer.end(t, s);
+      if (absorb_on && e == hipSuccess) e = launch_absorb(plan, dtab, w_n, pres, temp, G, gas, s);
     }
   }
   if (e != hipSuccess) rc = check_hip(e, "patch loop launch");
@@ -219,16 +242,18 @@ struct LearnedSwitch {
 int advance_fused(const LearnedSwitch& L, int variant, int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw,
                   const double* xlo, const double* dx, const enl_gas_t* gas, const int* reflect, int order,
                   const double* alpha_1, const double* alpha_2, const double* beta, double dt, int nsteps, int level,
-                  double* w_n, double* pres, double* temp, double* source, void* scratch, size_t scratch_bytes);
+                  double* w_n, double* pres, double* temp, double* source, void* scratch, size_t scratch_bytes,
+                  double absorb_width);
 }  // namespace
 
 extern "C" int enl_advance_patch_fused(int variant, int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw,
                                        const double* xlo, const double* dx, const enl_gas_t* gas, const int* reflect,
                                        int order, const double* alpha_1, const double* alpha_2, const double* beta,
                                        double dt, int nsteps, int level, double* w_n, double* pres, double* temp,
-                                       double* source, void* scratch, size_t scratch_bytes) {
+                                       double* source, void* scratch, size_t scratch_bytes, double absorb_width) {
   return advance_fused(LearnedSwitch(), variant, ifirst0, ilast0, ifirst1, ilast1, gcw, xlo, dx, gas, reflect, order,
-                       alpha_1, alpha_2, beta, dt, nsteps, level, w_n, pres, temp, source, scratch, scratch_bytes);
+                       alpha_1, alpha_2, beta, dt, nsteps, level, w_n, pres, temp, source, scratch, scratch_bytes,
+                       absorb_width);
 }
 
 extern "C" int enl_advance_patch_learned(void* sw, float lo, float hi, unsigned char* decide,
@@ -237,7 +262,7 @@ extern "C" int enl_advance_patch_learned(void* sw, float lo, float hi, unsigned 
                                          const enl_gas_t* gas, const int* reflect, int order, const double* alpha_1,
                                          const double* alpha_2, const double* beta, double dt, int nsteps,
                                          double* w_n, double* pres, double* temp, double* source, void* scratch,
-                                         size_t scratch_bytes) {
+                                         size_t scratch_bytes, double absorb_width) {
   if (!sw || !decide) return set_error(-2, "enl_advance_patch_learned: no switch or decision map");
   if (!(variant == kHybridHighLlf || variant == kHybridHigh || variant == kHybridLowLlf || variant == kHybridLow))
     return set_error(-1, "the learned switch replaces the sensor of the hybrid variants only");
@@ -251,14 +276,15 @@ extern "C" int enl_advance_patch_learned(void* sw, float lo, float hi, unsigned 
   L.decide = decide;
   L.totals = totals;
   return advance_fused(L, variant, ifirst0, ilast0, ifirst1, ilast1, gcw, xlo, dx, gas, reflect, order, alpha_1,
-                       alpha_2, beta, dt, nsteps, 0, w_n, pres, temp, source, scratch, scratch_bytes);
+                       alpha_2, beta, dt, nsteps, 0, w_n, pres, temp, source, scratch, scratch_bytes, absorb_width);
 }
 
 namespace {
 int advance_fused(const LearnedSwitch& L, int variant, int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw,
                   const double* xlo, const double* dx, const enl_gas_t* gas, const int* reflect, int order,
                   const double* alpha_1, const double* alpha_2, const double* beta, double dt, int nsteps, int level,
-                  double* w_n, double* pres, double* temp, double* source, void* scratch, size_t scratch_bytes) {
+                  double* w_n, double* pres, double* temp, double* source, void* scratch, size_t scratch_bytes,
+                  double absorb_width) {
   if (variant < 0 || variant >= kNumVariants) return set_error(-1, "unknown rhs variant");
   int need = (variant == kWenoLowLlf || variant == kWenoLow) ? 2 : 3;
   if (gcw < need) return set_error(-2, "ghost width below the stencil reach");
@@ -327,6 +353,10 @@ int advance_fused(const LearnedSwitch& L, int variant, int ifirst0, int ilast0, 
     bc.reflect[k] = reflect[k];
   }
 
+  AbsorbPlan plan;
+  const double* dtab = nullptr;
+  bool absorb_on = false;
+  if (int rc = whole_domain_absorb(G, xlo, dx, reflect, absorb_width, &plan, &dtab, &absorb_on, s)) return rc;
   hipError_t e = hipSuccess;
   long t;
   for (int step = 0; step < nsteps && e == hipSuccess; step++) {
@@ -362,6 +392,7 @@ int advance_fused(const LearnedSwitch& L, int variant, int ifirst0, int ilast0, 
       t = g_timer.begin(ENL_KERNEL_STAGE, s);
       e = launch_stage(variant, ra, s);
       g_timer.end(t, s);
+      if (absorb_on && e == hipSuccess) e = launch_absorb(plan, dtab, out.w, out.p, out.T, G, gas, s);
       in = out;
     }
     if (order == 1 && e == hipSuccess) {

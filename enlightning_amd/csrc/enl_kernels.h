@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "enlightning_hip.h"
 #include <stdint.h>
 
@@ -125,6 +127,24 @@ int switch_accumulate(Switch* sw, unsigned long long* totals, hipStream_t s);
 int set_error(int code, const char* fmt, ...);
 int check_hip(hipError_t e, const char* what);
 bool is_device_ptr(const void* p);
+// Absorbing envelopes (enl_absorb.hip): the cells each ABSORB side touches
+// (normal index range, interior-relative; left / right may skip the first
+// row) and their tmp table, built on the host; uploaded to a per-device
+// table and applied by one launch per side.
+struct AbsorbSide {
+  int lo = 0, hi = -1;
+  bool skip_first = false;
+  int off = 0;
+};
+struct AbsorbPlan {
+  AbsorbSide side[4];
+  std::vector<double> tab;
+};
+int absorb_plan(const Geom& G, const double* patch_xlo, const double* dx, const double* grid_xlo,
+                const double* grid_xhi, const int* absorb, double width, AbsorbPlan* out);
+int absorb_upload(const AbsorbPlan& plan, const double** dtab, hipStream_t s);
+hipError_t launch_absorb(const AbsorbPlan& plan, const double* dtab, double* w, double* p, double* T, const Geom& G,
+                         const enl_gas_t* gas, hipStream_t s);
 // Device workspace of the current device (grow-only, owned by the library):
 // the WENO-cell list for `cells` cells and its counter.
 int rhs_workspace(long cells, unsigned long long** list, unsigned** count);

@@ -89,6 +89,18 @@ def fill_physical_bc(box, g, touch, reflect, w, p=None, T=None, source=None):
     _lib.check(rc)
 
 
+def absorb(box, g, patch_xlo, dx, grid_xlo, grid_xhi, sides, width, gas, w, p, T):
+    """Absorbing envelopes (enl_absorb, enlightning.C:630-786) on the sides
+    with sides[s] set (left, right, bottom, top); device arrays."""
+    lib = _lib.load()
+    use_torch_stream()
+    gs = gas.to_struct()
+    rc = lib.enl_absorb(box[0], box[1], box[2], box[3], g, g, _darr(patch_xlo), _darr(dx), _darr(grid_xlo),
+                        _darr(grid_xhi), (_c_int * 4)(*sides), float(width), ctypes.byref(gs), _ptr(w), _ptr(p),
+                        _ptr(T))
+    _lib.check(rc)
+
+
 class DevicePatch:
     """One patch covering the whole domain, resident in HBM.
 
@@ -110,6 +122,8 @@ class DevicePatch:
         self.ww = torch.empty((5, wl.n, wl.n), **opts)
         self.KK = torch.empty((5, wl.n, wl.n), **opts)
         self.reflect = (1, 1, 1, 1)
+        # absorbing envelopes on the sides whose reflect flag is 0, when > 0
+        self.absorb_width = 0.0
 
     def load_host(self, w, p, T, src):
         self.w.copy_(torch.from_numpy(w))
@@ -136,7 +150,7 @@ class DevicePatch:
             vid, self.box[0], self.box[1], self.box[2], self.box[3], self.g, _darr(self.xlo),
             _darr(self.dx), ctypes.byref(gas_s), (_c_int * 4)(*self.reflect), len(a1), _darr(a1),
             _darr(a2), _darr(b), float(dt), int(nsteps), int(level), _ptr(self.w), _ptr(self.p),
-            _ptr(self.T), _ptr(self.src), _ptr(self.ww), _ptr(self.KK))
+            _ptr(self.T), _ptr(self.src), _ptr(self.ww), _ptr(self.KK), float(self.absorb_width))
         _lib.check(rc)
 
     def advance_fused(self, nsteps, dt, variant=None, level=0, rk=RK3):
@@ -155,7 +169,8 @@ class DevicePatch:
             vid, self.box[0], self.box[1], self.box[2], self.box[3], self.g, _darr(self.xlo),
             _darr(self.dx), ctypes.byref(gas_s), (_c_int * 4)(*self.reflect), len(a1), _darr(a1),
             _darr(a2), _darr(b), float(dt), int(nsteps), int(level), _ptr(self.w), _ptr(self.p),
-            _ptr(self.T), _ptr(self.src), ctypes.c_void_p(self._scratch.data_ptr()), nbytes)
+            _ptr(self.T), _ptr(self.src), ctypes.c_void_p(self._scratch.data_ptr()), nbytes,
+            float(self.absorb_width))
         _lib.check(rc)
 
     def to_host(self):

@@ -120,7 +120,7 @@ class Switch:
             self.handle, float(lo), float(hi), _ptr(dp._decide), _ptr(totals), vid, dp.box[0], dp.box[1],
             dp.box[2], dp.box[3], dp.g, D(*dp.xlo), D(*dp.dx), ctypes.byref(gs), (ctypes.c_int * 4)(*dp.reflect),
             len(a1), D(*a1), D(*a2), D(*b), float(dt), int(nsteps), _ptr(dp.w), _ptr(dp.p), _ptr(dp.T),
-            _ptr(dp.src), _ptr(dp._scratch), nbytes))
+            _ptr(dp.src), _ptr(dp._scratch), nbytes, float(getattr(dp, "absorb_width", 0.0))))
         return tuple(int(v) for v in totals.cpu())
 
     def close(self):

@@ -138,18 +138,33 @@ int enl_fill_physical_bc(int ifirst0, int ilast0, int ifirst1, int ilast1, int g
                          const int *touch, const int *reflect, double *w_n, double *pres,
                          double *temp, double *source);
 
+/* Absorbing envelopes of one patch: replaces
+ * enlightning::setAbsorbingBoundaryConditions, enlightning.C:630-786.
+ * absorb[4]: side (left, right, bottom, top) has bc_type ABSORB
+ * (input.txt:14-17); width: bc_absorb_width (input.txt:18); patch_xlo: the
+ * patch's lower corner, grid_xlo / grid_xhi: the domain's; dx: cell size.
+ * Damps the interior of w_n, pres, temp (device arrays) toward rho0, 0,
+ * T0*rho0, p0, T0 with the reference's factors and cell selection, sides in
+ * the reference's order.  Returns 0 or a negative error code. */
+int enl_absorb(int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw0, int gcw1, const double *patch_xlo,
+               const double *dx, const double *grid_xlo, const double *grid_xhi, const int *absorb, double width,
+               const enl_gas_t *gas, double *w_n, double *pres, double *temp);
+
 /* nsteps method-of-lines RK steps of one patch that covers the whole domain
  * (all four sides physical): per step ww <- w_n(interior), then per stage
  * the physical fill, rhs, runge_kutta and update_state, i.e. the per-patch
  * sequence of enlightning::singleStep (enlightning.C:459-583) inside the
  * SAMRAI MethodOfLinesIntegrator stage loop.  alpha_1/alpha_2/beta hold
  * `order` coefficients (input.txt:129-138).  ww and KK are device scratch of
- * nx*ny*5 doubles.  Returns 0 or a negative error code. */
+ * nx*ny*5 doubles.  absorb_width > 0: after each stage's update_state the
+ * absorbing envelopes of enl_absorb on the sides whose reflect flag is 0
+ * (enlightning.C:581-582), the domain being [xlo, xlo + n*dx]; <= 0: none.
+ * Returns 0 or a negative error code. */
 int enl_advance_patch(int variant, int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw,
                       const double *xlo, const double *dx, const enl_gas_t *gas, const int *reflect,
                       int order, const double *alpha_1, const double *alpha_2, const double *beta,
                       double dt, int nsteps, int level, double *w_n, double *pres, double *temp,
-                      double *source, double *ww, double *KK);
+                      double *source, double *ww, double *KK, double absorb_width);
 
 /* Bytes of device scratch enl_advance_patch_fused needs for this patch:
  * two more ghosted states (w_n, pres, temp) and an interior rhs. */
@@ -166,7 +181,7 @@ int enl_advance_patch_fused(int variant, int ifirst0, int ilast0, int ifirst1, i
                             const double *xlo, const double *dx, const enl_gas_t *gas, const int *reflect,
                             int order, const double *alpha_1, const double *alpha_2, const double *beta,
                             double dt, int nsteps, int level, double *w_n, double *pres, double *temp,
-                            double *source, void *scratch, size_t scratch_bytes);
+                            double *source, void *scratch, size_t scratch_bytes, double absorb_width);
 
 /* ---- 3. batched hierarchy (device patch table) -------------------------- */
 
@@ -287,7 +302,7 @@ int enl_advance_patch_learned(void *sw, float lo, float hi, unsigned char *decid
                               const double *xlo, const double *dx, const enl_gas_t *gas, const int *reflect, int order,
                               const double *alpha_1, const double *alpha_2, const double *beta, double dt, int nsteps,
                               double *w_n, double *pres, double *temp, double *source, void *scratch,
-                              size_t scratch_bytes);
+                              size_t scratch_bytes, double absorb_width);
 
 /* ---- measurement hooks (no reference counterpart) ---------------------- */
 

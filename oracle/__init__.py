@@ -68,8 +68,10 @@ def load():
     lib.orc_fill_physical_bc.argtypes = [_I] * 6 + [_P] * 6
     lib.orc_fill_physical_bc.restype = None
     lib.orc_advance_patch.argtypes = ([_I] * 6 + [_P, _P, ctypes.POINTER(OrcGas), _P, _I, _P, _P, _P, _D, _I, _I]
-                                      + [_P] * 6 + [_I, _I])
+                                      + [_P] * 6 + [_I, _I, _D])
     lib.orc_advance_patch.restype = _I
+    lib.orc_absorb.argtypes = [_I] * 6 + [_P] * 5 + [_D, ctypes.POINTER(OrcGas)] + [_P] * 3
+    lib.orc_absorb.restype = None
     _lib = lib
     return lib
 
@@ -169,6 +171,14 @@ def fill_physical_bc(box, g, touch, reflect, w, p=None, T=None, source=None):
                                 _ptr(w), _ptr(p), _ptr(T), _ptr(source))
 
 
+def absorb(box, g, patch_xlo, dx, grid_xlo, grid_xhi, sides, width, gas, w, p, T):
+    """Absorbing envelopes on the sides with sides[s] set (left, right, bottom,
+    top), enlightning.C:630-786."""
+    gs = gas_struct(gas)
+    load().orc_absorb(box[0], box[1], box[2], box[3], g, g, _darr(patch_xlo), _darr(dx), _darr(grid_xlo),
+                      _darr(grid_xhi), (_I * 4)(*sides), float(width), ctypes.byref(gs), _ptr(w), _ptr(p), _ptr(T))
+
+
 class HostPatch:
     """Host mirror of enlightning_amd.patch.DevicePatch driven by the oracle."""
 
@@ -179,6 +189,7 @@ class HostPatch:
         self.xlo = (0.0, 0.0)
         self.dx = (wl.dx, wl.dx)
         self.reflect = (1, 1, 1, 1)
+        self.absorb_width = 0.0  # envelopes on the sides whose reflect is 0 when > 0
         self.ww = np.zeros((5, wl.n, wl.n))
         self.KK = np.zeros((5, wl.n, wl.n))
 
@@ -197,6 +208,6 @@ class HostPatch:
             vid, self.box[0], self.box[1], self.box[2], self.box[3], self.g, _darr(self.xlo), _darr(self.dx),
             ctypes.byref(gs), (_I * 4)(*self.reflect), len(a1), _darr(a1), _darr(a2), _darr(b), float(dt),
             int(nsteps), int(level), _ptr(self.w), _ptr(self.p), _ptr(self.T), _ptr(self.src), _ptr(self.ww),
-            _ptr(self.KK), int(tile), int(nthreads))
+            _ptr(self.KK), int(tile), int(nthreads), float(self.absorb_width))
         if rc != 0:
             raise ValueError("orc_advance_patch failed (%d)" % rc)

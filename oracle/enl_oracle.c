@@ -582,6 +582,86 @@ void orc_fill_physical_bc(int if0, int il0, int if1, int il1, int g0, int g1, co
 #undef IDX
 }
 
+/* Absorbing envelopes, enlightning::setAbsorbingBoundaryConditions
+ * (enlightning.C:630-786).  Sides with absorb[s] set (0 left, 1 right,
+ * 2 bottom, 3 top, in that order) damp the interior cells near them toward
+ * the ambient by tmp = -exp(-log(2.0) / W^2 * d^2) + 1, d the distance of
+ * the cell centre from the domain edge.  Each side is walked from the
+ * patch edge inward: a cell (row) is processed while the previous tmp is
+ * more than 1e-8 from 1; the test before the first cell uses the edge
+ * cell's own tmp, and along the left / right sides every row after the
+ * first restarts with tmp = 0. */
+static double orc_env(double d, double width) { return -exp(-log(2.0) / (width * width) * (d * d)) + 1; }
+
+static void orc_damp(double *w_n, double *pres, double *temp, long area, long c, double tmp,
+                     const enl_gas_t *gas) {
+  pres[c] = (pres[c] - gas->p0) * tmp + gas->p0;
+  temp[c] = (temp[c] - gas->T0) * tmp + gas->T0;
+  w_n[c] = (w_n[c] - gas->rho0) * tmp + gas->rho0;
+  w_n[area + c] = w_n[area + c] * tmp;
+  w_n[2 * area + c] = w_n[2 * area + c] * tmp;
+  w_n[3 * area + c] = w_n[3 * area + c] * tmp;
+  w_n[4 * area + c] = (w_n[4 * area + c] - gas->T0 * gas->rho0) * tmp + gas->T0 * gas->rho0;
+}
+
+/* The patch box [if0, il0] x [if1, il1] inside arrays whose ghost box is
+ * [af0 - g0, al0 + g0] x [af1 - g1, al1 + g1]. */
+static void absorb_box(int if0, int il0, int if1, int il1, int af0, int al0, int af1, int al1, int g0, int g1,
+                       const double *pxlo, const double *dx, const double *gxlo, const double *gxhi,
+                       const int *absorb, double width, const enl_gas_t *gas, double *w_n, double *pres,
+                       double *temp) {
+  const long nxg = al0 - af0 + 1 + 2L * g0, area = nxg * (al1 - af1 + 1 + 2L * g1);
+  const double ep = 1.0e-8;
+#define C_(i, j) ((long)((j) - af1 + g1) * nxg + (long)((i) - af0 + g0))
+#define XC0(i) (pxlo[0] + dx[0] * ((double)((i) - if0) + 0.5))
+#define XC1(j) (pxlo[1] + dx[1] * ((double)((j) - if1) + 0.5))
+  double tmp;
+  if (absorb[0]) {
+    tmp = orc_env(XC0(if0) - gxlo[0], width);
+    for (int j = if1; j <= il1; j++) {
+      for (int i = if0; fabs(1.0 - tmp) > ep && i <= il0; i++) {
+        tmp = orc_env(XC0(i) - gxlo[0], width);
+        orc_damp(w_n, pres, temp, area, C_(i, j), tmp, gas);
+      }
+      tmp = 0;
+    }
+  }
+  if (absorb[1]) {
+    tmp = orc_env(gxhi[0] - XC0(il0), width);
+    for (int j = if1; j <= il1; j++) {
+      for (int i = il0; fabs(1.0 - tmp) > ep && i >= if0; i--) {
+        tmp = orc_env(gxhi[0] - XC0(i), width);
+        orc_damp(w_n, pres, temp, area, C_(i, j), tmp, gas);
+      }
+      tmp = 0;
+    }
+  }
+  if (absorb[2]) {
+    tmp = orc_env(XC1(if1) - gxlo[1], width);
+    for (int j = if1; fabs(1.0 - tmp) > ep && j <= il1; j++) {
+      tmp = orc_env(XC1(j) - gxlo[1], width);
+      for (int i = if0; i <= il0; i++) orc_damp(w_n, pres, temp, area, C_(i, j), tmp, gas);
+    }
+  }
+  if (absorb[3]) {
+    tmp = orc_env(gxhi[1] - XC1(il1), width);
+    for (int j = il1; fabs(1.0 - tmp) > ep && j >= if1; j--) {
+      tmp = orc_env(gxhi[1] - XC1(j), width);
+      for (int i = if0; i <= il0; i++) orc_damp(w_n, pres, temp, area, C_(i, j), tmp, gas);
+    }
+  }
+#undef C_
+#undef XC0
+#undef XC1
+}
+
+void orc_absorb(int if0, int il0, int if1, int il1, int g0, int g1, const double *pxlo, const double *dx,
+                const double *gxlo, const double *gxhi, const int *absorb, double width, const enl_gas_t *gas,
+                double *w_n, double *pres, double *temp) {
+  absorb_box(if0, il0, if1, il1, if0, il0, if1, il1, g0, g1, pxlo, dx, gxlo, gxhi, absorb, width, gas, w_n, pres,
+             temp);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Single-patch MOL driver (the shape of SAMRAI's advanceHierarchy around    */
 /* enlightning::singleStep, for one patch covering the whole domain).        */
@@ -607,7 +687,7 @@ int orc_advance_patch(int variant, int if0, int il0, int if1, int il1, int g, co
                       const double *dxy, const enl_gas_t *gas, const int *reflect, int order,
                       const double *alpha_1, const double *alpha_2, const double *beta, double dt,
                       int nsteps, int level, double *w_n, double *pres, double *temp, double *source,
-                      double *ww, double *KK, int tile, int nthreads) {
+                      double *ww, double *KK, int tile, int nthreads, double absorb_width) {
   scheme_t s = scheme_of(variant);
   if (s.order < 0) return -1;
   if (g < s.reach) return -2;
@@ -655,6 +735,20 @@ int orc_advance_patch(int variant, int if0, int il0, int if1, int il1, int g, co
         double txlo[2] = {xylo[0] + dxy[0] * (b0 - if0), xylo[1] + dxy[1] * (b1 - if1)};
         rk_box(&w, &a, &K, b0, e0, b1, e1, dt, alpha_1[st], alpha_2[st], beta[st]);
         state_box(&w, &p, &T, b0, e0, b1, e1, b1, dxy, txlo, &S);
+      }
+      if (absorb_width > 0 && (!reflect[0] || !reflect[1] || !reflect[2] || !reflect[3])) {
+        /* envelopes on the ABSORB sides, per patch (tile) as singleStep
+         * applies them (enlightning.C:581-582); the domain is this patch */
+        int ab[4] = {!reflect[0], !reflect[1], !reflect[2], !reflect[3]};
+        double gxhi[2] = {xylo[0] + dxy[0] * (double)nx, xylo[1] + dxy[1] * (double)ny};
+        for (int t = 0; t < ntiles; t++) {
+          int b0 = if0 + (t % ntx) * tsz, b1 = if1 + (t / ntx) * tsz;
+          int e0 = b0 + tsz - 1 < il0 ? b0 + tsz - 1 : il0;
+          int e1 = b1 + tsz - 1 < il1 ? b1 + tsz - 1 : il1;
+          double txlo[2] = {xylo[0] + dxy[0] * (b0 - if0), xylo[1] + dxy[1] * (b1 - if1)};
+          absorb_box(b0, e0, b1, e1, if0, il0, if1, il1, g, g, txlo, dxy, xylo, gxhi, ab, absorb_width, gas, w_n,
+                     pres, temp);
+        }
       }
     }
   }

@@ -67,7 +67,11 @@ int orc_advance_patch(int variant, int if0, int il0, int if1, int il1, int g, co
                       const double *dxy, const enl_gas_t *gas, const int *reflect, int order,
                       const double *alpha_1, const double *alpha_2, const double *beta, double dt,
                       int nsteps, int level, double *w_n, double *pres, double *temp, double *source,
-                      double *ww, double *KK, int tile, int nthreads);
+                      double *ww, double *KK, int tile, int nthreads, double absorb_width);
+
+void orc_absorb(int if0, int il0, int if1, int il1, int g0, int g1, const double *pxlo, const double *dx,
+                const double *gxlo, const double *gxhi, const int *absorb, double width, const enl_gas_t *gas,
+                double *w_n, double *pres, double *temp);
 
 #ifdef __cplusplus
 }
