@@ -91,6 +91,7 @@ PROTOTYPES = {
     "enl_batch_fill": ([_P, _P], _I),
     "enl_batch_update_state": ([_P, ctypes.POINTER(EnlGas)], _I),
     "enl_batch_advance": ([_P, _I, _I, ctypes.POINTER(EnlGas), _P, _I, _P, _P, _P, _D, _I], _I),
+    "enl_batch_set_absorb": ([_P, _P, _P, _D], _I),
     "enl_batch_destroy": ([_P], _I),
     "enl_advance_patch": ([_I] * 6 + [_P, _P, ctypes.POINTER(EnlGas), _P, _I, _P, _P, _P, _D, _I, _I]
                           + [_P] * 6 + [_D], _I),

@@ -36,6 +36,13 @@ struct LevelFill {
 
 struct Fp32Form;  // enl_batch32.hip
 
+// One patch's rectangle of cells touched by one absorbing side (see
+// enl_absorb.hip): normal index range [lo, hi], tangential [t0, t0 + nt),
+// tmp at tab[off + normal index].
+struct AbsRect {
+  int patch, lo, hi, t0, nt, off;
+};
+
 struct Batch {
   int npatch = 0, g = 0, ratio = 0, nlevels = 0;
   std::vector<PatchDev> hp;
@@ -62,6 +69,13 @@ struct Batch {
   hipGraphExec_t graph_exec = nullptr;
   std::vector<unsigned char> graph_key;
   Fp32Form* f32 = nullptr;  // the FP32-perturbation form's buffers, once set up
+  // absorbing envelopes (enl_batch_set_absorb): per side the rectangles of
+  // every patch, grouped by level, and their tmp tables
+  bool absorb = false;
+  double ab_glo[2] = {0, 0}, ab_ghi[2] = {0, 0}, ab_width = 0;
+  AbsRect* ab_rects = nullptr;
+  double* ab_tab = nullptr;
+  int ab_begin[4][9] = {{0}};
 };
 
 inline unsigned grid_for(long n, int block = 256, long cap = 4096) {
@@ -85,6 +99,10 @@ inline int upload(Batch* b, T** dst, const T* src, long n) {
 // writes block 0.
 inline int stage_out(int st, int order) { return (st == order - 1 && order > 1) ? 0 : (st % 2 == 0 ? 1 : 2); }
 inline int stage_in(int st, int order) { return st == 0 ? 0 : stage_out(st - 1, order); }
+
+// Absorbing envelopes of one level's patches in state block `block`, on the
+// sides whose reflect flag is 0, in the reference's side order.
+int absorb_level(Batch* b, int lev, int block, const int* reflect, const enl_gas_t* gas, hipStream_t s);
 
 // Ghost fill of one level of FP64 state block `buf` (-1: the source) from the
 // level's schedule; the coarse-to-fine refine reads block `cbuf` of the

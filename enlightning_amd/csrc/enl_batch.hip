@@ -381,7 +381,58 @@ int stage_variant(Batch* b, int variant, const RhsArgs* args, int st, int lev, h
 // output), step the level's tiles, and refresh the level's ghosts on its
 // output for the next finer level's refine.  Level 0 has no coarse-to-fine
 // part, so after stage 0 its refresh is already the next stage's fill.
-int enqueue_batched(Batch* b, int variant, const int* reflect, int order, int nsteps, hipStream_t s) {
+// Absorbing envelopes of one level: one launch per absorbing side (left,
+// right, bottom, top), one workgroup per patch rectangle.
+__global__ void __launch_bounds__(256) batch_absorb_kernel(double* pool, const PatchDev* __restrict__ dp,
+                                                           const AbsRect* __restrict__ rects,
+                                                           const double* __restrict__ tab, int side, int block,
+                                                           double rho0, double p0, double T0, double T0rho0) {
+  const AbsRect r = rects[blockIdx.x];
+  const PatchDev pd = dp[r.patch];
+  double* w = pool + pd.state[block];
+  double* p = w + 5 * pd.sk;
+  double* T = w + 6 * pd.sk;
+  const int nn = r.hi - r.lo + 1;
+  const long n = (long)nn * r.nt;
+  for (long q = threadIdx.x; q < n; q += blockDim.x) {
+    int i, j;
+    if (side < 2) {
+      i = (int)(q % nn) + r.lo;
+      j = (int)(q / nn) + r.t0;
+    } else {
+      i = (int)(q % r.nt) + r.t0;
+      j = (int)(q / r.nt) + r.lo;
+    }
+    const double tmp = tab[r.off + (side < 2 ? i : j)];
+    const long g = (long)(i + pd.g) + (long)(j + pd.g) * pd.sy;
+    p[g] = (p[g] - p0) * tmp + p0;
+    T[g] = (T[g] - T0) * tmp + T0;
+    w[g] = (w[g] - rho0) * tmp + rho0;
+    w[g + pd.sk] = w[g + pd.sk] * tmp;
+    w[g + 2 * pd.sk] = w[g + 2 * pd.sk] * tmp;
+    w[g + 3 * pd.sk] = w[g + 3 * pd.sk] * tmp;
+    w[g + 4 * pd.sk] = (w[g + 4 * pd.sk] - T0rho0) * tmp + T0rho0;
+  }
+}
+
+}  // namespace
+
+int absorb_level(Batch* b, int lev, int block, const int* reflect, const enl_gas_t* gas, hipStream_t s) {
+  if (!b->absorb) return 0;
+  for (int side = 0; side < 4; side++) {
+    if (reflect[side]) continue;
+    const int r0 = b->ab_begin[side][lev], r1 = b->ab_begin[side][lev + 1];
+    if (r1 <= r0) continue;
+    hipLaunchKernelGGL(batch_absorb_kernel, dim3((unsigned)(r1 - r0)), dim3(256), 0, s, b->pool, b->dp,
+                       b->ab_rects + r0, b->ab_tab, side, block, gas->rho0, gas->p0, gas->T0, gas->T0 * gas->rho0);
+  }
+  return check_hip(hipGetLastError(), "batched absorb launch");
+}
+
+namespace {
+
+int enqueue_batched(Batch* b, int variant, const enl_gas_t* gas, const int* reflect, int order, int nsteps,
+                    hipStream_t s) {
   for (int step = 0; step < nsteps; step++) {
     hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, b->count, kMaxStages * kMaxLevels);
     for (int st = 0; st < order; st++) {
@@ -392,6 +443,7 @@ int enqueue_batched(Batch* b, int variant, const int* reflect, int order, int ns
         if (!refreshed) fill_level(b, lev, in, out, 7, reflect, s);
         int rc = b->all_pow2 ? stage_variant<true>(b, variant, args, st, lev, s)
                              : stage_variant<false>(b, variant, args, st, lev, s);
+        if (!rc) rc = absorb_level(b, lev, out, reflect, gas, s);
         if (rc) return rc;
         if (lev + 1 < b->nlevels) fill_level(b, lev, out, out, 7, reflect, s);
       }
@@ -485,6 +537,13 @@ int enqueue_plain(Batch* b, int variant, const enl_gas_t* gas, const int* reflec
         char msg[8];
         int code = enl_last_error(msg, sizeof msg);
         if (code) return set_error(code, "plain per-patch call failed");
+        if (b->absorb) {
+          // singleStep's last call (enlightning.C:581-582), patch by patch
+          const int sides[4] = {!reflect[0], !reflect[1], !reflect[2], !reflect[3]};
+          rc = enl_absorb(d.box[0], d.box[1], d.box[2], d.box[3], b->g, b->g, d.xlo, d.dx, b->ab_glo, b->ab_ghi,
+                          sides, b->ab_width, gas, w, pres, temp);
+          if (rc) return rc;
+        }
       }
       if (lev + 1 < b->nlevels) fill_level(b, lev, 0, 0, 7, reflect, s);
       }
@@ -762,12 +821,15 @@ int enl_batch_advance(void* handle, int mode, int variant, const enl_gas_t* gas,
   if (order < 1 || order > kMaxStages) return set_error(-2, "RK order out of range");
   hipStream_t s = lib_stream();
   if (mode == ENL_BATCH_PLAIN) return enqueue_plain(b, variant, gas, reflect, order, alpha_1, alpha_2, beta, dt, nsteps, s);
-  if (mode == ENL_BATCH_FP32 || mode == ENL_BATCH_FP32_GRAPH)
+  if (mode == ENL_BATCH_FP32 || mode == ENL_BATCH_FP32_GRAPH) {
+    if (b->absorb && !(reflect[0] && reflect[1] && reflect[2] && reflect[3]))
+      return set_error(-2, "absorbing envelopes are not available in the FP32-perturbation form");
     return fp32_advance(b, mode, gas, reflect, order, alpha_1, alpha_2, beta, dt, nsteps, s);
+  }
   if (mode != ENL_BATCH_FUSED && mode != ENL_BATCH_GRAPH) return set_error(-2, "unknown batch mode");
   int prc = prepare_args(b, gas, order, alpha_1, alpha_2, beta, dt, s);
   if (prc) return prc;
-  if (mode == ENL_BATCH_FUSED) return enqueue_batched(b, variant, reflect, order, nsteps, s);
+  if (mode == ENL_BATCH_FUSED) return enqueue_batched(b, variant, gas, reflect, order, nsteps, s);
   // capture one step into a hipGraph (re-captured when the step changes) and
   // replay it nsteps times
   std::vector<unsigned char> key;
@@ -785,6 +847,8 @@ int enl_batch_advance(void* handle, int mode, int variant, const enl_gas_t* gas,
   put(beta, order * sizeof(double));
   put(b->tile_bx, sizeof b->tile_bx);
   put(b->tile_by, sizeof b->tile_by);
+  put(&b->absorb, sizeof b->absorb);
+  put(&b->ab_rects, sizeof b->ab_rects);
   bool same = b->graph_exec && key == b->graph_key;
   if (!same) {
     if (b->graph_exec) {
@@ -801,7 +865,7 @@ int enl_batch_advance(void* handle, int mode, int variant, const enl_gas_t* gas,
       (void)hipStreamDestroy(cs);
       return check_hip(e, "hipStreamBeginCapture");
     }
-    int rc = enqueue_batched(b, variant, reflect, order, 1, cs);
+    int rc = enqueue_batched(b, variant, gas, reflect, order, 1, cs);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);
@@ -825,6 +889,75 @@ int enl_batch_advance(void* handle, int mode, int variant, const enl_gas_t* gas,
   return 0;
 }
 
+int enl_batch_set_absorb(void* handle, const double* grid_xlo, const double* grid_xhi, double width) {
+  Batch* b = (Batch*)handle;
+  if (!b) return set_error(-2, "null batch");
+  // queued steps may still read the old tables
+  if (b->ab_rects || b->ab_tab) (void)hipStreamSynchronize(lib_stream());
+  if (b->ab_rects) {
+    (void)hipFree(b->ab_rects);
+    b->ab_rects = nullptr;
+  }
+  if (b->ab_tab) {
+    (void)hipFree(b->ab_tab);
+    b->ab_tab = nullptr;
+  }
+  for (int s = 0; s < 4; s++)
+    for (int l = 0; l < 9; l++) b->ab_begin[s][l] = 0;
+  b->absorb = width > 0;
+  if (!b->absorb) return 0;
+  if (!grid_xlo || !grid_xhi) return set_error(-3, "enl_batch_set_absorb: null domain bounds");
+  for (int d = 0; d < 2; d++) {
+    b->ab_glo[d] = grid_xlo[d];
+    b->ab_ghi[d] = grid_xhi[d];
+  }
+  b->ab_width = width;
+  // every patch's rectangles for all four sides (which sides apply is
+  // decided by the reflect flags of each advance), grouped by side and level
+  std::vector<double> tab;
+  std::vector<AbsRect> rects[4][8];
+  const int all[4] = {1, 1, 1, 1};
+  for (int p = 0; p < b->npatch; p++) {
+    const enl_patch_desc_t& d = b->desc[p];
+    const Geom G = make_geom(d.box[0], d.box[1], d.box[2], d.box[3], b->g, b->g);
+    AbsorbPlan plan;
+    int rc = absorb_plan(G, d.xlo, d.dx, grid_xlo, grid_xhi, all, width, &plan);
+    if (rc) return rc;
+    const int base = (int)tab.size();
+    tab.insert(tab.end(), plan.tab.begin(), plan.tab.end());
+    for (int s = 0; s < 4; s++) {
+      const AbsorbSide& P = plan.side[s];
+      if (P.hi < P.lo) continue;
+      AbsRect r;
+      r.patch = p;
+      r.lo = P.lo;
+      r.hi = P.hi;
+      r.t0 = (s < 2 && P.skip_first) ? 1 : 0;
+      r.nt = (s < 2 ? G.ny : G.nx) - r.t0;
+      r.off = base + P.off;
+      if (r.nt > 0) rects[s][d.level].push_back(r);
+    }
+  }
+  std::vector<AbsRect> flat;
+  for (int s = 0; s < 4; s++) {
+    for (int l = 0; l < 8; l++) {
+      b->ab_begin[s][l] = (int)flat.size();
+      flat.insert(flat.end(), rects[s][l].begin(), rects[s][l].end());
+    }
+    b->ab_begin[s][8] = (int)flat.size();
+  }
+  // levels are contiguous inside a side's block, so ab_begin[s][l + 1] ends
+  // level l
+  hipError_t e = hipSuccess;
+  if (!flat.empty()) e = hipMalloc((void**)&b->ab_rects, sizeof(AbsRect) * flat.size());
+  if (e == hipSuccess && !tab.empty()) e = hipMalloc((void**)&b->ab_tab, sizeof(double) * tab.size());
+  if (e == hipSuccess && !flat.empty())
+    e = hipMemcpy(b->ab_rects, flat.data(), sizeof(AbsRect) * flat.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !tab.empty())
+    e = hipMemcpy(b->ab_tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice);
+  return check_hip(e, "enl_batch_set_absorb upload");
+}
+
 int enl_batch_destroy(void* handle) {
   Batch* b = (Batch*)handle;
   if (!b) return 0;
@@ -832,6 +965,8 @@ int enl_batch_destroy(void* handle) {
   if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   fp32_release(b);
+  if (b->ab_rects) (void)hipFree(b->ab_rects);
+  if (b->ab_tab) (void)hipFree(b->ab_tab);
   for (void* a : b->allocations)
     if (a) (void)hipFree(a);
   delete b;

@@ -104,6 +104,13 @@ class DeviceHierarchy:
 
     TILE_SHAPES = ((32, 8), (16, 16), (16, 8), (8, 8))
 
+    def set_absorb(self, width):
+        """Absorbing envelopes (enl_batch_set_absorb) on the sides whose reflect
+        flag is 0; the domain is the level-0 index space at the level-0 dx."""
+        n0, dx0 = self.h.ncells(0), self.h.dx(0)
+        D = ctypes.c_double * 2
+        _lib.check(self.lib.enl_batch_set_absorb(self.handle, D(0.0, 0.0), D(n0 * dx0, n0 * dx0), float(width)))
+
     def set_tiles(self, bx, by, level=None):
         if level is None:
             _lib.check(self.lib.enl_batch_set_tiles(self.handle, bx, by))

@@ -241,6 +241,12 @@ int enl_batch_update_state(void *handle, const enl_gas_t *gas);
  * replayed.  All three give bit-identical results. */
 int enl_batch_advance(void *handle, int mode, int variant, const enl_gas_t *gas, const int *reflect, int order,
                       const double *alpha_1, const double *alpha_2, const double *beta, double dt, int nsteps);
+/* Absorbing envelopes for the batch (enl_absorb on every patch after each
+ * stage, enlightning.C:581-582), on the sides whose reflect flag is 0 in
+ * each enl_batch_advance: grid_xlo / grid_xhi the domain, width the
+ * bc_absorb_width.  width <= 0 turns them off.  Not available in the
+ * FP32-perturbation modes (they return an error when a side absorbs). */
+int enl_batch_set_absorb(void *handle, const double *grid_xlo, const double *grid_xhi, double width);
 int enl_batch_destroy(void *handle);
 
 /* ---- FP32-perturbation form of the batched hierarchy (no reference

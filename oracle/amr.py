@@ -17,6 +17,7 @@ import oracle
 class HostHierarchy:
     def __init__(self, h, gas, reflect=(1, 1, 1, 1)):
         self.h, self.gas, self.reflect = h, gas, tuple(reflect)
+        self.absorb_width = 0.0  # envelopes on the sides whose reflect flag is 0, when > 0
         self.off, self.sk, self.sy = [], [], []
         off = 0
         for p in h.patches:
@@ -113,6 +114,11 @@ class HostHierarchy:
                         oracle.rhs(variant, p.box, p.g, xlo, (dx, dx), w, src, KK, pr, T, self.gas, p.level)
                         oracle.runge_kutta(p.box, p.g, dt, a1[st], a2[st], b[st], w, ww[k], KK)
                         oracle.update_state(p.box, p.g, xlo, (dx, dx), w, pr, T, self.gas)
+                        if self.absorb_width > 0:
+                            n0 = h.ncells(0) * h.dx(0)
+                            sides = tuple(int(not r) for r in self.reflect)
+                            oracle.absorb(p.box, p.g, xlo, (dx, dx), (0.0, 0.0), (n0, n0), sides,
+                                          self.absorb_width, self.gas, w, pr, T)
                     if lev + 1 < h.nlevels:
                         self.fill_level(lev)
             self.coarsen()

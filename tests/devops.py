@@ -101,5 +101,8 @@ else:
         def set_tiles(self, bx, by, level=None):
             pass
 
+        def set_absorb(self, width):
+            self.absorb_width = width
+
         def close(self):
             pass
