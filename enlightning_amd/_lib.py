@@ -31,6 +31,14 @@ class EnlGas(ctypes.Structure):
         ("tag_tolerance", ctypes.c_double * 3)]
 
 
+class EnlSource(ctypes.Structure):
+    """Mirror of enl_source_t."""
+    _fields_ = [("mode", ctypes.c_int), ("type", ctypes.c_int), ("amp", ctypes.c_double),
+                ("spread", ctypes.c_double), ("pos", ctypes.c_double * 2), ("sine_amp", ctypes.c_double),
+                ("sine_freq", ctypes.c_double), ("omega", ctypes.c_double), ("nseg", ctypes.c_int),
+                ("coords", ctypes.c_void_p)]
+
+
 class EnlPatchDesc(ctypes.Structure):
     """Mirror of enl_patch_desc_t."""
     _fields_ = [("level", ctypes.c_int), ("box", ctypes.c_int * 4), ("xlo", ctypes.c_double * 2),
@@ -92,6 +100,8 @@ PROTOTYPES = {
     "enl_batch_update_state": ([_P, ctypes.POINTER(EnlGas)], _I),
     "enl_batch_advance": ([_P, _I, _I, ctypes.POINTER(EnlGas), _P, _I, _P, _P, _P, _D, _I], _I),
     "enl_batch_set_absorb": ([_P, _P, _P, _D], _I),
+    "enl_add_source": ([_I] * 4 + [_P, _P, ctypes.POINTER(EnlSource), _I, _I, _D, _P, ctypes.c_long], _I),
+    "enl_read_lightning": ([ctypes.c_char_p, _P, _I], _I),
     "enl_batch_destroy": ([_P], _I),
     "enl_advance_patch": ([_I] * 6 + [_P, _P, ctypes.POINTER(EnlGas), _P, _I, _P, _P, _P, _D, _I, _I]
                           + [_P] * 6 + [_D], _I),

@@ -1,0 +1,67 @@
+"""Source injection on the device: enl_add_source (enlightning::addSource,
+enlightning.C:2140-2307) and the lightning-channel file reader
+enl_read_lightning (enlightning.C:2333-2389).
+
+Defaults follow the reference input (input.txt:20-38)."""
+import ctypes
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .patch import _darr, _ptr, use_torch_stream
+
+POINT, SINE, LIGHTNING = 10, 20, 30  # enlightning.C:55-57
+PULSE, DRIVEN = 10, 20               # enlightning.C:60-61
+
+
+@dataclass
+class Source:
+    mode: int = PULSE
+    type: int = POINT
+    amp: float = 5.0e5
+    spread: float = 0.01
+    pos: tuple = (0.2, 0.2)
+    sine_amp: float = 1.0
+    sine_freq: float = 1.0
+    omega: float = 1000.0
+    coords: Optional[np.ndarray] = None  # lightning: (nseg, 4) x1 y1 x2 y2
+
+    def to_struct(self):
+        s = _lib.EnlSource()
+        s.mode, s.type = int(self.mode), int(self.type)
+        s.amp, s.spread = float(self.amp), float(self.spread)
+        s.pos[0], s.pos[1] = float(self.pos[0]), float(self.pos[1])
+        s.sine_amp, s.sine_freq, s.omega = float(self.sine_amp), float(self.sine_freq), float(self.omega)
+        if self.coords is not None:
+            c = np.ascontiguousarray(self.coords, dtype=np.float64).reshape(-1, 4)
+            s.nseg = c.shape[0]
+            s.coords = c.ctypes.data
+            s._keep = c  # the array must outlive the call
+        else:
+            s.nseg, s.coords = 0, None
+        return s
+
+
+def add_source(box, patch_xlo, dx, src, iteration, pulse_time0, dt, source, sy=None):
+    """Writes the patch's mass source into the device array `source` (interior
+    cells, row stride sy; default the patch width)."""
+    lib = _lib.load()
+    use_torch_stream()
+    st = src.to_struct()
+    nx = box[1] - box[0] + 1
+    # a view into a larger array is fine: the rows are addressed through sy
+    ptr = ctypes.c_void_p(source.data_ptr()) if hasattr(source, "data_ptr") else _ptr(source)
+    rc = lib.enl_add_source(box[0], box[1], box[2], box[3], _darr(patch_xlo), _darr(dx), ctypes.byref(st),
+                            int(iteration), int(pulse_time0), float(dt), ptr, int(sy if sy else nx))
+    _lib.check(rc)
+
+
+def read_lightning(path, max_segments=100000):
+    """Segments of a lightning-channel file, as an (n, 4) array."""
+    lib = _lib.load()
+    buf = np.zeros((max_segments, 4))
+    n = lib.enl_read_lightning(str(path).encode(), buf.ctypes.data, max_segments)
+    _lib.check(n if n < 0 else 0)
+    return buf[:min(n, max_segments)].copy()

@@ -150,6 +150,33 @@ int enl_absorb(int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw0, int g
                const double *dx, const double *grid_xlo, const double *grid_xhi, const int *absorb, double width,
                const enl_gas_t *gas, double *w_n, double *pres, double *temp);
 
+/* Source parameters of the reference input (input.txt:20-38): src_mode
+ * (10 PULSE, 20 DRIVEN), src_type (10 POINT, 20 SINE, 30 LIGHTNING) as in
+ * enlightning.C:55-61; for LIGHTNING, nseg segments of 4 coordinates
+ * (x1 y1 x2 y2, as enl_read_lightning returns them; host memory). */
+typedef struct enl_source {
+  int mode, type;
+  double amp, spread, pos[2], sine_amp, sine_freq, omega;
+  int nseg;
+  const double *coords;
+} enl_source_t;
+
+/* Mass source of one patch: replaces enlightning::addSource for a patch,
+ * enlightning.C:2140-2307.  source: device array of the interior cells
+ * (the interior context: cell (i,j) at source[(i-ifirst0) + sy*(j-ifirst1)],
+ * sy = nx there).  POINT and SINE overwrite it, LIGHTNING merges each
+ * segment into it (v + t - v/amp*t), DRIVEN POINT multiplies by
+ * sin(omega*(iteration + 1 - pulse_time0)*dt).  Returns 0 or a negative
+ * error code. */
+int enl_add_source(int ifirst0, int ilast0, int ifirst1, int ilast1, const double *patch_xlo, const double *dx,
+                   const enl_source_t *src, int iteration, int pulse_time0, double dt, double *source, long sy);
+
+/* Lightning-channel file: replaces enlightning::readLightningInput,
+ * enlightning.C:2333-2389.  Fills coords (4 per segment, up to max_segments;
+ * NULL to count only) and returns the number of segments, or a negative
+ * error code. */
+int enl_read_lightning(const char *path, double *coords, int max_segments);
+
 /* nsteps method-of-lines RK steps of one patch that covers the whole domain
  * (all four sides physical): per step ww <- w_n(interior), then per stage
  * the physical fill, rhs, runge_kutta and update_state, i.e. the per-patch

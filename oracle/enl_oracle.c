@@ -44,6 +44,7 @@
  * tests/test_oracle_*.py.
  */
 #include <math.h>
+#include <stdio.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -660,6 +661,91 @@ void orc_absorb(int if0, int il0, int if1, int il1, int g0, int g1, const double
                 double *w_n, double *pres, double *temp) {
   absorb_box(if0, il0, if1, il1, if0, il0, if1, il1, g0, g1, pxlo, dx, gxlo, gxhi, absorb, width, gas, w_n, pres,
              temp);
+}
+
+/* Mass source of a patch, enlightning::addSource (enlightning.C:2140-2307),
+ * cell by cell in the reference's loop order: POINT and SINE pulses and the
+ * DRIVEN point overwrite the interior source; a LIGHTNING channel merges
+ * each segment k in turn (end caps on their side of the normal lines
+ * through the end points, the perpendicular Gaussian between them) as
+ * v + t - (v / amp) * t.  source: interior cells, row stride sy. */
+#define ORC_PI 3.14159265358979323846264338
+int orc_add_source(int if0, int il0, int if1, int il1, const double *pxlo, const double *dx,
+                   const orc_source_t *src, int iteration, int pulse_time0, double dt, double *source, long sy) {
+  const double amp = src->amp, sp2 = src->spread * src->spread;
+  if (src->mode == 20 && src->type != 10) return -2;
+  for (int j = if1; j <= il1; j++) {
+    const double xc1 = pxlo[1] + dx[1] * ((double)(j - if1) + 0.5);
+    for (int i = if0; i <= il0; i++) {
+      const double xc0 = pxlo[0] + dx[0] * ((double)(i - if0) + 0.5);
+      double *o = source + (long)(j - if1) * sy + (i - if0);
+      const double x0 = xc0 - src->pos[0], x1 = xc1 - src->pos[1];
+      if (src->type == 10 && src->mode == 10) {
+        *o = amp * exp(-log(2.0) / sp2 * (x0 * x0 + x1 * x1));
+      } else if (src->type == 10) {
+        *o = amp * exp(-log(2.0) / sp2 * (x0 * x0 + x1 * x1)) *
+             sin(src->omega * (iteration + 1 - pulse_time0) * dt);
+      } else if (src->type == 20) {
+        double d = x0 - src->sine_amp * sin(src->sine_freq * x1);
+        *o = amp * exp(-log(2.0) / sp2 * (d * d));
+      }
+    }
+  }
+  if (src->type != 30) return 0;
+  /* the channel: segment by segment over the whole patch */
+  for (int k = 0; k < src->nseg; k++) {
+    const double *c = src->coords + 4 * k;
+    const double theta = -atan((c[3] - c[1]) / (c[2] - c[0]));
+    int orient = 0;
+    if ((c[0] > c[2] && c[1] > c[3]) || (c[0] <= c[2] && c[1] >= c[3])) orient = 1;
+    else if ((c[0] >= c[2] && c[1] <= c[3]) || (c[0] < c[2] && c[1] < c[3])) orient = 2;
+    for (int j = if1; j <= il1; j++) {
+      const double xc1 = pxlo[1] + dx[1] * ((double)(j - if1) + 0.5);
+      for (int i = if0; i <= il0; i++) {
+        const double xc0 = pxlo[0] + dx[0] * ((double)(i - if0) + 0.5);
+        const double l0 = tan(ORC_PI / 2.0 - theta) * (xc0 - c[0]) + c[1];
+        const double l1 = tan(ORC_PI / 2.0 - theta) * (xc0 - c[2]) + c[3];
+        double *o = source + (long)(j - if1) * sy + (i - if0);
+        if ((orient == 1 && xc1 >= l0) || (orient == 2 && xc1 <= l0)) {
+          double t = amp * exp(-log(2.0) / sp2 * ((xc0 - c[0]) * (xc0 - c[0]) + (xc1 - c[1]) * (xc1 - c[1])));
+          *o = *o + t - (*o / amp) * t;
+        }
+        if ((orient == 1 && xc1 <= l1) || (orient == 2 && xc1 >= l1)) {
+          double t = amp * exp(-log(2.0) / sp2 * ((xc0 - c[2]) * (xc0 - c[2]) + (xc1 - c[3]) * (xc1 - c[3])));
+          *o = *o + t - (*o / amp) * t;
+        }
+        if ((orient == 1 && xc1 < l0 && xc1 > l1) || (orient == 2 && xc1 > l0 && xc1 < l1)) {
+          double d = sin(theta) * (xc0 - c[0]) + cos(theta) * (xc1 - c[1]);
+          double t = amp * exp(-log(2.0) / sp2 * (d * d));
+          *o = *o + t - (*o / amp) * t;
+        }
+      }
+    }
+  }
+  return 0;
+}
+
+/* enlightning::readLightningInput (enlightning.C:2333-2389): lines read with
+ * fgets(line, 80) and sscanf("%f %f %f %f"); an unparsed line repeats the
+ * previous values.  Returns the segment count (coords may be NULL). */
+int orc_read_lightning(const char *path, double *coords, int max_segments) {
+  FILE *fr = fopen(path, "rt");
+  if (!fr) return -1;
+  char line[80];
+  float x1 = 0, y1 = 0, x2 = 0, y2 = 0;
+  int n = 0;
+  while (fgets(line, 80, fr) != NULL) {
+    if (sscanf(line, "%f %f %f %f", &x1, &y1, &x2, &y2) < 0) { /* values kept */ }
+    if (coords && n < max_segments) {
+      coords[4 * n] = x1;
+      coords[4 * n + 1] = y1;
+      coords[4 * n + 2] = x2;
+      coords[4 * n + 3] = y2;
+    }
+    n++;
+  }
+  fclose(fr);
+  return n;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -69,6 +69,19 @@ int orc_advance_patch(int variant, int if0, int il0, int if1, int il1, int g, co
                       int nsteps, int level, double *w_n, double *pres, double *temp, double *source,
                       double *ww, double *KK, int tile, int nthreads, double absorb_width);
 
+/* Source parameters (input.txt:20-38); mode 10 PULSE / 20 DRIVEN, type
+ * 10 POINT / 20 SINE / 30 LIGHTNING (enlightning.C:55-61). */
+typedef struct {
+  int mode, type;
+  double amp, spread, pos[2], sine_amp, sine_freq, omega;
+  int nseg;
+  const double *coords;
+} orc_source_t;
+
+int orc_add_source(int if0, int il0, int if1, int il1, const double *pxlo, const double *dx,
+                   const orc_source_t *src, int iteration, int pulse_time0, double dt, double *source, long sy);
+int orc_read_lightning(const char *path, double *coords, int max_segments);
+
 void orc_absorb(int if0, int il0, int if1, int il1, int g0, int g1, const double *pxlo, const double *dx,
                 const double *gxlo, const double *gxhi, const int *absorb, double width, const enl_gas_t *gas,
                 double *w_n, double *pres, double *temp);
