@@ -1,6 +1,7 @@
-// Source injection: enlightning::addSource (enlightning.C:2140-2307) and the
+// Source injection: enlightning::addSource (enlightning.C:2140-2307), the
 // lightning-channel file reader (enlightning::readLightningInput,
-// enlightning.C:2333-2389) for the device.
+// enlightning.C:2333-2389) and the microphones' pressure read
+// (enlightning::recordAudio, enlightning.C:2392-2506) for the device.
 //
 // addSource writes the mass source of every interior cell of a patch:
 //   PULSE POINT      amp * exp(-log 2 / s^2 * (x0^2 + x1^2))
@@ -118,6 +119,28 @@ std::vector<Segment> segments(const double* c, int n) {
   return v;
 }
 
+// Pressure at each microphone inside the patch (enlightning::recordAudio,
+// enlightning.C:2414-2436): cell i = (int)((x - xlo) / dx) of the interior
+// array, the bounds inclusive as in the reference.
+__global__ void __launch_bounds__(64) mic_kernel(const double* pres, long sy, int nx, int ny, double xlo0,
+                                                 double xlo1, double xhi0, double xhi1, double dx0, double dx1,
+                                                 int nmics, const double* mx, const double* my, double* out,
+                                                 int* hit) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nmics) return;
+  const double x = mx[k], y = my[k];
+  if (x >= xlo0 && x <= xhi0 && y >= xlo1 && y <= xhi1) {
+    const int i = (int)((x - xlo0) / dx0), j = (int)((y - xlo1) / dx1);
+    // the reference reads cell (i, j) even on the upper bound (i == nx);
+    // the device clamps it into the patch instead of reading past it
+    const int ic = i < nx ? i : nx - 1, jc = j < ny ? j : ny - 1;
+    out[k] = pres[(long)ic + (long)jc * sy];
+    hit[k] = 1;
+  } else {
+    hit[k] = 0;
+  }
+}
+
 }  // namespace
 
 }  // namespace enl
@@ -180,6 +203,24 @@ int enl_add_source(int ifirst0, int ilast0, int ifirst1, int ilast1, const doubl
     if (e == hipSuccess) e = f;
   }
   return check_hip(e, "source launch");
+}
+
+int enl_sample_mics(int ifirst0, int ilast0, int ifirst1, int ilast1, const double* patch_xlo, const double* patch_xhi,
+                    const double* dx, int nmics, const double* mic_x, const double* mic_y, const double* pres, long sy,
+                    double* out, int* hit) {
+  if (ilast0 < ifirst0 || ilast1 < ifirst1)
+    return set_error(-2, "invalid box [%d,%d]x[%d,%d]", ifirst0, ilast0, ifirst1, ilast1);
+  if (nmics <= 0) return 0;
+  if (!patch_xlo || !patch_xhi || !dx) return set_error(-3, "null argument");
+  const void* dev[5] = {mic_x, mic_y, pres, out, hit};
+  for (int i = 0; i < 5; i++)
+    if (!dev[i] || !is_device_ptr(dev[i])) return set_error(-4, "enl_sample_mics needs device arrays");
+  const int nx = ilast0 - ifirst0 + 1, ny = ilast1 - ifirst1 + 1;
+  if (sy < nx) return set_error(-2, "pressure row stride below the patch width");
+  hipLaunchKernelGGL(mic_kernel, dim3((unsigned)((nmics + 63) / 64)), dim3(64), 0, lib_stream(), pres, sy, nx, ny,
+                     patch_xlo[0], patch_xlo[1], patch_xhi[0], patch_xhi[1], dx[0], dx[1], nmics, mic_x, mic_y, out,
+                     hit);
+  return check_hip(hipGetLastError(), "mic gather launch");
 }
 
 // enlightning::readLightningInput: one segment per line read with

@@ -65,3 +65,24 @@ def read_lightning(path, max_segments=100000):
     n = lib.enl_read_lightning(str(path).encode(), buf.ctypes.data, max_segments)
     _lib.check(n if n < 0 else 0)
     return buf[:min(n, max_segments)].copy()
+
+
+def sample_mics(box, patch_xlo, patch_xhi, dx, mic_x, mic_y, pres, sy=None):
+    """Pressure at the microphones inside the patch (enl_sample_mics,
+    enlightning.C:2414-2436) from the device pressure `pres` (interior cells,
+    row stride sy; a view into a ghosted array is fine): device tensors
+    (values, hit)."""
+    import torch
+    lib = _lib.load()
+    use_torch_stream()
+    mx = torch.as_tensor(np.asarray(mic_x, dtype=np.float64), device=pres.device)
+    my = torch.as_tensor(np.asarray(mic_y, dtype=np.float64), device=pres.device)
+    out = torch.zeros(len(mx), dtype=torch.float64, device=pres.device)
+    hit = torch.zeros(len(mx), dtype=torch.int32, device=pres.device)
+    nx = box[1] - box[0] + 1
+    rc = lib.enl_sample_mics(box[0], box[1], box[2], box[3], _darr(patch_xlo), _darr(patch_xhi), _darr(dx), len(mx),
+                             ctypes.c_void_p(mx.data_ptr()), ctypes.c_void_p(my.data_ptr()),
+                             ctypes.c_void_p(pres.data_ptr()), int(sy if sy else nx), ctypes.c_void_p(out.data_ptr()),
+                             ctypes.c_void_p(hit.data_ptr()))
+    _lib.check(rc)
+    return out, hit

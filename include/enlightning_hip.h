@@ -177,6 +177,18 @@ int enl_add_source(int ifirst0, int ilast0, int ifirst1, int ilast1, const doubl
  * error code. */
 int enl_read_lightning(const char *path, double *coords, int max_segments);
 
+/* Microphones: the pressure read of enlightning::recordAudio,
+ * enlightning.C:2392-2506, for one patch.  For each microphone k inside
+ * [patch_xlo, patch_xhi] (bounds inclusive) out[k] = pres at cell
+ * ((int)((x-xlo0)/dx0), (int)((y-xlo1)/dx1)) of the interior pressure array
+ * (row stride sy) and hit[k] = 1; else hit[k] = 0.  The reference reads one
+ * cell past the patch for a microphone on its upper bound; this reads the
+ * last cell.  mic_x, mic_y, pres, out, hit: device arrays.  The per-step
+ * files stay with the caller. */
+int enl_sample_mics(int ifirst0, int ilast0, int ifirst1, int ilast1, const double *patch_xlo,
+                    const double *patch_xhi, const double *dx, int nmics, const double *mic_x,
+                    const double *mic_y, const double *pres, long sy, double *out, int *hit);
+
 /* nsteps method-of-lines RK steps of one patch that covers the whole domain
  * (all four sides physical): per step ww <- w_n(interior), then per stage
  * the physical fill, rhs, runge_kutta and update_state, i.e. the per-patch

@@ -76,6 +76,8 @@ def load():
     lib.orc_add_source.restype = _I
     lib.orc_read_lightning.argtypes = [ctypes.c_char_p, _P, _I]
     lib.orc_read_lightning.restype = _I
+    lib.orc_sample_mics.argtypes = [_I] * 4 + [_P, _P, _P, _I, _P, _P, _P, ctypes.c_long, _P, _P]
+    lib.orc_sample_mics.restype = None
     _lib = lib
     return lib
 
@@ -200,6 +202,20 @@ def add_source(box, patch_xlo, dx, src, iteration, pulse_time0, dt, source, sy=N
     del keep
     if rc:
         raise ValueError("orc_add_source failed (%d)" % rc)
+
+
+def sample_mics(box, patch_xlo, patch_xhi, dx, mic_x, mic_y, pres, sy=None):
+    """Pressure at the microphones inside the patch (enlightning.C:2414-2436):
+    (values, hit) for interior pressure `pres` (row stride sy)."""
+    mx = np.ascontiguousarray(mic_x, dtype=np.float64)
+    my = np.ascontiguousarray(mic_y, dtype=np.float64)
+    out = np.zeros(len(mx))
+    hit = np.zeros(len(mx), np.int32)
+    nx = box[1] - box[0] + 1
+    load().orc_sample_mics(box[0], box[1], box[2], box[3], _darr(patch_xlo), _darr(patch_xhi), _darr(dx), len(mx),
+                           _ptr(mx), _ptr(my), _ptr(np.ascontiguousarray(pres).ravel()), int(sy if sy else nx),
+                           _ptr(out), _ptr(hit))
+    return out, hit
 
 
 def read_lightning(path, max_segments=100000):

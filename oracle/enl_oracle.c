@@ -725,6 +725,25 @@ int orc_add_source(int if0, int il0, int if1, int il1, const double *pxlo, const
   return 0;
 }
 
+/* Microphones, enlightning::recordAudio (enlightning.C:2414-2436): the
+ * pressure of cell ((int)((x-xlo0)/dx0), (int)((y-xlo1)/dx1)) of the
+ * interior array (row stride sy) for each microphone inside the patch
+ * (bounds inclusive); hit[k] says whether it was. */
+void orc_sample_mics(int if0, int il0, int if1, int il1, const double *pxlo, const double *pxhi, const double *dx,
+                     int nmics, const double *mx, const double *my, const double *pres, long sy, double *out,
+                     int *hit) {
+  (void)if0; (void)il0; (void)if1; (void)il1;
+  for (int k = 0; k < nmics; k++) {
+    hit[k] = 0;
+    if (mx[k] >= pxlo[0] && mx[k] <= pxhi[0] && my[k] >= pxlo[1] && my[k] <= pxhi[1]) {
+      int i = (int)((mx[k] - pxlo[0]) / dx[0]);
+      int j = (int)((my[k] - pxlo[1]) / dx[1]);
+      out[k] = pres[(long)j * sy + i];
+      hit[k] = 1;
+    }
+  }
+}
+
 /* enlightning::readLightningInput (enlightning.C:2333-2389): lines read with
  * fgets(line, 80) and sscanf("%f %f %f %f"); an unparsed line repeats the
  * previous values.  Returns the segment count (coords may be NULL). */

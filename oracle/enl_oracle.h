@@ -81,6 +81,9 @@ typedef struct {
 int orc_add_source(int if0, int il0, int if1, int il1, const double *pxlo, const double *dx,
                    const orc_source_t *src, int iteration, int pulse_time0, double dt, double *source, long sy);
 int orc_read_lightning(const char *path, double *coords, int max_segments);
+void orc_sample_mics(int if0, int il0, int if1, int il1, const double *pxlo, const double *pxhi, const double *dx,
+                     int nmics, const double *mx, const double *my, const double *pres, long sy, double *out,
+                     int *hit);
 
 void orc_absorb(int if0, int il0, int if1, int il1, int g0, int g1, const double *pxlo, const double *dx,
                 const double *gxlo, const double *gxhi, const int *absorb, double width, const enl_gas_t *gas,

@@ -73,3 +73,23 @@ def test_read_lightning_matches_oracle(tmp_path):
     a = S.read_lightning(f)
     b = oracle.read_lightning(f)
     assert a.shape == (13, 4) and np.array_equal(a, b)
+
+
+def test_mics_match_oracle():
+    """Microphones inside, outside and on the lower bounds of a patch in a
+    ghosted pressure array."""
+    box = (16, 79, 8, 55)
+    dx = (1 / 128, 1 / 128)
+    xlo = (16 / 128, 8 / 128)
+    xhi = (80 / 128, 56 / 128)
+    nx, ny, g = 64, 48, 4
+    rng = np.random.default_rng(8)
+    p = 101325.0 + rng.standard_normal((ny + 2 * g, nx + 2 * g))
+    mx = np.concatenate([rng.uniform(0.0, 0.7, 40), [xlo[0], 0.3]])
+    my = np.concatenate([rng.uniform(0.0, 0.5, 40), [0.2, xlo[1]]])
+    ref, rhit = oracle.sample_mics(box, xlo, xhi, dx, mx, my, p[g:-g, g:-g].copy())
+    dp = D.dev(p)
+    got, ghit = S.sample_mics(box, xlo, xhi, dx, mx, my, dp[g:, g:], sy=nx + 2 * g)
+    got, ghit = D.host(got), D.host(ghit)
+    assert np.array_equal(ghit, rhit) and rhit.sum() > 10 and rhit.sum() < len(mx)
+    assert np.array_equal(got[ghit == 1], ref[rhit == 1])

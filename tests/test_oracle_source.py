@@ -96,3 +96,16 @@ def test_lightning_file(tmp_path):
     g = tmp_path / "lightning.txt"
     g.write_text("2.0 10.0 2.0 0.0\n\n")
     assert np.array_equal(oracle.read_lightning(g), [[2, 10, 2, 0], [2, 10, 2, 0]])
+
+
+def test_mics():
+    """A microphone reads the cell that contains it: at a cell centre, on the
+    patch's lower corner, and not at all outside the patch."""
+    p = np.arange(48 * 64, dtype=np.float64).reshape(48, 64)
+    box, dx = (16, 79, 8, 55), (1 / 128, 1 / 128)
+    xlo, xhi = (16 / 128, 8 / 128), (80 / 128, 56 / 128)
+    mx = [xlo[0] + 10.5 / 128, xlo[0], 0.05, 0.3]
+    my = [xlo[1] + 3.5 / 128, xlo[1], 0.2, 0.9]
+    v, hit = oracle.sample_mics(box, xlo, xhi, dx, mx, my, p)
+    assert list(hit) == [1, 1, 0, 0]
+    assert v[0] == p[3, 10] and v[1] == p[0, 0]
