@@ -115,50 +115,65 @@ __device__ inline const double* coarse_base(const FillArgs& f, const PatchDev& p
 }
 
 // same-level copy: every component of the cell
-__global__ void __launch_bounds__(256) fill_copy_kernel(FillArgs f, int n, const int* dp, const int* dl,
-                                                         const int* sp, const int* sl) {
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-    const PatchDev& D = f.patches[dp[q]];
-    const PatchDev& S = f.patches[sp[q]];
-    double* d = fill_base(f, D) + dl[q];
-    const double* s = fill_base(f, S) + sl[q];
-    for (int k = 0; k < f.ncomp; k++) d[k * D.sk] = s[k * S.sk];
-  }
+__device__ inline void fill_copy(const FillArgs& f, int q, const int* dp, const int* dl, const int* sp, const int* sl) {
+  const PatchDev& D = f.patches[dp[q]];
+  const PatchDev& S = f.patches[sp[q]];
+  double* d = fill_base(f, D) + dl[q];
+  const double* s = fill_base(f, S) + sl[q];
+  for (int k = 0; k < f.ncomp; k++) d[k * D.sk] = s[k * S.sk];
 }
 
 // linear refinement from the coarse cell holding the fine ghost cell:
 // c + fx*(c[i+1]-c[i-1])/2 + fy*(c[j+1]-c[j-1])/2, (fx, fy) the fine cell
 // centre's offset from the coarse centre in coarse cell widths
-__global__ void __launch_bounds__(256) fill_interp_kernel(FillArgs f, int n, const int* dp, const int* dl,
-                                                           const int* sp, const int* sl, const double* fx,
-                                                           const double* fy) {
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-    const PatchDev& D = f.patches[dp[q]];
-    const PatchDev& C = f.patches[sp[q]];
-    double* d = fill_base(f, D) + dl[q];
-    const double* c = coarse_base(f, C) + sl[q];
-    const double ax = fx[q], ay = fy[q];
-    for (int k = 0; k < f.ncomp; k++) {
-      const double* ck = c + k * C.sk;
-      d[k * D.sk] = ck[0] + ax * (0.5 * (ck[1] - ck[-1])) + ay * (0.5 * (ck[C.sy] - ck[-C.sy]));
-    }
+__device__ inline void fill_interp(const FillArgs& f, int q, const int* dp, const int* dl, const int* sp,
+                                   const int* sl, const double* fx, const double* fy) {
+  const PatchDev& D = f.patches[dp[q]];
+  const PatchDev& C = f.patches[sp[q]];
+  double* d = fill_base(f, D) + dl[q];
+  const double* c = coarse_base(f, C) + sl[q];
+  const double ax = fx[q], ay = fy[q];
+  for (int k = 0; k < f.ncomp; k++) {
+    const double* ck = c + k * C.sk;
+    d[k * D.sk] = ck[0] + ax * (0.5 * (ck[1] - ck[-1])) + ay * (0.5 * (ck[C.sy] - ck[-C.sy]));
   }
 }
 
 // physical boundary: copy within the patch; side 1..4 = left, right, bottom,
 // top selects the REFLECT flag and the momentum component it negates
-__global__ void __launch_bounds__(256) fill_bc_kernel(FillArgs f, int n, const int* pp, const int* dl,
-                                                       const int* sl, const int* side) {
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-    const PatchDev& P = f.patches[pp[q]];
-    double* base = fill_base(f, P);
-    const int sd = side[q];
-    const int negk = (f.ncomp > 2 && f.reflect[sd - 1]) ? (sd <= 2 ? 1 : 2) : -1;
-    for (int k = 0; k < f.ncomp; k++) {
-      double v = base[sl[q] + k * P.sk];
-      base[dl[q] + k * P.sk] = k == negk ? -v : v;
-    }
+__device__ inline void fill_bc(const FillArgs& f, int q, const int* pp, const int* dl, const int* sl,
+                               const int* side) {
+  const PatchDev& P = f.patches[pp[q]];
+  double* base = fill_base(f, P);
+  const int sd = side[q];
+  const int negk = (f.ncomp > 2 && f.reflect[sd - 1]) ? (sd <= 2 ? 1 : 2) : -1;
+  for (int k = 0; k < f.ncomp; k++) {
+    double v = base[sl[q] + k * P.sk];
+    base[dl[q] + k * P.sk] = k == negk ? -v : v;
   }
+}
+
+// Same-level copies, coarse-to-fine refinement and the physical edge strips
+// of one level in one launch: their destinations are distinct ghost cells
+// and every source is an interior cell (of a neighbour, of the coarse level,
+// of the patch itself), so the three lists do not depend on each other.
+__global__ void __launch_bounds__(256) fill_edges_kernel(FillArgs f, LevelFill L) {
+  const int n = L.ncopy + L.ninterp + L.nedge;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    if (q < L.ncopy)
+      fill_copy(f, q, L.c_dp, L.c_dl, L.c_sp, L.c_sl);
+    else if (q < L.ncopy + L.ninterp)
+      fill_interp(f, q - L.ncopy, L.i_dp, L.i_dl, L.i_sp, L.i_sl, L.i_fx, L.i_fy);
+    else
+      fill_bc(f, q - L.ncopy - L.ninterp, L.e_p, L.e_d, L.e_s, L.e_n);
+  }
+}
+
+// corner blocks: they copy ghost cells the edge pass wrote, so they follow it
+__global__ void __launch_bounds__(256) fill_corner_kernel(FillArgs f, int n, const int* pp, const int* dl,
+                                                           const int* sl, const int* side) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+    fill_bc(f, q, pp, dl, sl, side);
 }
 
 // fine-to-coarse average of the r x r fine cells over a coarse cell
@@ -205,17 +220,10 @@ void fill_level(Batch* b, int level, int buf, int cbuf, int ncomp, const int* re
   f.ncomp = ncomp;
   for (int k = 0; k < 4; k++) f.reflect[k] = reflect[k];
   const LevelFill& L = b->fill[level];
-  if (L.ncopy)
-    hipLaunchKernelGGL(fill_copy_kernel, dim3(grid_for(L.ncopy)), dim3(256), 0, s, f, L.ncopy, L.c_dp, L.c_dl,
-                       L.c_sp, L.c_sl);
-  if (L.ninterp)
-    hipLaunchKernelGGL(fill_interp_kernel, dim3(grid_for(L.ninterp)), dim3(256), 0, s, f, L.ninterp, L.i_dp,
-                       L.i_dl, L.i_sp, L.i_sl, L.i_fx, L.i_fy);
-  if (L.nedge)
-    hipLaunchKernelGGL(fill_bc_kernel, dim3(grid_for(L.nedge)), dim3(256), 0, s, f, L.nedge, L.e_p, L.e_d, L.e_s,
-                       L.e_n);
+  const long n = (long)L.ncopy + L.ninterp + L.nedge;
+  if (n) hipLaunchKernelGGL(fill_edges_kernel, dim3(grid_for(n)), dim3(256), 0, s, f, L);
   if (L.ncorner)
-    hipLaunchKernelGGL(fill_bc_kernel, dim3(grid_for(L.ncorner)), dim3(256), 0, s, f, L.ncorner, L.k_p, L.k_d,
+    hipLaunchKernelGGL(fill_corner_kernel, dim3(grid_for(L.ncorner)), dim3(256), 0, s, f, L.ncorner, L.k_p, L.k_d,
                        L.k_s, L.k_n);
 }
 

@@ -455,55 +455,68 @@ __device__ inline const double* row_rec(const Fill32& f, const Patch32& P, long 
   return f.amb + (P.amb + local / P.sy) * kRec;
 }
 
-__global__ void __launch_bounds__(256) fill32_copy_kernel(Fill32 f, int n, const int* dp, const int* dl,
-                                                           const int* sp, const int* sl) {
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-    const Patch32& D = f.patches[dp[q]];
-    const Patch32& Sd = f.patches[sp[q]];
-    float* d = f.pool + D.blk[f.buf] + dl[q];
-    const float* s = f.pool + Sd.blk[f.buf] + sl[q];
-    // the same global cell: the same row, the same ambient
-    for (int k = 0; k < 7; k++) d[k * D.sk] = s[k * Sd.sk];
+__device__ inline void fill32_copy(const Fill32& f, int q, const int* dp, const int* dl, const int* sp,
+                                   const int* sl) {
+  const Patch32& D = f.patches[dp[q]];
+  const Patch32& Sd = f.patches[sp[q]];
+  float* d = f.pool + D.blk[f.buf] + dl[q];
+  const float* s = f.pool + Sd.blk[f.buf] + sl[q];
+  // the same global cell: the same row, the same ambient
+  for (int k = 0; k < 7; k++) d[k * D.sk] = s[k * Sd.sk];
+}
+
+__device__ inline void fill32_interp(const Fill32& f, int q, const int* dp, const int* dl, const int* sp,
+                                     const int* sl, const double* fx, const double* fy) {
+  const Patch32& D = f.patches[dp[q]];
+  const Patch32& C = f.patches[sp[q]];
+  float* d = f.pool + D.blk[f.buf] + dl[q];
+  const float* c = f.pool + C.blk[f.cbuf] + sl[q];
+  const double* rf = row_rec(f, D, dl[q]);
+  const double* rc = row_rec(f, C, sl[q]);
+  const double ax = fx[q], ay = fy[q];
+  for (int k = 0; k < 7; k++) {
+    const float* ck = c + k * C.sk;
+    const double A0 = amb_of(rc, k), Am = amb_of(rc - kRec, k), Ap = amb_of(rc + kRec, k);
+    const double c0 = A0 + (double)ck[0];
+    const double v = c0 + ax * (0.5 * ((A0 + (double)ck[1]) - (A0 + (double)ck[-1]))) +
+                     ay * (0.5 * ((Ap + (double)ck[C.sy]) - (Am + (double)ck[-C.sy])));
+    d[k * D.sk] = (float)(v - amb_of(rf, k));
   }
 }
 
-__global__ void __launch_bounds__(256) fill32_interp_kernel(Fill32 f, int n, const int* dp, const int* dl,
-                                                             const int* sp, const int* sl, const double* fx,
-                                                             const double* fy) {
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-    const Patch32& D = f.patches[dp[q]];
-    const Patch32& C = f.patches[sp[q]];
-    float* d = f.pool + D.blk[f.buf] + dl[q];
-    const float* c = f.pool + C.blk[f.cbuf] + sl[q];
-    const double* rf = row_rec(f, D, dl[q]);
-    const double* rc = row_rec(f, C, sl[q]);
-    const double ax = fx[q], ay = fy[q];
-    for (int k = 0; k < 7; k++) {
-      const float* ck = c + k * C.sk;
-      const double A0 = amb_of(rc, k), Am = amb_of(rc - kRec, k), Ap = amb_of(rc + kRec, k);
-      const double c0 = A0 + (double)ck[0];
-      const double v = c0 + ax * (0.5 * ((A0 + (double)ck[1]) - (A0 + (double)ck[-1]))) +
-                       ay * (0.5 * ((Ap + (double)ck[C.sy]) - (Am + (double)ck[-C.sy])));
-      d[k * D.sk] = (float)(v - amb_of(rf, k));
-    }
+__device__ inline void fill32_bc(const Fill32& f, int q, const int* pp, const int* dl, const int* sl,
+                                 const int* side) {
+  const Patch32& P = f.patches[pp[q]];
+  float* base = f.pool + P.blk[f.buf];
+  const int sd = side[q];
+  const int negk = f.reflect[sd - 1] ? (sd <= 2 ? 1 : 2) : -1;
+  const double* rd = row_rec(f, P, dl[q]);
+  const double* rs = row_rec(f, P, sl[q]);
+  for (int k = 0; k < 7; k++) {
+    double v = amb_of(rs, k) + (double)base[sl[q] + k * P.sk];
+    v = k == negk ? -v : v;
+    base[dl[q] + k * P.sk] = (float)(v - amb_of(rd, k));
   }
 }
 
-__global__ void __launch_bounds__(256) fill32_bc_kernel(Fill32 f, int n, const int* pp, const int* dl,
-                                                         const int* sl, const int* side) {
+// copies, refinement and edge strips of a level in one launch (disjoint
+// destinations, interior sources), corners after them
+__global__ void __launch_bounds__(256) fill32_edges_kernel(Fill32 f, LevelFill L) {
+  const int n = L.ncopy + L.ninterp + L.nedge;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-    const Patch32& P = f.patches[pp[q]];
-    float* base = f.pool + P.blk[f.buf];
-    const int sd = side[q];
-    const int negk = f.reflect[sd - 1] ? (sd <= 2 ? 1 : 2) : -1;
-    const double* rd = row_rec(f, P, dl[q]);
-    const double* rs = row_rec(f, P, sl[q]);
-    for (int k = 0; k < 7; k++) {
-      double v = amb_of(rs, k) + (double)base[sl[q] + k * P.sk];
-      v = k == negk ? -v : v;
-      base[dl[q] + k * P.sk] = (float)(v - amb_of(rd, k));
-    }
+    if (q < L.ncopy)
+      fill32_copy(f, q, L.c_dp, L.c_dl, L.c_sp, L.c_sl);
+    else if (q < L.ncopy + L.ninterp)
+      fill32_interp(f, q - L.ncopy, L.i_dp, L.i_dl, L.i_sp, L.i_sl, L.i_fx, L.i_fy);
+    else
+      fill32_bc(f, q - L.ncopy - L.ninterp, L.e_p, L.e_d, L.e_s, L.e_n);
   }
+}
+
+__global__ void __launch_bounds__(256) fill32_corner_kernel(Fill32 f, int n, const int* pp, const int* dl,
+                                                             const int* sl, const int* side) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+    fill32_bc(f, q, pp, dl, sl, side);
 }
 
 // Fine-to-coarse average of block 0 and the carried state, on full values.
@@ -631,17 +644,10 @@ void fill32_level(Batch* b, int level, int buf, int cbuf, const int* reflect, hi
   f.cbuf = cbuf;
   for (int k = 0; k < 4; k++) f.reflect[k] = reflect[k];
   const LevelFill& L = b->fill[level];
-  if (L.ncopy)
-    hipLaunchKernelGGL(fill32_copy_kernel, dim3(grid_for(L.ncopy)), dim3(256), 0, s, f, L.ncopy, L.c_dp, L.c_dl,
-                       L.c_sp, L.c_sl);
-  if (L.ninterp)
-    hipLaunchKernelGGL(fill32_interp_kernel, dim3(grid_for(L.ninterp)), dim3(256), 0, s, f, L.ninterp, L.i_dp,
-                       L.i_dl, L.i_sp, L.i_sl, L.i_fx, L.i_fy);
-  if (L.nedge)
-    hipLaunchKernelGGL(fill32_bc_kernel, dim3(grid_for(L.nedge)), dim3(256), 0, s, f, L.nedge, L.e_p, L.e_d,
-                       L.e_s, L.e_n);
+  const long n = (long)L.ncopy + L.ninterp + L.nedge;
+  if (n) hipLaunchKernelGGL(fill32_edges_kernel, dim3(grid_for(n)), dim3(256), 0, s, f, L);
   if (L.ncorner)
-    hipLaunchKernelGGL(fill32_bc_kernel, dim3(grid_for(L.ncorner)), dim3(256), 0, s, f, L.ncorner, L.k_p, L.k_d,
+    hipLaunchKernelGGL(fill32_corner_kernel, dim3(grid_for(L.ncorner)), dim3(256), 0, s, f, L.ncorner, L.k_p, L.k_d,
                        L.k_s, L.k_n);
 }
 
