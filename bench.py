@@ -151,6 +151,24 @@ def amr_bench(args):
     graph_states = d.states()
     same = all(np.array_equal(a[k][..., p.g:-p.g, p.g:-p.g], b[k][..., p.g:-p.g, p.g:-p.g])
                for a, b, p in zip(plain_states, graph_states, h.patches) for k in range(3))
+    # one regrid of the evolved state (input.txt:7 regrids every 2 steps):
+    # tags on the device, clustering and schedules on the host, the new
+    # batch and the transfer on the device
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rtags = d.level0_tags()
+    t1 = time.perf_counter()
+    h2 = amr.build(n, 4, 4, 1.0, rtags, tile0=64)
+    tr = amr.make_transfer(h2, h)
+    t2 = time.perf_counter()
+    d2 = d.regrid(h2, tr)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    d2.close()
+    regrid = {"tags_device_ms": (t1 - t0) * 1e3, "cluster_and_schedules_host_ms": (t2 - t1) * 1e3,
+              "new_batch_and_transfer_device_ms": (t3 - t2) * 1e3, "total_ms": (t3 - t0) * 1e3,
+              "fine_patches_after": len(h2.level_patches(1)), "copied_cells": int(len(tr["copy"][0])),
+              "refined_cells": int(len(tr["interp"][0]))}
     d.close()
     sizes = [max(p.nx, p.ny) for p in h.level_patches(1)]
     return {
@@ -169,6 +187,7 @@ def amr_bench(args):
                                    "steps": args.amr_plain_steps},
         "speedup_graph_vs_plain": (t_plain / args.amr_plain_steps) / (t_graph / args.amr_steps),
         "graph_bit_identical_to_plain": same,
+        "regrid": regrid,
         "unit": "cell-updates/s",
     }, tags
 

@@ -23,7 +23,12 @@ per-patch kernels (not vendored, so none of it is parity-pinned):
        or last interior column, negating x momentum at a REFLECT left/right
        wall (enlightning.C:948-1008), after passes 1 and 2;
 * the fine-to-coarse average of w_n, p, T after each step
-  (CONSERVATIVE_COARSEN, enlightning.C:282-288).
+  (CONSERVATIVE_COARSEN, enlightning.C:282-288);
+* at a regrid (gridding_algorithm->regridAllFinerLevels, main.C:489-502) the
+  transfer of the SOLN variables (w_n, p, T, source) into the new
+  hierarchy's patch interiors: from the old patch of the same level that
+  held the cell, else by the same linear refinement from the old coarser
+  level (make_transfer).
 
 All cell addresses are (patch id, local index) pairs with local index
 (i - ifirst0 + g) + Nx * (j - ifirst1 + g) inside the patch's ghosted box.
@@ -301,3 +306,62 @@ def make_coarsen(h):
             fp.append(np.full(sel.sum(), f.pid, np.int32))
             fl.append(f.local(IC[sel] * r, JC[sel] * r).astype(np.int32))
     return tuple(np.concatenate(x).astype(np.int32) for x in (cp, cl, fp, fl))
+
+
+def make_transfer(new, old):
+    """Regrid transfer schedule from hierarchy `old` to `new` (same base
+    grid, ratio and ghost width): for every interior cell of every new
+    patch, either a copy item (new patch, new local) <- (old patch of the
+    same level holding the cell, its local), or, where no old patch of that
+    level held it, an interp item from the old coarser level's cell with the
+    fine centre's offset (fx, fy) in coarse widths.  Ghosts are not listed:
+    the new hierarchy's own ghost fill takes them afterwards."""
+    if (new.n0, new.ratio, new.g) != (old.n0, old.ratio, old.g):
+        raise ValueError("make_transfer: the hierarchies differ in base grid, ratio or ghost width")
+    r = new.ratio
+    c_dp, c_dl, c_sp, c_sl = [], [], [], []
+    i_dp, i_dl, i_sp, i_sl, i_fx, i_fy = [], [], [], [], [], []
+    byid = {p.pid: p for p in old.patches}
+    owners = {}
+    for lev in range(new.nlevels):
+        if lev < old.nlevels:
+            owners[lev] = _owner_map(old.level_patches(lev), old.ncells(lev))
+    for p in new.patches:
+        lev = p.level
+        I, J = np.meshgrid(np.arange(p.box[0], p.box[1] + 1), np.arange(p.box[2], p.box[3] + 1))
+        I, J = I.ravel(), J.ravel()
+        owner = owners[lev][J, I] if lev in owners else np.full(I.size, -1, np.int32)
+        have = owner >= 0
+        for q in np.unique(owner[have]):
+            sel = have & (owner == q)
+            Q = byid[int(q)]
+            c_dp.append(np.full(sel.sum(), p.pid, np.int32))
+            c_dl.append(p.local(I[sel], J[sel]).astype(np.int32))
+            c_sp.append(np.full(sel.sum(), Q.pid, np.int32))
+            c_sl.append(Q.local(I[sel], J[sel]).astype(np.int32))
+        if (~have).any():
+            if lev == 0 or lev - 1 not in owners:
+                raise ValueError("make_transfer: new level-%d cells have no old source" % lev)
+            ui, uj = I[~have], J[~have]
+            ic, jc = ui // r, uj // r
+            cown = owners[lev - 1][jc, ic]
+            if (cown < 0).any():
+                raise ValueError("make_transfer: a new fine cell lies outside the old coarser level")
+            for q in np.unique(cown):
+                sel = cown == q
+                C = byid[int(q)]
+                i_dp.append(np.full(sel.sum(), p.pid, np.int32))
+                i_dl.append(p.local(ui[sel], uj[sel]).astype(np.int32))
+                i_sp.append(np.full(sel.sum(), C.pid, np.int32))
+                i_sl.append(C.local(ic[sel], jc[sel]).astype(np.int32))
+                i_fx.append(((ui[sel] - ic[sel] * r) + 0.5) / r - 0.5)
+                i_fy.append(((uj[sel] - jc[sel] * r) + 0.5) / r - 0.5)
+
+    def cat(lst, dtype):
+        return np.concatenate(lst).astype(dtype) if lst else np.zeros(0, dtype)
+
+    return {
+        "copy": (cat(c_dp, np.int32), cat(c_dl, np.int32), cat(c_sp, np.int32), cat(c_sl, np.int32)),
+        "interp": (cat(i_dp, np.int32), cat(i_dl, np.int32), cat(i_sp, np.int32), cat(i_sl, np.int32),
+                   cat(i_fx, np.float64), cat(i_fy, np.float64)),
+    }

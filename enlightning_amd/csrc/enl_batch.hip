@@ -114,6 +114,12 @@ __device__ inline const double* coarse_base(const FillArgs& f, const PatchDev& p
   return f.pool + (f.cbuf >= 0 ? p.state[f.cbuf] : p.src);
 }
 
+// linear refinement of the coarse cell at c (row stride sy) to the fine
+// centre (ax, ay) coarse widths from its centre, unlimited centred slopes
+__device__ inline double lin_refine(const double* c, long sy, double ax, double ay) {
+  return c[0] + ax * (0.5 * (c[1] - c[-1])) + ay * (0.5 * (c[sy] - c[-sy]));
+}
+
 // same-level copy: every component of the cell
 __device__ inline void fill_copy(const FillArgs& f, int q, const int* dp, const int* dl, const int* sp, const int* sl) {
   const PatchDev& D = f.patches[dp[q]];
@@ -133,10 +139,7 @@ __device__ inline void fill_interp(const FillArgs& f, int q, const int* dp, cons
   double* d = fill_base(f, D) + dl[q];
   const double* c = coarse_base(f, C) + sl[q];
   const double ax = fx[q], ay = fy[q];
-  for (int k = 0; k < f.ncomp; k++) {
-    const double* ck = c + k * C.sk;
-    d[k * D.sk] = ck[0] + ax * (0.5 * (ck[1] - ck[-1])) + ay * (0.5 * (ck[C.sy] - ck[-C.sy]));
-  }
+  for (int k = 0; k < f.ncomp; k++) d[k * D.sk] = lin_refine(c + k * C.sk, C.sy, ax, ay);
 }
 
 // physical boundary: copy within the patch; side 1..4 = left, right, bottom,
@@ -174,6 +177,45 @@ __global__ void __launch_bounds__(256) fill_corner_kernel(FillArgs f, int n, con
                                                            const int* sl, const int* side) {
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
     fill_bc(f, q, pp, dl, sl, side);
+}
+
+// Regrid transfer: the SOLN variables (w_n, pres, temp in state block 0,
+// and the source) of each listed cell of a new hierarchy, copied from the
+// old hierarchy's patch of the same level that held the cell, or refined
+// from the old coarser level where none did.
+struct TransferArgs {
+  double* dpool;
+  const PatchDev* dpat;
+  const double* spool;
+  const PatchDev* spat;
+  int ncopy, ninterp;
+  const int *c_dp, *c_dl, *c_sp, *c_sl;
+  const int *i_dp, *i_dl, *i_sp, *i_sl;
+  const double *i_fx, *i_fy;
+};
+
+__global__ void __launch_bounds__(256) transfer_kernel(TransferArgs t) {
+  const int n = t.ncopy + t.ninterp;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const bool copy = q < t.ncopy;
+    const int r = copy ? q : q - t.ncopy;
+    const PatchDev& D = t.dpat[copy ? t.c_dp[r] : t.i_dp[r]];
+    const PatchDev& S = t.spat[copy ? t.c_sp[r] : t.i_sp[r]];
+    const long dl = copy ? t.c_dl[r] : t.i_dl[r];
+    const long sl = copy ? t.c_sl[r] : t.i_sl[r];
+    double* d = t.dpool + D.state[0] + dl;
+    const double* c = t.spool + S.state[0] + sl;
+    double* dsrc = t.dpool + D.src + dl;
+    const double* csrc = t.spool + S.src + sl;
+    if (copy) {
+      for (int k = 0; k < 7; k++) d[k * D.sk] = c[k * S.sk];
+      *dsrc = *csrc;
+    } else {
+      const double ax = t.i_fx[r], ay = t.i_fy[r];
+      for (int k = 0; k < 7; k++) d[k * D.sk] = lin_refine(c + k * S.sk, S.sy, ax, ay);
+      *dsrc = lin_refine(csrc, S.sy, ax, ay);
+    }
+  }
 }
 
 // fine-to-coarse average of the r x r fine cells over a coarse cell
@@ -964,6 +1006,95 @@ int enl_batch_set_absorb(void* handle, const double* grid_xlo, const double* gri
   if (e == hipSuccess && !tab.empty())
     e = hipMemcpy(b->ab_tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice);
   return check_hip(e, "enl_batch_set_absorb upload");
+}
+
+int enl_batch_transfer(void* new_handle, void* old_handle, const int* reflect, int ncopy, const int* c_dp,
+                       const int* c_dl, const int* c_sp, const int* c_sl, int ninterp, const int* i_dp,
+                       const int* i_dl, const int* i_sp, const int* i_sl, const double* i_fx, const double* i_fy) {
+  Batch* nb = (Batch*)new_handle;
+  Batch* ob = (Batch*)old_handle;
+  if (!nb || !ob || nb == ob) return set_error(-2, "enl_batch_transfer: needs two distinct batches");
+  if (!reflect) return set_error(-3, "enl_batch_transfer: null reflect");
+  if (nb->g != ob->g || nb->ratio != ob->ratio) return set_error(-2, "enl_batch_transfer: ghost width or ratio differ");
+  if (ncopy < 0 || ninterp < 0 || (ncopy && !(c_dp && c_dl && c_sp && c_sl)) ||
+      (ninterp && !(i_dp && i_dl && i_sp && i_sl && i_fx && i_fy)))
+    return set_error(-3, "enl_batch_transfer: bad item lists");
+  // every item must address cells of its patches' ghosted boxes, with the
+  // refinement stencil (+-1 cell, +-1 row) inside the coarse patch's box
+  for (int q = 0; q < ncopy; q++) {
+    if (c_dp[q] < 0 || c_dp[q] >= nb->npatch || c_sp[q] < 0 || c_sp[q] >= ob->npatch)
+      return set_error(-2, "enl_batch_transfer: copy item %d names a bad patch", q);
+    const PatchDev &D = nb->hp[c_dp[q]], &S = ob->hp[c_sp[q]];
+    if (D.level != S.level || c_dl[q] < 0 || c_dl[q] >= D.sk || c_sl[q] < 0 || c_sl[q] >= S.sk)
+      return set_error(-2, "enl_batch_transfer: copy item %d out of range", q);
+  }
+  for (int q = 0; q < ninterp; q++) {
+    if (i_dp[q] < 0 || i_dp[q] >= nb->npatch || i_sp[q] < 0 || i_sp[q] >= ob->npatch)
+      return set_error(-2, "enl_batch_transfer: refine item %d names a bad patch", q);
+    const PatchDev &D = nb->hp[i_dp[q]], &S = ob->hp[i_sp[q]];
+    const long sl = i_sl[q];
+    if (S.level != D.level - 1 || i_dl[q] < 0 || i_dl[q] >= D.sk || sl - S.sy - 1 < 0 || sl + S.sy + 1 >= S.sk)
+      return set_error(-2, "enl_batch_transfer: refine item %d out of range", q);
+  }
+  hipStream_t s = lib_stream();
+  // the old hierarchy's ghosts (source, then state block 0) feed the
+  // refinement stencils at its patch edges
+  for (int lev = 0; lev < ob->nlevels; lev++) fill_level(ob, lev, -1, -1, 1, reflect, s);
+  fill_all(ob, 0, reflect, s);
+  const long n = (long)ncopy + ninterp;
+  if (n > 0) {
+    const size_t ibytes = sizeof(int) * (size_t)(4 * ncopy + 4 * ninterp);
+    const size_t bytes = ibytes + sizeof(double) * (size_t)(2 * ninterp);
+    char* buf = nullptr;
+    hipError_t e = hipMalloc((void**)&buf, bytes);
+    if (e != hipSuccess) return check_hip(e, "hipMalloc(transfer items)");
+    std::vector<char> h(bytes);
+    int* hi = (int*)h.data();
+    const int* lists[8] = {c_dp, c_dl, c_sp, c_sl, i_dp, i_dl, i_sp, i_sl};
+    for (int a = 0; a < 8; a++) {
+      const int cnt = a < 4 ? ncopy : ninterp;
+      if (cnt) memcpy(hi, lists[a], sizeof(int) * (size_t)cnt);
+      hi += cnt;
+    }
+    double* hd = (double*)(h.data() + ibytes);
+    if (ninterp) {
+      memcpy(hd, i_fx, sizeof(double) * (size_t)ninterp);
+      memcpy(hd + ninterp, i_fy, sizeof(double) * (size_t)ninterp);
+    }
+    e = hipMemcpyAsync(buf, h.data(), bytes, hipMemcpyHostToDevice, s);
+    TransferArgs t;
+    t.dpool = nb->pool;
+    t.dpat = nb->dp;
+    t.spool = ob->pool;
+    t.spat = ob->dp;
+    t.ncopy = ncopy;
+    t.ninterp = ninterp;
+    const int* di = (const int*)buf;
+    t.c_dp = di;
+    t.c_dl = di + ncopy;
+    t.c_sp = di + 2 * ncopy;
+    t.c_sl = di + 3 * ncopy;
+    t.i_dp = di + 4 * ncopy;
+    t.i_dl = t.i_dp + ninterp;
+    t.i_sp = t.i_dp + 2 * ninterp;
+    t.i_sl = t.i_dp + 3 * ninterp;
+    t.i_fx = (const double*)(buf + ibytes);
+    t.i_fy = t.i_fx + ninterp;
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(transfer_kernel, dim3(grid_for(n)), dim3(256), 0, s, t);
+      e = hipGetLastError();
+    }
+    // the staging copy reads h, and the kernel reads buf: wait for both
+    const hipError_t e2 = hipStreamSynchronize(s);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return check_hip(e, "transfer launch");
+    if (e2 != hipSuccess) return check_hip(e2, "transfer sync");
+  }
+  // the new hierarchy's ghosts: source (and its interior copy), then state
+  int rc = enl_batch_prepare(nb, reflect);
+  if (rc) return rc;
+  fill_all(nb, 0, reflect, s);
+  return check_hip(hipGetLastError(), "enl_batch_transfer");
 }
 
 int enl_batch_destroy(void* handle) {

@@ -1,11 +1,12 @@
 """A two-level hierarchy resident in HBM, driven through the batch C ABI
 (enl_batch_*): build from an amr.Hierarchy, load the initial state, advance
-in plain (drop-in per patch), fused or graph mode, read states back."""
+in plain (drop-in per patch), fused or graph mode, read states back, tag the
+base level and regrid onto new boxes (main.C:489-502)."""
 import ctypes
 
 import numpy as np
 
-from . import _lib
+from . import _lib, amr
 from .gas import RK3
 from .workloads import pulses_at, state_from_overpressure
 
@@ -69,6 +70,7 @@ def refine_from_base(h, patch, wb):
 class DeviceHierarchy:
     def __init__(self, h, gas, reflect=(1, 1, 1, 1)):
         self.h, self.gas, self.reflect = h, gas, tuple(reflect)
+        self.absorb_width = 0.0
         self.lib = _lib.load()
         n = len(h.patches)
         desc = (_lib.EnlPatchDesc * n)()
@@ -110,6 +112,7 @@ class DeviceHierarchy:
         n0, dx0 = self.h.ncells(0), self.h.dx(0)
         D = ctypes.c_double * 2
         _lib.check(self.lib.enl_batch_set_absorb(self.handle, D(0.0, 0.0), D(n0 * dx0, n0 * dx0), float(width)))
+        self.absorb_width = float(width)
 
     def set_tiles(self, bx, by, level=None):
         if level is None:
@@ -151,13 +154,14 @@ class DeviceHierarchy:
             self.set_tiles(*best[lev], level=lev)
         return best, trials
 
-    def load_initial(self, wl, states=None):
-        """Upload the analytic initial state (or given per-patch w arrays),
-        fill ghosts, derive p, T and fill again."""
+    def load_initial(self, wl, states=None, sources=None):
+        """Upload the analytic initial state (or given per-patch w arrays)
+        and the ghosted sources (zero unless given), fill ghosts, derive p, T
+        and fill again."""
         gas = self.gas
         for k, p in enumerate(self.h.patches):
             w = patch_initial_state(self.h, p, wl, gas) if states is None else np.ascontiguousarray(states[k])
-            src = np.zeros((p.Ny, p.Nx))
+            src = np.zeros((p.Ny, p.Nx)) if sources is None else np.ascontiguousarray(sources[k], dtype=np.float64)
             _lib.check(self.lib.enl_batch_copy_state(self.handle, k, 1, _p(w), None, None, _p(src)))
         refl = self._reflect()
         _lib.check(self.lib.enl_batch_prepare(self.handle, refl))
@@ -184,6 +188,67 @@ class DeviceHierarchy:
             _lib.check(self.lib.enl_batch_copy_state(self.handle, k, 0, _p(w), _p(pr), _p(T), None))
             out.append((w, pr, T))
         return out
+
+    def sources(self):
+        """Per patch host copies of the ghosted source."""
+        out = []
+        for k, p in enumerate(self.h.patches):
+            src = np.empty((p.Ny, p.Nx))
+            _lib.check(self.lib.enl_batch_copy_state(self.handle, k, 0, None, None, None, _p(src)))
+            out.append(src)
+        return out
+
+    def set_states(self, states):
+        """Upload per patch (w, p, T) ghosted host arrays as the current state
+        (no ghost fill, no update_state)."""
+        for k, (w, pr, T) in enumerate(states):
+            arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (w, pr, T)]
+            _lib.check(self.lib.enl_batch_copy_state(self.handle, k, 1, *[_p(a) for a in arrs], None))
+
+    # ---- regrid -------------------------------------------------------------
+    def level0_tags(self, refine_tag_val=1):
+        """Tags of the current state on the base grid, (n0, n0) [j, i]: the
+        state's ghosts are filled, then tag_cells_ runs on every level-0
+        patch's device data (enlightning::tagGradientDetectorCells,
+        enlightning.C:586-628), the input of the next regrid's clustering."""
+        _lib.check(self.lib.enl_batch_fill(self.handle, self._reflect()))
+        n0 = self.h.ncells(0)
+        tags = np.zeros((n0, n0), np.int32)
+        ci = ctypes.c_int
+        tol = (ctypes.c_double * len(self.gas.tag_tolerance))(*self.gas.tag_tolerance)
+        for p in self.h.level_patches(0):
+            w, src = ctypes.c_void_p(), ctypes.c_void_p()
+            _lib.check(self.lib.enl_batch_patch_state(self.handle, p.pid, ctypes.byref(w), None, None,
+                                                      ctypes.byref(src)))
+            t = np.zeros((p.ny, p.nx), np.int32)
+            self.lib.tag_cells_(ctypes.byref(ci(p.box[0])), ctypes.byref(ci(p.box[1])), ctypes.byref(ci(p.box[2])),
+                                ctypes.byref(ci(p.box[3])), ctypes.byref(ci(p.g)), ctypes.byref(ci(p.g)), _p(t), w,
+                                src, ctypes.byref(ci(refine_tag_val)), tol, ctypes.byref(ci(5)))
+            _lib.check()
+            tags[p.box[2]:p.box[3] + 1, p.box[0]:p.box[1] + 1] = t
+        return tags
+
+    def regrid(self, new_h, transfer=None):
+        """A DeviceHierarchy on new_h's boxes holding this hierarchy's state,
+        moved over on the device by enl_batch_transfer (schedule
+        amr.make_transfer: copies from the old patches of each level, linear
+        refinement from the old coarser level elsewhere).  This hierarchy
+        keeps its interiors and stays usable; absorbing envelopes carry
+        over."""
+        tr = amr.make_transfer(new_h, self.h) if transfer is None else transfer
+        new = DeviceHierarchy(new_h, self.gas, self.reflect)
+        try:
+            c, it = tr["copy"], tr["interp"]
+            arrs = [np.ascontiguousarray(a) for a in c + it]
+            P = [_p(a) for a in arrs]
+            _lib.check(self.lib.enl_batch_transfer(new.handle, self.handle, self._reflect(), len(c[0]), *P[0:4],
+                                                   len(it[0]), *P[4:10]))
+            if self.absorb_width > 0:
+                new.set_absorb(self.absorb_width)
+        except Exception:
+            new.close()
+            raise
+        return new
 
     # ---- FP32-perturbation form ------------------------------------------
     def set_ambient(self, ambient):

@@ -5,9 +5,12 @@ level, fill the level's ghosts from the schedule (numpy, same arithmetic as
 the device fill kernels), call the oracle's rhs / runge_kutta / update_state
 on each patch (enlightning.C:459-583), refill the level's ghosts for the next
 finer level's coarse-to-fine refine; after the step, average fine data onto
-the coarse level.  The ghost-fill and coarsen semantics restate SAMRAI's
-operators, which are not vendored: parity with the reference is unpinned
-there; the per-patch kernels are the oracle's.
+the coarse level.  At a regrid (main.C:489-502) transfer_from moves the SOLN
+variables into a new hierarchy.  The ghost-fill, coarsen and transfer
+semantics restate SAMRAI's operators, which are not vendored: parity with
+the reference is unpinned there (closed forms pin them: copies are exact,
+linear refinement reproduces linear fields and conserves coarse averages);
+the per-patch kernels are the oracle's.
 """
 import numpy as np
 
@@ -70,6 +73,64 @@ class HostHierarchy:
         for lev in range(self.h.nlevels):
             self.fill_level(lev)
 
+    def _src_pool(self):
+        """The per-patch ghosted sources as one flat array (offsets, strides)."""
+        off = np.concatenate([[0], np.cumsum([s.size for s in self.src])[:-1]]).astype(np.int64)
+        return np.concatenate([s.ravel() for s in self.src]), off
+
+    def _set_src_pool(self, flat, off):
+        for k, s in enumerate(self.src):
+            s[...] = flat[off[k]:off[k] + s.size].reshape(s.shape)
+
+    def fill_source(self):
+        """Ghost fill of the source (one component, nothing negated at the
+        walls), level by level, with the state's schedule."""
+        S, off = self._src_pool()
+        sy = self.sy
+        for lev in range(self.h.nlevels):
+            f = self.h.fill[lev]
+            dp, dl, sp, sl = f["copy"]
+            S[off[dp] + dl] = S[off[sp] + sl]
+            dp, dl, sp, sl, fx, fy = f["interp"]
+            c = off[sp] + sl
+            S[off[dp] + dl] = S[c] + fx * (0.5 * (S[c + 1] - S[c - 1])) + fy * (0.5 * (S[c + sy[sp]] - S[c - sy[sp]]))
+            for key in ("edge", "corner"):
+                pp, dl, sl, _ = f[key]
+                S[off[pp] + dl] = S[off[pp] + sl]
+        self._set_src_pool(S, off)
+
+    def transfer_from(self, old, tr):
+        """Regrid transfer from HostHierarchy `old` (schedule
+        enlightning_amd.amr.make_transfer): old ghosts refilled (source, then
+        state), copies and linear refinements of w_n, p, T and the source
+        into this hierarchy's interiors, then this hierarchy's source and
+        state ghost fills.  Same arithmetic as the device transfer kernel."""
+        old.fill_source()
+        old.fill_all()
+        P, Q = self.pool, old.pool
+        S, soff = self._src_pool()
+        OS, osoff = old._src_pool()
+        dp, dl, sp, sl = tr["copy"]
+        if len(dp):
+            d0, s0 = self.off[dp] + dl, old.off[sp] + sl
+            for k in range(7):
+                P[d0 + k * self.sk[dp]] = Q[s0 + k * old.sk[sp]]
+            S[soff[dp] + dl] = OS[osoff[sp] + sl]
+        dp, dl, sp, sl, fx, fy = tr["interp"]
+        if len(dp):
+            d0 = self.off[dp] + dl
+            sy = old.sy[sp]
+
+            def refine(A, c):
+                return A[c] + fx * (0.5 * (A[c + 1] - A[c - 1])) + fy * (0.5 * (A[c + sy] - A[c - sy]))
+
+            for k in range(7):
+                P[d0 + k * self.sk[dp]] = refine(Q, old.off[sp] + sl + k * old.sk[sp])
+            S[soff[dp] + dl] = refine(OS, osoff[sp] + sl)
+        self._set_src_pool(S, soff)
+        self.fill_source()
+        self.fill_all()
+
     def coarsen(self):
         if self.h.coarsen is None:
             return
@@ -85,9 +146,12 @@ class HostHierarchy:
             P[c0 + k * self.sk[cp]] = s * (1.0 / (r * r))
 
     # ---- state --------------------------------------------------------------
-    def load_initial(self, states):
+    def load_initial(self, states, sources=None):
         for k, w in enumerate(states):
             self.view(k)[0][...] = w
+            if sources is not None:
+                self.src[k][...] = sources[k]
+        self.fill_source()
         self.fill_all()
         for k, p in enumerate(self.h.patches):
             w, pr, T = self.view(k)

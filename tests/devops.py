@@ -89,11 +89,19 @@ else:
     class Hierarchy(_HostHierarchy):
         """The CPU twin behind DeviceHierarchy's interface."""
 
-        def load_initial(self, wl, states=None):
+        def load_initial(self, wl, states=None, sources=None):
             from enlightning_amd.hierarchy import patch_initial_state
             if states is None:
                 states = [patch_initial_state(self.h, p, wl, self.gas) for p in self.h.patches]
-            super().load_initial(states)
+            super().load_initial(states, sources)
+
+        def sources(self):
+            return [s.copy() for s in self.src]
+
+        def set_states(self, states):
+            for k, (w, pr, T) in enumerate(states):
+                a, b, c = self.view(k)
+                a[...], b[...], c[...] = w, pr, T
 
         def advance(self, nsteps, dt, mode="graph", variant="hybrid_high_llf", rk=RK3):
             super().advance(nsteps, dt, rk, variant=variant)
@@ -103,6 +111,26 @@ else:
 
         def set_absorb(self, width):
             self.absorb_width = width
+
+        def level0_tags(self, refine_tag_val=1):
+            self.fill_all()
+            n0 = self.h.ncells(0)
+            tags = np.zeros((n0, n0), np.int32)
+            for p in self.h.level_patches(0):
+                w, _, _ = self.view(p.pid)
+                t = np.zeros((p.ny, p.nx), np.int32)
+                P.tag_cells(p.box, p.g, t, np.ascontiguousarray(w), self.src[p.pid], refine_tag_val,
+                                 self.gas.tag_tolerance)
+                tags[p.box[2]:p.box[3] + 1, p.box[0]:p.box[1] + 1] = t
+            return tags
+
+        def regrid(self, new_h, transfer=None):
+            from enlightning_amd import amr
+            tr = amr.make_transfer(new_h, self.h) if transfer is None else transfer
+            new = Hierarchy(new_h, self.gas, self.reflect)
+            new.absorb_width = self.absorb_width
+            new.transfer_from(self, tr)
+            return new
 
         def close(self):
             pass
