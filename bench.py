@@ -158,7 +158,7 @@ def amr_bench(args):
     t0 = time.perf_counter()
     rtags = d.level0_tags()
     t1 = time.perf_counter()
-    h2 = amr.build(n, 4, 4, 1.0, rtags, tile0=64)
+    h2 = amr.build(n, 4, 4, 1.0, rtags, tile0=64, previous=h)
     tr = amr.make_transfer(h2, h)
     t2 = time.perf_counter()
     d2 = d.regrid(h2, tr)

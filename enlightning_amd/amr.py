@@ -75,6 +75,8 @@ class Hierarchy:
     # ghost-fill schedule per level: dicts of int32/float64 numpy arrays
     fill: list = field(default_factory=list)
     coarsen: dict = None
+    # owner map per level (level index space -> pid or -1), built on demand
+    owners: dict = field(default_factory=dict, repr=False, compare=False)
 
     @property
     def nlevels(self):
@@ -159,6 +161,13 @@ def cluster_fine_boxes(tags, ratio, fine_max=64, fine_min=8):
     return boxes
 
 
+def _owner(h, level):
+    """The level's owner map, kept on the hierarchy."""
+    if level not in h.owners:
+        h.owners[level] = _owner_map(h.level_patches(level), h.ncells(level))
+    return h.owners[level]
+
+
 def _owner_map(patches, n):
     own = np.full((n, n), -1, np.int32)
     for p in patches:
@@ -166,9 +175,11 @@ def _owner_map(patches, n):
     return own
 
 
-def build(n0, ratio, g, L, coarse_tags, tile0=64, tag_buffer=2, fine_max=64, fine_min=8):
+def build(n0, ratio, g, L, coarse_tags, tile0=64, tag_buffer=2, fine_max=64, fine_min=8, previous=None):
     """Hierarchy with level 0 = base patches and, if any cell is tagged, a
-    level-1 of clustered boxes; with its ghost-fill and coarsen schedules."""
+    level-1 of clustered boxes; with its ghost-fill and coarsen schedules.
+    At a regrid, `previous` (the hierarchy being replaced) lends its level-0
+    fill schedule when its base patches are the same."""
     h = Hierarchy(n0=n0, ratio=ratio, g=g, L=L)
     for b in base_boxes(n0, tile0):
         h.patches.append(Patch(len(h.patches), 0, b, g))
@@ -176,59 +187,85 @@ def build(n0, ratio, g, L, coarse_tags, tile0=64, tag_buffer=2, fine_max=64, fin
         tb = buffer_tags(coarse_tags, tag_buffer)
         for b in cluster_fine_boxes(tb, ratio, fine_max, fine_min):
             h.patches.append(Patch(len(h.patches), 1, b, g))
-    h.fill = [make_fill(h, lev) for lev in range(h.nlevels)]
+    same_base = (previous is not None and (previous.n0, previous.ratio, previous.g) == (n0, ratio, g)
+                 and [p.box for p in previous.level_patches(0)] == [p.box for p in h.level_patches(0)]
+                 and [p.pid for p in previous.level_patches(0)] == [p.pid for p in h.level_patches(0)])
+    if same_base and 0 in previous.owners:
+        h.owners[0] = previous.owners[0]
+    h.fill = [previous.fill[0] if (lev == 0 and same_base) else make_fill(h, lev) for lev in range(h.nlevels)]
     h.coarsen = make_coarsen(h) if h.nlevels > 1 else None
     return h
 
 
+def _ghost_cells(pats, g):
+    """Ghost cells of every patch of a level as flat arrays (k, i, j), k the
+    index into pats: per patch the rows below and above the box (full
+    ghosted width), then the columns left and right of it (box rows)."""
+    ks, I, J = [], [], []
+    for k, p in enumerate(pats):
+        i0, i1, j0, j1 = p.box
+        ci = np.arange(i0 - g, i1 + g + 1)
+        rows = np.concatenate([np.arange(j0 - g, j0), np.arange(j1 + 1, j1 + g + 1)])
+        cols = np.concatenate([np.arange(i0 - g, i0), np.arange(i1 + 1, i1 + g + 1)])
+        cj = np.arange(j0, j1 + 1)
+        I.append(np.tile(ci, rows.size))
+        J.append(np.repeat(rows, ci.size))
+        I.append(np.tile(cols, cj.size))
+        J.append(np.repeat(cj, cols.size))
+        ks.append(2 * g * (ci.size + cj.size))
+    if not ks:
+        z = np.zeros(0, np.int64)
+        return z, z, z
+    return np.repeat(np.arange(len(ks)), ks), np.concatenate(I), np.concatenate(J)
+
+
+def _locals(box0, box2, Nx, g, i, j):
+    return ((i - box0 + g) + Nx * (j - box2 + g)).astype(np.int32)
+
+
 def make_fill(h, level):
-    """Ghost-fill schedule of one level (see the module docstring)."""
+    """Ghost-fill schedule of one level (see the module docstring).  Items
+    of one kind are independent (distinct destinations, interior sources),
+    so their order is free: here patch by patch, strip by strip."""
     n = h.ncells(level)
+    g = h.g
     pats = h.level_patches(level)
-    own = _owner_map(pats, n)
-    byid = {p.pid: p for p in h.patches}
-    if level > 0:
-        coarse = h.level_patches(level - 1)
-        cown = _owner_map(coarse, h.ncells(level - 1))
-    copy_dst_p, copy_dst_l, copy_src_p, copy_src_l = [], [], [], []
-    int_dst_p, int_dst_l, int_src_p, int_src_l, int_fx, int_fy = [], [], [], [], [], []
+    own = _owner(h, level)
+    # per global pid: box origin and ghosted row length
+    B0 = np.array([p.box[0] for p in h.patches], np.int64)
+    B2 = np.array([p.box[2] for p in h.patches], np.int64)
+    NX = np.array([p.Nx for p in h.patches], np.int64)
+    pid = np.array([p.pid for p in pats], np.int64)
+    r = h.ratio
+    K, I, J = _ghost_cells(pats, g)
+    inside = (I >= 0) & (I < n) & (J >= 0) & (J < n)
+    K, I, J = K[inside], I[inside], J[inside]
+    dpid = pid[K]
+    dst = _locals(B0[dpid], B2[dpid], NX[dpid], g, I, J)
+    owner = own[J, I].astype(np.int64)
+    have = owner >= 0
+    o = owner[have]
+    copy = (dpid[have].astype(np.int32), dst[have], o.astype(np.int32),
+            _locals(B0[o], B2[o], NX[o], g, I[have], J[have]))
+    ui, uj, up, ud = I[~have], J[~have], dpid[~have], dst[~have]
+    if ui.size:
+        assert level > 0, "a level-0 ghost cell inside the domain has no owner"
+        cown = _owner(h, level - 1)
+        ic, jc = ui // r, uj // r
+        c = cown[jc, ic].astype(np.int64)
+        assert (c >= 0).all()
+        interp = (up.astype(np.int32), ud, c.astype(np.int32), _locals(B0[c], B2[c], NX[c], g, ic, jc),
+                  ((ui - ic * r) + 0.5) / r - 0.5, ((uj - jc * r) + 0.5) / r - 0.5)
+    else:
+        interp = tuple(np.zeros(0, np.int32) for _ in range(4)) + (np.zeros(0), np.zeros(0))
     edge_p, edge_dst, edge_src, edge_neg = [], [], [], []
     corn_p, corn_dst, corn_src, corn_neg = [], [], [], []
-    r = h.ratio
     for p in pats:
-        g = p.g
         i0, i1, j0, j1 = p.box
-        I, J = np.meshgrid(np.arange(i0 - g, i1 + g + 1), np.arange(j0 - g, j1 + g + 1))
-        ghost = ~((I >= i0) & (I <= i1) & (J >= j0) & (J <= j1))
-        inside = (I >= 0) & (I < n) & (J >= 0) & (J < n)
-        gi, gj = I[ghost & inside], J[ghost & inside]
-        owner = own[gj, gi]
-        dst = p.local(gi, gj)
-        have = owner >= 0
-        for q in np.unique(owner[have]):
-            sel = have & (owner == q)
-            Q = byid[int(q)]
-            copy_dst_p.append(np.full(sel.sum(), p.pid, np.int32))
-            copy_dst_l.append(dst[sel].astype(np.int32))
-            copy_src_p.append(np.full(sel.sum(), Q.pid, np.int32))
-            copy_src_l.append(Q.local(gi[sel], gj[sel]).astype(np.int32))
-        if (~have).any():
-            assert level > 0, "a level-0 ghost cell inside the domain has no owner"
-            ui, uj = gi[~have], gj[~have]
-            ic, jc = ui // r, uj // r
-            cowner = cown[jc, ic]
-            assert (cowner >= 0).all()
-            for q in np.unique(cowner):
-                sel = cowner == q
-                C = byid[int(q)]
-                int_dst_p.append(np.full(sel.sum(), p.pid, np.int32))
-                int_dst_l.append(p.local(ui[sel], uj[sel]).astype(np.int32))
-                int_src_p.append(np.full(sel.sum(), C.pid, np.int32))
-                int_src_l.append(C.local(ic[sel], jc[sel]).astype(np.int32))
-                int_fx.append(((ui[sel] - ic[sel] * r) + 0.5) / r - 0.5)
-                int_fy.append(((uj[sel] - jc[sel] * r) + 0.5) / r - 0.5)
-        # physical boundary: edge strips over the patch's own tangential range
         touch = (i0 == 0, i1 == n - 1, j0 == 0, j1 == n - 1)
+        if not any(touch):
+            continue
+        # physical boundary: edge strips over the patch's own tangential range
         if touch[0]:
             jj, ll = np.meshgrid(np.arange(j0, j1 + 1), np.arange(1, g + 1))
             edge_p.append(np.full(jj.size, p.pid, np.int32))
@@ -273,10 +310,8 @@ def make_fill(h, level):
         return np.concatenate(lst).astype(dtype) if lst else np.zeros(0, dtype)
 
     return {
-        "copy": (cat(copy_dst_p, np.int32), cat(copy_dst_l, np.int32), cat(copy_src_p, np.int32),
-                 cat(copy_src_l, np.int32)),
-        "interp": (cat(int_dst_p, np.int32), cat(int_dst_l, np.int32), cat(int_src_p, np.int32),
-                   cat(int_src_l, np.int32), cat(int_fx, np.float64), cat(int_fy, np.float64)),
+        "copy": copy,
+        "interp": interp,
         # neg: side code 1..4 = left, right, bottom, top (which wall's REFLECT
         # flag applies and which momentum it negates)
         "edge": (cat(edge_p, np.int32), cat(edge_dst, np.int32), cat(edge_src, np.int32), cat(edge_neg, np.int32)),
@@ -285,83 +320,102 @@ def make_fill(h, level):
     }
 
 
+def _interior_cells(pats):
+    """Interior cells of every patch as flat arrays (k, i, j), k the index
+    into pats, row by row."""
+    ks, I, J = [], [], []
+    for k, p in enumerate(pats):
+        ci, cj = np.arange(p.box[0], p.box[1] + 1), np.arange(p.box[2], p.box[3] + 1)
+        I.append(np.tile(ci, cj.size))
+        J.append(np.repeat(cj, ci.size))
+        ks.append(ci.size * cj.size)
+    if not ks:
+        z = np.zeros(0, np.int64)
+        return z, z, z
+    return np.repeat(np.arange(len(ks)), ks), np.concatenate(I), np.concatenate(J)
+
+
+def _patch_arrays(h):
+    """Per global pid: box origin and ghosted row length."""
+    return (np.array([p.box[0] for p in h.patches], np.int64), np.array([p.box[2] for p in h.patches], np.int64),
+            np.array([p.Nx for p in h.patches], np.int64))
+
+
 def make_coarsen(h):
     """(coarse patch, coarse local, fine patch, fine local of the lower-left
     fine cell) for every coarse cell under a fine patch."""
     r = h.ratio
-    coarse = h.level_patches(0)
-    cown = _owner_map(coarse, h.ncells(0))
-    byid = {p.pid: p for p in h.patches}
-    cp, cl, fp, fl = [], [], [], []
-    for f in h.level_patches(1):
+    cown = _owner(h, 0)
+    B0, B2, NX = _patch_arrays(h)
+    coarse_boxes = []
+    fine = h.level_patches(1)
+    for f in fine:
         i0, i1, j0, j1 = f.box
-        IC, JC = np.meshgrid(np.arange(i0 // r, (i1 + 1) // r), np.arange(j0 // r, (j1 + 1) // r))
-        IC, JC = IC.ravel(), JC.ravel()
-        owner = cown[JC, IC]
-        for q in np.unique(owner):
-            sel = owner == q
-            C = byid[int(q)]
-            cp.append(np.full(sel.sum(), C.pid, np.int32))
-            cl.append(C.local(IC[sel], JC[sel]).astype(np.int32))
-            fp.append(np.full(sel.sum(), f.pid, np.int32))
-            fl.append(f.local(IC[sel] * r, JC[sel] * r).astype(np.int32))
-    return tuple(np.concatenate(x).astype(np.int32) for x in (cp, cl, fp, fl))
+        coarse_boxes.append(Patch(f.pid, 0, (i0 // r, (i1 + 1) // r - 1, j0 // r, (j1 + 1) // r - 1), f.g))
+    K, IC, JC = _interior_cells(coarse_boxes)
+    fp = np.array([f.pid for f in fine], np.int64)[K]
+    cp = cown[JC, IC].astype(np.int64)
+    g = h.g
+    return (cp.astype(np.int32), _locals(B0[cp], B2[cp], NX[cp], g, IC, JC), fp.astype(np.int32),
+            _locals(B0[fp], B2[fp], NX[fp], g, IC * r, JC * r))
 
 
 def make_transfer(new, old):
     """Regrid transfer schedule from hierarchy `old` to `new` (same base
-    grid, ratio and ghost width): for every interior cell of every new
-    patch, either a copy item (new patch, new local) <- (old patch of the
-    same level holding the cell, its local), or, where no old patch of that
-    level held it, an interp item from the old coarser level's cell with the
-    fine centre's offset (fx, fy) in coarse widths.  Ghosts are not listed:
-    the new hierarchy's own ghost fill takes them afterwards."""
+    grid, ratio and ghost width).  A new patch with the level and box of an
+    old one is a "same" pair (new pid, old pid), copied whole; for every
+    interior cell of the other new patches, either a copy item (new patch,
+    new local) <- (old patch of the same level holding the cell, its local),
+    or, where no old patch of that level held it, an interp item from the
+    old coarser level's cell with the fine centre's offset (fx, fy) in
+    coarse widths.  Ghosts are not listed: the new hierarchy's own ghost
+    fill takes them afterwards."""
     if (new.n0, new.ratio, new.g) != (old.n0, old.ratio, old.g):
         raise ValueError("make_transfer: the hierarchies differ in base grid, ratio or ghost width")
-    r = new.ratio
-    c_dp, c_dl, c_sp, c_sl = [], [], [], []
-    i_dp, i_dl, i_sp, i_sl, i_fx, i_fy = [], [], [], [], [], []
-    byid = {p.pid: p for p in old.patches}
-    owners = {}
-    for lev in range(new.nlevels):
-        if lev < old.nlevels:
-            owners[lev] = _owner_map(old.level_patches(lev), old.ncells(lev))
+    r, g = new.ratio, new.g
+    old_box = {(p.level, p.box): p.pid for p in old.patches}
+    s_new, s_old, rest = [], [], []
     for p in new.patches:
-        lev = p.level
-        I, J = np.meshgrid(np.arange(p.box[0], p.box[1] + 1), np.arange(p.box[2], p.box[3] + 1))
-        I, J = I.ravel(), J.ravel()
-        owner = owners[lev][J, I] if lev in owners else np.full(I.size, -1, np.int32)
+        q = old_box.get((p.level, p.box))
+        if q is None:
+            rest.append(p)
+        else:
+            s_new.append(p.pid)
+            s_old.append(q)
+    nB0, nB2, nNX = _patch_arrays(new)
+    oB0, oB2, oNX = _patch_arrays(old)
+    copy = [[], [], [], []]
+    interp = [[], [], [], [], [], []]
+    for lev in sorted({p.level for p in rest}):
+        pats = [p for p in rest if p.level == lev]
+        K, I, J = _interior_cells(pats)
+        dp = np.array([p.pid for p in pats], np.int64)[K]
+        dl = _locals(nB0[dp], nB2[dp], nNX[dp], g, I, J)
+        if lev < old.nlevels:
+            owner = _owner(old, lev)[J, I].astype(np.int64)
+        else:
+            owner = np.full(I.size, -1, np.int64)
         have = owner >= 0
-        for q in np.unique(owner[have]):
-            sel = have & (owner == q)
-            Q = byid[int(q)]
-            c_dp.append(np.full(sel.sum(), p.pid, np.int32))
-            c_dl.append(p.local(I[sel], J[sel]).astype(np.int32))
-            c_sp.append(np.full(sel.sum(), Q.pid, np.int32))
-            c_sl.append(Q.local(I[sel], J[sel]).astype(np.int32))
+        o = owner[have]
+        for lst, a in zip(copy, (dp[have], dl[have], o, _locals(oB0[o], oB2[o], oNX[o], g, I[have], J[have]))):
+            lst.append(a)
         if (~have).any():
-            if lev == 0 or lev - 1 not in owners:
+            if lev == 0 or lev - 1 >= old.nlevels:
                 raise ValueError("make_transfer: new level-%d cells have no old source" % lev)
             ui, uj = I[~have], J[~have]
             ic, jc = ui // r, uj // r
-            cown = owners[lev - 1][jc, ic]
-            if (cown < 0).any():
+            c = _owner(old, lev - 1)[jc, ic].astype(np.int64)
+            if (c < 0).any():
                 raise ValueError("make_transfer: a new fine cell lies outside the old coarser level")
-            for q in np.unique(cown):
-                sel = cown == q
-                C = byid[int(q)]
-                i_dp.append(np.full(sel.sum(), p.pid, np.int32))
-                i_dl.append(p.local(ui[sel], uj[sel]).astype(np.int32))
-                i_sp.append(np.full(sel.sum(), C.pid, np.int32))
-                i_sl.append(C.local(ic[sel], jc[sel]).astype(np.int32))
-                i_fx.append(((ui[sel] - ic[sel] * r) + 0.5) / r - 0.5)
-                i_fy.append(((uj[sel] - jc[sel] * r) + 0.5) / r - 0.5)
+            for lst, a in zip(interp, (dp[~have], dl[~have], c, _locals(oB0[c], oB2[c], oNX[c], g, ic, jc),
+                                       ((ui - ic * r) + 0.5) / r - 0.5, ((uj - jc * r) + 0.5) / r - 0.5)):
+                lst.append(a)
 
     def cat(lst, dtype):
         return np.concatenate(lst).astype(dtype) if lst else np.zeros(0, dtype)
 
     return {
-        "copy": (cat(c_dp, np.int32), cat(c_dl, np.int32), cat(c_sp, np.int32), cat(c_sl, np.int32)),
-        "interp": (cat(i_dp, np.int32), cat(i_dl, np.int32), cat(i_sp, np.int32), cat(i_sl, np.int32),
-                   cat(i_fx, np.float64), cat(i_fy, np.float64)),
+        "same": (np.array(s_new, np.int32), np.array(s_old, np.int32)),
+        "copy": tuple(cat(x, np.int32) for x in copy),
+        "interp": tuple(cat(x, np.int32) for x in interp[:4]) + tuple(cat(x, np.float64) for x in interp[4:]),
     }

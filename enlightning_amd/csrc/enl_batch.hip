@@ -218,6 +218,27 @@ __global__ void __launch_bounds__(256) transfer_kernel(TransferArgs t) {
   }
 }
 
+// Regrid transfer of whole patches whose box did not change: the interior
+// of old patch s_old[y] into new patch s_new[y] (same shape).
+__global__ void __launch_bounds__(256) transfer_patch_kernel(double* dpool, const PatchDev* dpat, const double* spool,
+                                                              const PatchDev* spat, const int* s_new,
+                                                              const int* s_old, int npairs) {
+  for (int y = blockIdx.y; y < npairs; y += gridDim.y) {
+    const PatchDev& D = dpat[s_new[y]];
+    const PatchDev& S = spat[s_old[y]];
+    const int n = D.nx * D.ny;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+      const int i = c % D.nx, j = c / D.nx;
+      const long dl = (i + D.g) + (long)(j + D.g) * D.sy;
+      const long sl = (i + S.g) + (long)(j + S.g) * S.sy;
+      const double* src = spool + S.state[0] + sl;
+      double* dst = dpool + D.state[0] + dl;
+      for (int k = 0; k < 7; k++) dst[k * D.sk] = src[k * S.sk];
+      dpool[D.src + dl] = spool[S.src + sl];
+    }
+  }
+}
+
 // fine-to-coarse average of the r x r fine cells over a coarse cell
 __global__ void __launch_bounds__(256) coarsen_kernel(double* pool, const PatchDev* patches, int buf, int r, int n,
                                                        const int* cp, const int* cl, const int* fp, const int* fl) {
@@ -1008,17 +1029,25 @@ int enl_batch_set_absorb(void* handle, const double* grid_xlo, const double* gri
   return check_hip(e, "enl_batch_set_absorb upload");
 }
 
-int enl_batch_transfer(void* new_handle, void* old_handle, const int* reflect, int ncopy, const int* c_dp,
-                       const int* c_dl, const int* c_sp, const int* c_sl, int ninterp, const int* i_dp,
-                       const int* i_dl, const int* i_sp, const int* i_sl, const double* i_fx, const double* i_fy) {
+int enl_batch_transfer(void* new_handle, void* old_handle, const int* reflect, int nsame, const int* s_new,
+                       const int* s_old, int ncopy, const int* c_dp, const int* c_dl, const int* c_sp,
+                       const int* c_sl, int ninterp, const int* i_dp, const int* i_dl, const int* i_sp,
+                       const int* i_sl, const double* i_fx, const double* i_fy) {
   Batch* nb = (Batch*)new_handle;
   Batch* ob = (Batch*)old_handle;
   if (!nb || !ob || nb == ob) return set_error(-2, "enl_batch_transfer: needs two distinct batches");
   if (!reflect) return set_error(-3, "enl_batch_transfer: null reflect");
   if (nb->g != ob->g || nb->ratio != ob->ratio) return set_error(-2, "enl_batch_transfer: ghost width or ratio differ");
-  if (ncopy < 0 || ninterp < 0 || (ncopy && !(c_dp && c_dl && c_sp && c_sl)) ||
-      (ninterp && !(i_dp && i_dl && i_sp && i_sl && i_fx && i_fy)))
+  if (nsame < 0 || ncopy < 0 || ninterp < 0 || (nsame && !(s_new && s_old)) ||
+      (ncopy && !(c_dp && c_dl && c_sp && c_sl)) || (ninterp && !(i_dp && i_dl && i_sp && i_sl && i_fx && i_fy)))
     return set_error(-3, "enl_batch_transfer: bad item lists");
+  for (int q = 0; q < nsame; q++) {
+    if (s_new[q] < 0 || s_new[q] >= nb->npatch || s_old[q] < 0 || s_old[q] >= ob->npatch)
+      return set_error(-2, "enl_batch_transfer: patch pair %d names a bad patch", q);
+    const PatchDev &D = nb->hp[s_new[q]], &S = ob->hp[s_old[q]];
+    if (D.level != S.level || D.nx != S.nx || D.ny != S.ny)
+      return set_error(-2, "enl_batch_transfer: patch pair %d differs in level or shape", q);
+  }
   // every item must address cells of its patches' ghosted boxes, with the
   // refinement stencil (+-1 cell, +-1 row) inside the coarse patch's box
   for (int q = 0; q < ncopy; q++) {
@@ -1042,17 +1071,17 @@ int enl_batch_transfer(void* new_handle, void* old_handle, const int* reflect, i
   for (int lev = 0; lev < ob->nlevels; lev++) fill_level(ob, lev, -1, -1, 1, reflect, s);
   fill_all(ob, 0, reflect, s);
   const long n = (long)ncopy + ninterp;
-  if (n > 0) {
-    const size_t ibytes = sizeof(int) * (size_t)(4 * ncopy + 4 * ninterp);
+  if (n + nsame > 0) {
+    const size_t ibytes = sizeof(int) * (size_t)(2 * nsame + 4 * ncopy + 4 * ninterp);
     const size_t bytes = ibytes + sizeof(double) * (size_t)(2 * ninterp);
     char* buf = nullptr;
     hipError_t e = hipMalloc((void**)&buf, bytes);
     if (e != hipSuccess) return check_hip(e, "hipMalloc(transfer items)");
     std::vector<char> h(bytes);
     int* hi = (int*)h.data();
-    const int* lists[8] = {c_dp, c_dl, c_sp, c_sl, i_dp, i_dl, i_sp, i_sl};
-    for (int a = 0; a < 8; a++) {
-      const int cnt = a < 4 ? ncopy : ninterp;
+    const int* lists[10] = {c_dp, c_dl, c_sp, c_sl, i_dp, i_dl, i_sp, i_sl, s_new, s_old};
+    for (int a = 0; a < 10; a++) {
+      const int cnt = a < 4 ? ncopy : (a < 8 ? ninterp : nsame);
       if (cnt) memcpy(hi, lists[a], sizeof(int) * (size_t)cnt);
       hi += cnt;
     }
@@ -1080,8 +1109,15 @@ int enl_batch_transfer(void* new_handle, void* old_handle, const int* reflect, i
     t.i_sl = t.i_dp + 3 * ninterp;
     t.i_fx = (const double*)(buf + ibytes);
     t.i_fy = t.i_fx + ninterp;
-    if (e == hipSuccess) {
+    const int* sn = t.i_dp + 4 * ninterp;
+    if (e == hipSuccess && n > 0) {
       hipLaunchKernelGGL(transfer_kernel, dim3(grid_for(n)), dim3(256), 0, s, t);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess && nsame > 0) {
+      const unsigned gy = (unsigned)(nsame < 65535 ? nsame : 65535);
+      hipLaunchKernelGGL(transfer_patch_kernel, dim3(16, gy), dim3(256), 0, s, nb->pool, nb->dp, ob->pool, ob->dp,
+                         sn, sn + nsame, nsame);
       e = hipGetLastError();
     }
     // the staging copy reads h, and the kernel reads buf: wait for both

@@ -231,18 +231,19 @@ class DeviceHierarchy:
     def regrid(self, new_h, transfer=None):
         """A DeviceHierarchy on new_h's boxes holding this hierarchy's state,
         moved over on the device by enl_batch_transfer (schedule
-        amr.make_transfer: copies from the old patches of each level, linear
-        refinement from the old coarser level elsewhere).  This hierarchy
+        amr.make_transfer: unchanged patches whole, copies from the old
+        patches of each level, linear refinement from the old coarser level
+        elsewhere).  This hierarchy
         keeps its interiors and stays usable; absorbing envelopes carry
         over."""
         tr = amr.make_transfer(new_h, self.h) if transfer is None else transfer
         new = DeviceHierarchy(new_h, self.gas, self.reflect)
         try:
-            c, it = tr["copy"], tr["interp"]
-            arrs = [np.ascontiguousarray(a) for a in c + it]
+            sm, c, it = tr["same"], tr["copy"], tr["interp"]
+            arrs = [np.ascontiguousarray(a) for a in sm + c + it]
             P = [_p(a) for a in arrs]
-            _lib.check(self.lib.enl_batch_transfer(new.handle, self.handle, self._reflect(), len(c[0]), *P[0:4],
-                                                   len(it[0]), *P[4:10]))
+            _lib.check(self.lib.enl_batch_transfer(new.handle, self.handle, self._reflect(), len(sm[0]), *P[0:2],
+                                                   len(c[0]), *P[2:6], len(it[0]), *P[6:12]))
             if self.absorb_width > 0:
                 new.set_absorb(self.absorb_width)
         except Exception:

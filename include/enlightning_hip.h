@@ -290,20 +290,24 @@ int enl_batch_set_absorb(void *handle, const double *grid_xlo, const double *gri
  * (gridding_algorithm->regridAllFinerLevels, main.C:489-502, which fills a
  * new level's SOLN variables w_n, pressure, temperature and source,
  * enlightning.C:259-281, from the old level where it existed and by
- * LINEAR_REFINE from the coarser level elsewhere).  Items are
- * (new patch, new local index) <- (old patch, old local index) pairs:
- *   copy:    every SOLN component of an old cell of the same level;
- *   interp:  the linear refinement of an old coarser-level cell (same
- *            expression as the ghost fill's interp).
+ * LINEAR_REFINE from the coarser level elsewhere).  Lists:
+ *   same:    (new patch, old patch) pairs with the same level and box: the
+ *            whole interior is copied;
+ *   copy:    (new patch, new local) <- (old patch, old local): every SOLN
+ *            component of an old cell of the same level;
+ *   interp:  (new patch, new local) <- (old patch, old local): the linear
+ *            refinement of an old coarser-level cell (same expression as
+ *            the ghost fill's interp).
  * The old hierarchy's ghosts are refilled first (its interiors are not
  * written); afterwards the new hierarchy's source and state ghosts are
  * filled, so it is ready for enl_batch_advance.  Items that address cells
  * outside their patches' ghosted boxes (or refinement stencils that leave
- * the coarse patch) are rejected before anything runs. */
-int enl_batch_transfer(void *new_handle, void *old_handle, const int *reflect, int ncopy, const int *c_dp,
-                       const int *c_dl, const int *c_sp, const int *c_sl, int ninterp, const int *i_dp,
-                       const int *i_dl, const int *i_sp, const int *i_sl, const double *i_fx,
-                       const double *i_fy);
+ * the coarse patch) and pairs of different shape are rejected before
+ * anything runs. */
+int enl_batch_transfer(void *new_handle, void *old_handle, const int *reflect, int nsame, const int *s_new,
+                       const int *s_old, int ncopy, const int *c_dp, const int *c_dl, const int *c_sp,
+                       const int *c_sl, int ninterp, const int *i_dp, const int *i_dl, const int *i_sp,
+                       const int *i_sl, const double *i_fx, const double *i_fy);
 int enl_batch_destroy(void *handle);
 
 /* ---- FP32-perturbation form of the batched hierarchy (no reference

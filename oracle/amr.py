@@ -102,12 +102,19 @@ class HostHierarchy:
     def transfer_from(self, old, tr):
         """Regrid transfer from HostHierarchy `old` (schedule
         enlightning_amd.amr.make_transfer): old ghosts refilled (source, then
-        state), copies and linear refinements of w_n, p, T and the source
-        into this hierarchy's interiors, then this hierarchy's source and
+        state), whole interiors of unchanged patches, copies and linear
+        refinements of w_n, p, T and the source into this hierarchy's
+        interiors, then this hierarchy's source and
         state ghost fills.  Same arithmetic as the device transfer kernel."""
         old.fill_source()
         old.fill_all()
         P, Q = self.pool, old.pool
+        for dk, sk in zip(*tr["same"]):
+            a, b = self.view(int(dk)), old.view(int(sk))
+            g = self.h.patches[dk].g
+            for x, y in zip(a, b):
+                x[..., g:-g, g:-g] = y[..., g:-g, g:-g]
+            self.src[dk][g:-g, g:-g] = old.src[sk][g:-g, g:-g]
         S, soff = self._src_pool()
         OS, osoff = old._src_pool()
         dp, dl, sp, sl = tr["copy"]

@@ -5,9 +5,11 @@
 SAMRAI's regrid is not vendored, so the transfer is pinned by what its
 operators must do, independent of the code under test:
 
-* every interior cell of every new patch is listed exactly once; a copy
-  names the same global cell on an old patch of the same level; a refine
-  names the coarse cell (i // r, j // r) with the fine centre's offset;
+* every interior cell of every new patch is listed exactly once (whole
+  patches whose box did not change, else cell by cell); a copy names the
+  same global cell on an old patch of the same level; a refine names the
+  coarse cell (i // r, j // r) with the fine centre's offset; the level-0
+  fill schedule lent by the previous hierarchy equals a fresh one;
 * copies are exact, and a transfer onto the same boxes is the identity;
 * linear refinement reproduces linear fields (cells whose coarse stencil
   leaves the domain excepted: the wall ghosts are copies, not linear);
@@ -48,7 +50,16 @@ def test_schedule_covers_each_interior_cell_once(pair):
     dp, dl, sp, sl = tr["copy"]
     ip, il, isp, isl, fx, fy = tr["interp"]
     assert len(ip) > 0 and len(dp) > 0
+    sn, so = tr["same"]
     seen = {p.pid: np.zeros(p.Nx * p.Ny, np.int32) for p in new.patches}
+    # unchanged patches: every level-0 patch, and fine boxes that survived
+    assert len(sn) > len(new.level_patches(0))
+    for a, b in zip(sn, so):
+        A, B = new.patches[a], old.patches[b]
+        assert (A.level, A.box) == (B.level, B.box)
+        m = np.zeros((A.Ny, A.Nx), np.int32)
+        m[A.g:-A.g, A.g:-A.g] = 1
+        seen[a] += m.ravel()
     for q in range(len(dp)):
         seen[dp[q]][dl[q]] += 1
     for q in range(len(ip)):
@@ -75,8 +86,26 @@ def test_schedule_covers_each_interior_cell_once(pair):
         assert not old_fine[j, i]
         assert global_of(C, isl[q]) == (i // r, j // r)
         assert fx[q] == ((i % r) + 0.5) / r - 0.5 and fy[q] == ((j % r) + 0.5) / r - 0.5
-    # level 0 is unchanged: copies only
-    assert all(newp[p].level == 1 for p in ip)
+    # level 0 is unchanged: whole-patch pairs only
+    assert all(newp[p].level == 1 for p in ip) and all(newp[p].level == 1 for p in dp)
+
+
+def test_level0_fill_lent_by_the_previous_hierarchy(pair):
+    old, new = pair
+    tags = np.zeros((WL.n, WL.n), np.int32)
+    tags[10:14, 20:30] = 1
+    fresh = amr.build(WL.n, 4, 4, 1.0, tags, tile0=16)
+    lent = amr.build(WL.n, 4, 4, 1.0, tags, tile0=16, previous=old)
+    assert lent.fill[0] is old.fill[0]
+    for key in fresh.fill[0]:
+        for x, y in zip(fresh.fill[0][key], lent.fill[0][key]):
+            assert np.array_equal(x, y), key
+    for key in fresh.fill[1]:
+        for x, y in zip(fresh.fill[1][key], lent.fill[1][key]):
+            assert np.array_equal(x, y), key
+    # a different base tiling is not lent
+    other = amr.build(WL.n, 4, 4, 1.0, tags, tile0=32, previous=old)
+    assert other.fill[0] is not old.fill[0]
 
 
 def test_identity_on_the_same_boxes(pair):
@@ -88,8 +117,9 @@ def test_identity_on_the_same_boxes(pair):
     for s in a.src:
         s[...] = rng.standard_normal(s.shape)
     b = HostHierarchy(old, gas)
-    b.transfer_from(a, amr.make_transfer(old, old))
-    assert len(amr.make_transfer(old, old)["interp"][0]) == 0
+    tr = amr.make_transfer(old, old)
+    b.transfer_from(a, tr)
+    assert len(tr["interp"][0]) == 0 and len(tr["copy"][0]) == 0 and len(tr["same"][0]) == len(old.patches)
     for k, p in enumerate(old.patches):
         g = p.g
         wa, pa, Ta = a.view(k)
@@ -150,6 +180,10 @@ def test_linear_fields_and_exact_copies(pair):
         checked += ok.sum()
     assert checked > 0.9 * sum(p.nx * p.ny for p in new.patches)
     # copies carry the old values bit for bit
+    for x, y in zip(*tr["same"]):
+        g = new.patches[x].g
+        for u, v in zip(b.view(x), a.view(y)):
+            assert np.array_equal(u[..., g:-g, g:-g], v[..., g:-g, g:-g])
     dp, dl, sp, sl = tr["copy"]
     for q in range(0, len(dp), 7):
         D, S = new.patches[dp[q]], old.patches[sp[q]]

@@ -84,10 +84,19 @@ def test_transfer_rejects_out_of_range_items():
     tr = amr.make_transfer(new, old)
     dp, dl, sp, sl, fx, fy = (a.copy() for a in tr["interp"])
     sl[0] = 0  # the refinement stencil would leave the coarse patch's ghosted box
-    bad = {"copy": tr["copy"], "interp": (dp, dl, sp, sl, fx, fy)}
+    bad = {"same": tr["same"], "copy": tr["copy"], "interp": (dp, dl, sp, sl, fx, fy)}
     if D.DRY:
         pytest.skip("the range check is the C ABI's")
     with pytest.raises(RuntimeError, match="out of range"):
+        dev.regrid(new, bad)
+    # a whole-patch pair of different shapes
+    sn, so = (a.copy() for a in tr["same"])
+    fine_new = [p.pid for p in new.level_patches(1)]
+    shape = (new.patches[fine_new[0]].nx, new.patches[fine_new[0]].ny)
+    fine_old = [p.pid for p in old.level_patches(1) if (p.nx, p.ny) != shape]
+    sn[0], so[0] = fine_new[0], fine_old[0]
+    bad = {"same": (sn, so), "copy": tr["copy"], "interp": tr["interp"]}
+    with pytest.raises(RuntimeError, match="differs in level or shape"):
         dev.regrid(new, bad)
     dev.close()
 
@@ -122,7 +131,7 @@ def test_regrid_inside_a_run_follows_cpu_twin():
     dev.advance(2, dt, mode="graph")
     tags = dev.level0_tags()
     tags = tags | np.roll(tags, (3, -2), axis=(0, 1))
-    new = amr.build(WL64.n, 4, 4, 1.0, tags, tile0=16)
+    new = amr.build(WL64.n, 4, 4, 1.0, tags, tile0=16, previous=old)
     assert [p.box for p in new.level_patches(1)] != [p.box for p in old.level_patches(1)]
     tr = amr.make_transfer(new, old)
     dnew = dev.regrid(new, tr)
