@@ -187,7 +187,21 @@ def build(n0, ratio, g, L, coarse_tags, tile0=64, tag_buffer=2, fine_max=64, fin
         tb = buffer_tags(coarse_tags, tag_buffer)
         for b in cluster_fine_boxes(tb, ratio, fine_max, fine_min):
             h.patches.append(Patch(len(h.patches), 1, b, g))
-    same_base = (previous is not None and (previous.n0, previous.ratio, previous.g) == (n0, ratio, g)
+    return _schedules(h, previous)
+
+
+def from_boxes(n0, ratio, g, L, boxes):
+    """Hierarchy from (level, ifirst0, ilast0, ifirst1, ilast1) rows in pid
+    order (a restart's boxes), with its schedules."""
+    h = Hierarchy(n0=n0, ratio=ratio, g=g, L=L)
+    for row in boxes:
+        lev, b = int(row[0]), tuple(int(v) for v in row[1:5])
+        h.patches.append(Patch(len(h.patches), lev, b, g))
+    return _schedules(h, None)
+
+
+def _schedules(h, previous):
+    same_base = (previous is not None and (previous.n0, previous.ratio, previous.g) == (h.n0, h.ratio, h.g)
                  and [p.box for p in previous.level_patches(0)] == [p.box for p in h.level_patches(0)]
                  and [p.pid for p in previous.level_patches(0)] == [p.pid for p in h.level_patches(0)])
     if same_base and 0 in previous.owners:

@@ -205,6 +205,16 @@ class DeviceHierarchy:
             arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (w, pr, T)]
             _lib.check(self.lib.enl_batch_copy_state(self.handle, k, 1, *[_p(a) for a in arrs], None))
 
+    def upload_all(self, states, sources):
+        """A saved state: per patch (w, p, T) and the ghosted source, then the
+        source's and the state's ghost fills (p, T are taken as saved)."""
+        for k, ((w, pr, T), src) in enumerate(zip(states, sources)):
+            arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (w, pr, T, src)]
+            _lib.check(self.lib.enl_batch_copy_state(self.handle, k, 1, *[_p(a) for a in arrs]))
+        refl = self._reflect()
+        _lib.check(self.lib.enl_batch_prepare(self.handle, refl))
+        _lib.check(self.lib.enl_batch_fill(self.handle, refl))
+
     # ---- regrid -------------------------------------------------------------
     def level0_tags(self, refine_tag_val=1):
         """Tags of the current state on the base grid, (n0, n0) [j, i]: the

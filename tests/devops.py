@@ -103,6 +103,13 @@ else:
                 a, b, c = self.view(k)
                 a[...], b[...], c[...] = w, pr, T
 
+        def upload_all(self, states, sources):
+            self.set_states(states)
+            for k, src in enumerate(sources):
+                self.src[k][...] = src
+            self.fill_source()
+            self.fill_all()
+
         def advance(self, nsteps, dt, mode="graph", variant="hybrid_high_llf", rk=RK3):
             super().advance(nsteps, dt, rk, variant=variant)
 
