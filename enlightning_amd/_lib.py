@@ -100,6 +100,7 @@ PROTOTYPES = {
     "enl_batch_update_state": ([_P, ctypes.POINTER(EnlGas)], _I),
     "enl_batch_advance": ([_P, _I, _I, ctypes.POINTER(EnlGas), _P, _I, _P, _P, _P, _D, _I], _I),
     "enl_batch_set_absorb": ([_P, _P, _P, _D], _I),
+    "enl_batch_tag_cells": ([_P, _I, _P, _I, _P, _P], _I),
     "enl_batch_transfer": ([_P, _P, _P, _I, _P, _P, _I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P], _I),
     "enl_add_source": ([_I] * 4 + [_P, _P, ctypes.POINTER(EnlSource), _I, _I, _D, _P, ctypes.c_long], _I),
     "enl_read_lightning": ([ctypes.c_char_p, _P, _I], _I),

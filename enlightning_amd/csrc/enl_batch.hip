@@ -1133,6 +1133,51 @@ int enl_batch_transfer(void* new_handle, void* old_handle, const int* reflect, i
   return check_hip(hipGetLastError(), "enl_batch_transfer");
 }
 
+int enl_batch_tag_cells(void* handle, int level, const int* reflect, int refine_tag_val,
+                        const double* tag_tolerance, int* tags) {
+  Batch* b = (Batch*)handle;
+  if (!b || !reflect || !tag_tolerance || !tags) return set_error(-3, "enl_batch_tag_cells: null argument");
+  if (level < 0 || level >= b->nlevels) return set_error(-2, "enl_batch_tag_cells: bad level %d", level);
+  long total = 0;
+  for (const PatchDev& p : b->hp)
+    if (p.level == level) total += (long)p.nx * p.ny;
+  if (total == 0) return 0;
+  hipStream_t s = lib_stream();
+  const bool dev = is_device_ptr(tags);
+  int* out = tags;
+  if (!dev) {
+    hipError_t e = hipMalloc((void**)&out, sizeof(int) * (size_t)total);
+    if (e != hipSuccess) return check_hip(e, "hipMalloc(tags)");
+  }
+  // the stencil reads the state's ghosts; the source's are filled by
+  // enl_batch_prepare
+  fill_all(b, 0, reflect, s);
+  hipError_t e = hipSuccess;
+  long off = 0;
+  for (const PatchDev& p : b->hp) {
+    if (p.level != level) continue;
+    TagArgs a;
+    a.tags = out + off;
+    a.w = b->pool + p.state[0];
+    a.H0 = b->pool + p.src;
+    a.geom.nx = p.nx;
+    a.geom.ny = p.ny;
+    a.geom.g0 = a.geom.g1 = p.g;
+    a.geom.sy = p.sy;
+    a.geom.sk = p.sk;
+    a.refine_tag_val = refine_tag_val;
+    a.ep0 = sensor_eps(tag_tolerance[0]);
+    a.ep1 = sensor_eps(tag_tolerance[1]);
+    if ((e = launch_tag_cells(a, s)) != hipSuccess) break;
+    off += (long)p.nx * p.ny;
+  }
+  if (e == hipSuccess && !dev) e = hipMemcpyAsync(tags, out, sizeof(int) * (size_t)total, hipMemcpyDeviceToHost, s);
+  const hipError_t e2 = dev ? hipSuccess : hipStreamSynchronize(s);
+  if (!dev) (void)hipFree(out);
+  if (e != hipSuccess) return check_hip(e, "enl_batch_tag_cells");
+  return check_hip(e2, "enl_batch_tag_cells sync");
+}
+
 int enl_batch_destroy(void* handle) {
   Batch* b = (Batch*)handle;
   if (!b) return 0;

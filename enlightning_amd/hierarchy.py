@@ -218,24 +218,20 @@ class DeviceHierarchy:
     # ---- regrid -------------------------------------------------------------
     def level0_tags(self, refine_tag_val=1):
         """Tags of the current state on the base grid, (n0, n0) [j, i]: the
-        state's ghosts are filled, then tag_cells_ runs on every level-0
-        patch's device data (enlightning::tagGradientDetectorCells,
-        enlightning.C:586-628), the input of the next regrid's clustering."""
-        _lib.check(self.lib.enl_batch_fill(self.handle, self._reflect()))
+        state's ghosts are filled, then tag_cells runs on every level-0
+        patch's device data in one call (enl_batch_tag_cells;
+        enlightning::tagGradientDetectorCells, enlightning.C:586-628), the
+        input of the next regrid's clustering."""
+        pats = self.h.level_patches(0)
+        flat = np.zeros(sum(p.nx * p.ny for p in pats), np.int32)
+        tol = (ctypes.c_double * len(self.gas.tag_tolerance))(*self.gas.tag_tolerance)
+        _lib.check(self.lib.enl_batch_tag_cells(self.handle, 0, self._reflect(), int(refine_tag_val), tol, _p(flat)))
         n0 = self.h.ncells(0)
         tags = np.zeros((n0, n0), np.int32)
-        ci = ctypes.c_int
-        tol = (ctypes.c_double * len(self.gas.tag_tolerance))(*self.gas.tag_tolerance)
-        for p in self.h.level_patches(0):
-            w, src = ctypes.c_void_p(), ctypes.c_void_p()
-            _lib.check(self.lib.enl_batch_patch_state(self.handle, p.pid, ctypes.byref(w), None, None,
-                                                      ctypes.byref(src)))
-            t = np.zeros((p.ny, p.nx), np.int32)
-            self.lib.tag_cells_(ctypes.byref(ci(p.box[0])), ctypes.byref(ci(p.box[1])), ctypes.byref(ci(p.box[2])),
-                                ctypes.byref(ci(p.box[3])), ctypes.byref(ci(p.g)), ctypes.byref(ci(p.g)), _p(t), w,
-                                src, ctypes.byref(ci(refine_tag_val)), tol, ctypes.byref(ci(5)))
-            _lib.check()
-            tags[p.box[2]:p.box[3] + 1, p.box[0]:p.box[1] + 1] = t
+        off = 0
+        for p in pats:
+            tags[p.box[2]:p.box[3] + 1, p.box[0]:p.box[1] + 1] = flat[off:off + p.nx * p.ny].reshape(p.ny, p.nx)
+            off += p.nx * p.ny
         return tags
 
     def regrid(self, new_h, transfer=None):

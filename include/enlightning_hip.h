@@ -308,6 +308,13 @@ int enl_batch_transfer(void *new_handle, void *old_handle, const int *reflect, i
                        const int *s_old, int ncopy, const int *c_dp, const int *c_dl, const int *c_sp,
                        const int *c_sl, int ninterp, const int *i_dp, const int *i_dl, const int *i_sp,
                        const int *i_sl, const double *i_fx, const double *i_fy);
+/* tag_cells of every patch of one level (enlightning::tagGradientDetectorCells,
+ * enlightning.C:586-628, the tagging step of regridAllFinerLevels) after a
+ * ghost fill of the current state: tags receives, patch after patch in pid
+ * order, each patch's nx*ny interior tags row by row (host or device
+ * memory; host memory is filled before the call returns). */
+int enl_batch_tag_cells(void *handle, int level, const int *reflect, int refine_tag_val,
+                        const double *tag_tolerance, int *tags);
 int enl_batch_destroy(void *handle);
 
 /* ---- FP32-perturbation form of the batched hierarchy (no reference
