@@ -59,21 +59,44 @@ struct Scheme {
 
 __host__ __device__ inline double sq(double x) { return x * x; }
 
+// The WENO combination sum_k (zeta_k / sigma) f_k with zeta_k = c_k / D_k
+// and sigma = sum_k zeta_k, evaluated with one division: with
+// n_k = c_k * prod_{j != k} D_j the weights are n_k / sum_j n_j, so the
+// combination is (sum_k n_k f_k) / (sum_k n_k).  Same value as the
+// reference's expressions up to a few ulps; where the products leave the
+// finite range (D beyond ~1e100, or non-finite data) the reference's
+// expressions are evaluated instead.
+__device__ inline double weno_combine(double c0, double c1, double c2, double D0, double D1, double D2, double f0,
+                                      double f1, double f2) {
+#pragma clang fp contract(fast)
+  const double n0 = c0 * (D1 * D2), n1 = c1 * (D0 * D2), n2 = c2 * (D0 * D1);
+  const double r = (n0 * f0 + n1 * f1 + n2 * f2) / (n0 + n1 + n2);
+  if (__builtin_isfinite(r)) return r;
+  const double z0 = c0 / D0, z1 = c1 / D1, z2 = c2 / D2;
+  const double sigma = z0 + z1 + z2;
+  return (z0 / sigma) * f0 + (z1 / sigma) * f1 + (z2 / sigma) * f2;
+}
+__device__ inline double weno_combine(double c0, double c1, double D0, double D1, double f0, double f1) {
+#pragma clang fp contract(fast)
+  const double n0 = c0 * D1, n1 = c1 * D0;
+  const double r = (n0 * f0 + n1 * f1) / (n0 + n1);
+  if (__builtin_isfinite(r)) return r;
+  const double z0 = c0 / D0, z1 = c1 / D1;
+  const double sigma = z0 + z1;
+  return (z0 / sigma) * f0 + (z1 / sigma) * f1;
+}
+
 // fortran/rhs_hybrid_high_llf.m4:365-398
 __device__ inline double wenom5(double fgm2, double fgm1, double fg0, double fgp1, double fgp2) {
 #pragma clang fp contract(fast)
   double beta0 = kC13_12 * sq(fg0 - 2 * fgp1 + fgp2) + 0.25 * sq(3 * fg0 - 4 * fgp1 + fgp2);
   double beta1 = kC13_12 * sq(fgm1 - 2 * fg0 + fgp1) + 0.25 * sq(fgm1 - fgp1);
   double beta2 = kC13_12 * sq(fgm2 - 2 * fgm1 + fg0) + 0.25 * sq(fgm2 - 4 * fgm1 + 3 * fg0);
-  double zeta0 = kW01 / sq(kEpsWeno + beta0);
-  double zeta1 = kW06 / sq(kEpsWeno + beta1);
-  double zeta2 = kW03 / sq(kEpsWeno + beta2);
-  double sigma = zeta0 + zeta1 + zeta2;
-  double omega0 = zeta0 / sigma, omega1 = zeta1 / sigma, omega2 = zeta2 / sigma;
   double f0 = kC1_6 * (11 * fg0 - 7 * fgp1 + 2 * fgp2);
   double f1 = kC1_6 * (2 * fgm1 + 5 * fg0 - fgp1);
   double f2 = kC1_6 * (-fgm2 + 5 * fgm1 + 2 * fg0);
-  return omega0 * f0 + omega1 * f1 + omega2 * f2;
+  return weno_combine(kW01, kW06, kW03, sq(kEpsWeno + beta0), sq(kEpsWeno + beta1), sq(kEpsWeno + beta2), f0, f1,
+                      f2);
 }
 
 // fortran/rhs_hybrid_high_llf.m4:400-433
@@ -82,15 +105,11 @@ __device__ inline double wenop5(double fgm2, double fgm1, double fg0, double fgp
   double beta0 = kC13_12 * sq(fg0 - 2 * fgp1 + fgp2) + 0.25 * sq(3 * fg0 - 4 * fgp1 + fgp2);
   double beta1 = kC13_12 * sq(fgm1 - 2 * fg0 + fgp1) + 0.25 * sq(fgm1 - fgp1);
   double beta2 = kC13_12 * sq(fgm2 - 2 * fgm1 + fg0) + 0.25 * sq(fgm2 - 4 * fgm1 + 3 * fg0);
-  double zeta0 = kW03 / sq(kEpsWeno + beta0);
-  double zeta1 = kW06 / sq(kEpsWeno + beta1);
-  double zeta2 = kW01 / sq(kEpsWeno + beta2);
-  double sigma = zeta0 + zeta1 + zeta2;
-  double omega0 = zeta0 / sigma, omega1 = zeta1 / sigma, omega2 = zeta2 / sigma;
   double f0 = kC1_6 * (2 * fg0 + 5 * fgp1 - fgp2);
   double f1 = kC1_6 * (-fgm1 + 5 * fg0 + 2 * fgp1);
   double f2 = kC1_6 * (2 * fgm2 - 7 * fgm1 + 11 * fg0);
-  return omega0 * f0 + omega1 * f1 + omega2 * f2;
+  return weno_combine(kW03, kW06, kW01, sq(kEpsWeno + beta0), sq(kEpsWeno + beta1), sq(kEpsWeno + beta2), f0, f1,
+                      f2);
 }
 
 // fortran/rhs_hybrid_low_llf.m4:349-373
@@ -98,13 +117,9 @@ __device__ inline double wenom3(double fgm1, double fg0, double fgp1) {
 #pragma clang fp contract(fast)
   double beta0 = sq(fgp1 - fg0);
   double beta1 = sq(fg0 - fgm1);
-  double zeta0 = kC1_3 / sq(kEpsWeno + beta0);
-  double zeta1 = kC2_3 / sq(kEpsWeno + beta1);
-  double sigma = zeta0 + zeta1;
-  double omega0 = zeta0 / sigma, omega1 = zeta1 / sigma;
   double f0 = 0.5 * (3 * fg0 - fgp1);
   double f1 = 0.5 * (fgm1 + fg0);
-  return omega0 * f0 + omega1 * f1;
+  return weno_combine(kC1_3, kC2_3, sq(kEpsWeno + beta0), sq(kEpsWeno + beta1), f0, f1);
 }
 
 // fortran/rhs_hybrid_low_llf.m4:375-399
@@ -112,13 +127,9 @@ __device__ inline double wenop3(double fgm1, double fg0, double fgp1) {
 #pragma clang fp contract(fast)
   double beta0 = sq(fgp1 - fg0);
   double beta1 = sq(fg0 - fgm1);
-  double zeta0 = kC2_3 / sq(kEpsWeno + beta0);
-  double zeta1 = kC1_3 / sq(kEpsWeno + beta1);
-  double sigma = zeta0 + zeta1;
-  double omega0 = zeta0 / sigma, omega1 = zeta1 / sigma;
   double f0 = 0.5 * (fg0 + fgp1);
   double f1 = 0.5 * (-fgm1 + 3 * fg0);
-  return omega0 * f0 + omega1 * f1;
+  return weno_combine(kC2_3, kC1_3, sq(kEpsWeno + beta0), sq(kEpsWeno + beta1), f0, f1);
 }
 
 // Smoothness ratio of the hybrid sensor and of tag_cells
