@@ -1044,9 +1044,9 @@ int enl_batch_transfer(void* new_handle, void* old_handle, const int* reflect, i
   for (int q = 0; q < nsame; q++) {
     if (s_new[q] < 0 || s_new[q] >= nb->npatch || s_old[q] < 0 || s_old[q] >= ob->npatch)
       return set_error(-2, "enl_batch_transfer: patch pair %d names a bad patch", q);
-    const PatchDev &D = nb->hp[s_new[q]], &S = ob->hp[s_old[q]];
-    if (D.level != S.level || D.nx != S.nx || D.ny != S.ny)
-      return set_error(-2, "enl_batch_transfer: patch pair %d differs in level or shape", q);
+    const enl_patch_desc_t &D = nb->desc[s_new[q]], &S = ob->desc[s_old[q]];
+    if (D.level != S.level || memcmp(D.box, S.box, sizeof D.box) != 0)
+      return set_error(-2, "enl_batch_transfer: patch pair %d differs in level or box", q);
   }
   // every item must address cells of its patches' ghosted boxes, with the
   // refinement stencil (+-1 cell, +-1 row) inside the coarse patch's box

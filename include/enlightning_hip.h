@@ -302,7 +302,7 @@ int enl_batch_set_absorb(void *handle, const double *grid_xlo, const double *gri
  * written); afterwards the new hierarchy's source and state ghosts are
  * filled, so it is ready for enl_batch_advance.  Items that address cells
  * outside their patches' ghosted boxes (or refinement stencils that leave
- * the coarse patch) and pairs of different shape are rejected before
+ * the coarse patch) and pairs whose level or box differ are rejected before
  * anything runs. */
 int enl_batch_transfer(void *new_handle, void *old_handle, const int *reflect, int nsame, const int *s_new,
                        const int *s_old, int ncopy, const int *c_dp, const int *c_dl, const int *c_sp,

@@ -89,15 +89,18 @@ def test_transfer_rejects_out_of_range_items():
         pytest.skip("the range check is the C ABI's")
     with pytest.raises(RuntimeError, match="out of range"):
         dev.regrid(new, bad)
-    # a whole-patch pair of different shapes
-    sn, so = (a.copy() for a in tr["same"])
+    # whole-patch pairs whose boxes differ: another shape, and the same
+    # shape at another place
     fine_new = [p.pid for p in new.level_patches(1)]
-    shape = (new.patches[fine_new[0]].nx, new.patches[fine_new[0]].ny)
-    fine_old = [p.pid for p in old.level_patches(1) if (p.nx, p.ny) != shape]
-    sn[0], so[0] = fine_new[0], fine_old[0]
-    bad = {"same": (sn, so), "copy": tr["copy"], "interp": tr["interp"]}
-    with pytest.raises(RuntimeError, match="differs in level or shape"):
-        dev.regrid(new, bad)
+    A = new.patches[fine_new[0]]
+    other_shape = [p.pid for p in old.level_patches(1) if (p.nx, p.ny) != (A.nx, A.ny)]
+    same_shape = [p.pid for p in old.level_patches(1) if (p.nx, p.ny) == (A.nx, A.ny) and p.box != A.box]
+    for q in other_shape[:1] + same_shape[:1]:
+        sn, so = (a.copy() for a in tr["same"])
+        sn[0], so[0] = A.pid, q
+        bad = {"same": (sn, so), "copy": tr["copy"], "interp": tr["interp"]}
+        with pytest.raises(RuntimeError, match="differs in level or box"):
+            dev.regrid(new, bad)
     dev.close()
 
 
