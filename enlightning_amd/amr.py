@@ -104,20 +104,18 @@ def base_boxes(n0, tile0):
 
 
 def buffer_tags(tags, nbuf):
-    """Grow the tagged set by nbuf cells (a square neighbourhood)."""
-    t = tags.astype(bool)
-    out = t.copy()
-    for dj in range(-nbuf, nbuf + 1):
-        for di in range(-nbuf, nbuf + 1):
-            if di == 0 and dj == 0:
-                continue
-            sh = np.zeros_like(t)
-            ys = slice(max(dj, 0), t.shape[0] + min(dj, 0))
-            yd = slice(max(-dj, 0), t.shape[0] + min(-dj, 0))
-            xs = slice(max(di, 0), t.shape[1] + min(di, 0))
-            xd = slice(max(-di, 0), t.shape[1] + min(-di, 0))
-            sh[yd, xd] = t[ys, xs]
-            out |= sh
+    """Grow the tagged set by nbuf cells (a square neighbourhood: a dilation
+    along rows, then along columns)."""
+    out = tags.astype(bool)
+    for axis in (1, 0):
+        src = out.copy()
+        for d in range(1, nbuf + 1):
+            if axis == 1:
+                out[:, d:] |= src[:, :-d]
+                out[:, :-d] |= src[:, d:]
+            else:
+                out[d:, :] |= src[:-d, :]
+                out[:-d, :] |= src[d:, :]
     return out
 
 
@@ -126,38 +124,44 @@ def cluster_fine_boxes(tags, ratio, fine_max=64, fine_min=8):
     n0 = tags.shape[0]
     blk = fine_max // ratio
     cmin = max(1, fine_min // ratio)
+    # per block of blk x blk coarse cells the bounding box of its tags,
+    # all blocks at once: t[bj, jj, bi, ii]
+    nb = -(-n0 // blk)
+    t = np.zeros((nb * blk, nb * blk), bool)
+    t[:n0, :n0] = tags != 0
+    t = t.reshape(nb, blk, nb, blk)
+    cols = t.any(axis=1)  # [bj, bi, ii]
+    rows = t.any(axis=3)  # [bj, jj, bi]
+    ilo, ihi = cols.argmax(axis=2), blk - 1 - cols[:, :, ::-1].argmax(axis=2)
+    jlo, jhi = rows.argmax(axis=1), blk - 1 - rows[:, ::-1, :].argmax(axis=1)
     boxes = []
-    for bj in range(0, n0, blk):
-        for bi in range(0, n0, blk):
-            sub = tags[bj:bj + blk, bi:bi + blk]
-            if not sub.any():
-                continue
-            jj, ii = np.nonzero(sub)
-            i0, i1 = bi + ii.min(), bi + ii.max()
-            j0, j1 = bj + jj.min(), bj + jj.max()
-            # at least cmin coarse cells a side, kept inside the block
-            bi1, bj1 = min(bi + blk, n0) - 1, min(bj + blk, n0) - 1
-            while i1 - i0 + 1 < cmin:
-                if i1 < bi1:
-                    i1 += 1
-                else:
-                    i0 -= 1
-            while j1 - j0 + 1 < cmin:
-                if j1 < bj1:
-                    j1 += 1
-                else:
-                    j0 -= 1
-            # touch the domain edge when within one coarse cell of it (the
-            # block edge coincides with the domain edge there)
-            if i0 == 1:
-                i0 = 0
-            if j0 == 1:
-                j0 = 0
-            if i1 == n0 - 2:
-                i1 = n0 - 1
-            if j1 == n0 - 2:
-                j1 = n0 - 1
-            boxes.append((i0 * ratio, (i1 + 1) * ratio - 1, j0 * ratio, (j1 + 1) * ratio - 1))
+    for kj, ki in zip(*np.nonzero(cols.any(axis=2))):
+        bj, bi = int(kj) * blk, int(ki) * blk
+        i0, i1 = bi + int(ilo[kj, ki]), bi + int(ihi[kj, ki])
+        j0, j1 = bj + int(jlo[kj, ki]), bj + int(jhi[kj, ki])
+        # at least cmin coarse cells a side, kept inside the block
+        bi1, bj1 = min(bi + blk, n0) - 1, min(bj + blk, n0) - 1
+        while i1 - i0 + 1 < cmin:
+            if i1 < bi1:
+                i1 += 1
+            else:
+                i0 -= 1
+        while j1 - j0 + 1 < cmin:
+            if j1 < bj1:
+                j1 += 1
+            else:
+                j0 -= 1
+        # touch the domain edge when within one coarse cell of it (the
+        # block edge coincides with the domain edge there)
+        if i0 == 1:
+            i0 = 0
+        if j0 == 1:
+            j0 = 0
+        if i1 == n0 - 2:
+            i1 = n0 - 1
+        if j1 == n0 - 2:
+            j1 = n0 - 1
+        boxes.append((i0 * ratio, (i1 + 1) * ratio - 1, j0 * ratio, (j1 + 1) * ratio - 1))
     return boxes
 
 
