@@ -166,17 +166,28 @@ def cluster_fine_boxes(tags, ratio, fine_max=64, fine_min=8):
 
 
 def _owner(h, level):
-    """The level's owner map, kept on the hierarchy."""
+    """The level's owner map, kept on the hierarchy: (map, s), the patch
+    holding cell (i, j) (or -1) at map[j // s, i // s].  s is the refinement
+    ratio when every box of a level above 0 is aligned to it (the clustered
+    boxes are), which keeps the map at the coarser level's size, else 1."""
     if level not in h.owners:
-        h.owners[level] = _owner_map(h.level_patches(level), h.ncells(level))
+        pats = h.level_patches(level)
+        n, r = h.ncells(level), h.ratio
+        aligned = level > 0 and n % r == 0 and all(
+            p.box[0] % r == 0 and (p.box[1] + 1) % r == 0 and p.box[2] % r == 0 and (p.box[3] + 1) % r == 0
+            for p in pats)
+        s = r if aligned else 1
+        own = np.full((n // s, n // s), -1, np.int32)
+        for p in pats:
+            own[p.box[2] // s:(p.box[3] + 1) // s, p.box[0] // s:(p.box[1] + 1) // s] = p.pid
+        h.owners[level] = (own, s)
     return h.owners[level]
 
 
-def _owner_map(patches, n):
-    own = np.full((n, n), -1, np.int32)
-    for p in patches:
-        own[p.box[2]:p.box[3] + 1, p.box[0]:p.box[1] + 1] = p.pid
-    return own
+def _owner_at(h, level, i, j):
+    """Owners of the cells (i, j) of a level (index arrays inside it)."""
+    own, s = _owner(h, level)
+    return own[j, i] if s == 1 else own[j // s, i // s]
 
 
 def build(n0, ratio, g, L, coarse_tags, tile0=64, tag_buffer=2, fine_max=64, fine_min=8, previous=None):
@@ -248,7 +259,6 @@ def make_fill(h, level):
     n = h.ncells(level)
     g = h.g
     pats = h.level_patches(level)
-    own = _owner(h, level)
     # per global pid: box origin and ghosted row length
     B0 = np.array([p.box[0] for p in h.patches], np.int64)
     B2 = np.array([p.box[2] for p in h.patches], np.int64)
@@ -260,7 +270,7 @@ def make_fill(h, level):
     K, I, J = K[inside], I[inside], J[inside]
     dpid = pid[K]
     dst = _locals(B0[dpid], B2[dpid], NX[dpid], g, I, J)
-    owner = own[J, I].astype(np.int64)
+    owner = _owner_at(h, level, I, J).astype(np.int64)
     have = owner >= 0
     o = owner[have]
     copy = (dpid[have].astype(np.int32), dst[have], o.astype(np.int32),
@@ -268,9 +278,8 @@ def make_fill(h, level):
     ui, uj, up, ud = I[~have], J[~have], dpid[~have], dst[~have]
     if ui.size:
         assert level > 0, "a level-0 ghost cell inside the domain has no owner"
-        cown = _owner(h, level - 1)
         ic, jc = ui // r, uj // r
-        c = cown[jc, ic].astype(np.int64)
+        c = _owner_at(h, level - 1, ic, jc).astype(np.int64)
         assert (c >= 0).all()
         interp = (up.astype(np.int32), ud, c.astype(np.int32), _locals(B0[c], B2[c], NX[c], g, ic, jc),
                   ((ui - ic * r) + 0.5) / r - 0.5, ((uj - jc * r) + 0.5) / r - 0.5)
@@ -363,7 +372,6 @@ def make_coarsen(h):
     """(coarse patch, coarse local, fine patch, fine local of the lower-left
     fine cell) for every coarse cell under a fine patch."""
     r = h.ratio
-    cown = _owner(h, 0)
     B0, B2, NX = _patch_arrays(h)
     coarse_boxes = []
     fine = h.level_patches(1)
@@ -372,7 +380,7 @@ def make_coarsen(h):
         coarse_boxes.append(Patch(f.pid, 0, (i0 // r, (i1 + 1) // r - 1, j0 // r, (j1 + 1) // r - 1), f.g))
     K, IC, JC = _interior_cells(coarse_boxes)
     fp = np.array([f.pid for f in fine], np.int64)[K]
-    cp = cown[JC, IC].astype(np.int64)
+    cp = _owner_at(h, 0, IC, JC).astype(np.int64)
     g = h.g
     return (cp.astype(np.int32), _locals(B0[cp], B2[cp], NX[cp], g, IC, JC), fp.astype(np.int32),
             _locals(B0[fp], B2[fp], NX[fp], g, IC * r, JC * r))
@@ -410,7 +418,7 @@ def make_transfer(new, old):
         dp = np.array([p.pid for p in pats], np.int64)[K]
         dl = _locals(nB0[dp], nB2[dp], nNX[dp], g, I, J)
         if lev < old.nlevels:
-            owner = _owner(old, lev)[J, I].astype(np.int64)
+            owner = _owner_at(old, lev, I, J).astype(np.int64)
         else:
             owner = np.full(I.size, -1, np.int64)
         have = owner >= 0
@@ -422,7 +430,7 @@ def make_transfer(new, old):
                 raise ValueError("make_transfer: new level-%d cells have no old source" % lev)
             ui, uj = I[~have], J[~have]
             ic, jc = ui // r, uj // r
-            c = _owner(old, lev - 1)[jc, ic].astype(np.int64)
+            c = _owner_at(old, lev - 1, ic, jc).astype(np.int64)
             if (c < 0).any():
                 raise ValueError("make_transfer: a new fine cell lies outside the old coarser level")
             for lst, a in zip(interp, (dp[~have], dl[~have], c, _locals(oB0[c], oB2[c], oNX[c], g, ic, jc),
