@@ -7,6 +7,8 @@ regrid; main.C:489-502).
   unfused linear refinement are the same arithmetic.
 * level0_tags (tag_cells_ on the device data of every level-0 patch) equals
   the oracle's tag_cells on the same filled states.
+* enl_batch_tag_cells into device memory equals the host path; a transfer
+  onto the same boxes (whole-patch pairs only) is the identity.
 * Regrid inside a run: two RK3 steps, tags of the evolved state (plus the
   same tags moved, so the boxes change), regrid, two more steps; the device
   follows its CPU twin to 1e-10 per field, as without the regrid (with a
@@ -154,4 +156,43 @@ def test_regrid_inside_a_run_follows_cpu_twin():
             assert np.abs(a - b).max() <= max(1e-10 * np.abs(b).max(), 16 * ulp * natural[k]), (p.pid, k)
         assert np.abs(pd[g:-g, g:-g] - pc[g:-g, g:-g]).max() <= 1e-10 * np.abs(pc[g:-g, g:-g] - gas.p0).max() + 1e-9
     dnew.close()
+    dev.close()
+
+
+def test_tags_into_device_memory_and_identity_transfer():
+    """enl_batch_tag_cells into a device buffer equals the host path; a
+    transfer onto the same boxes (whole-patch pairs only) is the identity."""
+    if D.DRY:
+        pytest.skip("device buffers and the C ABI's pair path")
+    import ctypes
+
+    import torch
+
+    from enlightning_amd import _lib
+    gas, old, _ = hierarchies()
+    dev = D.Hierarchy(old, gas)
+    dev.load_initial(WL64, sources=random_sources(old, 11))
+    dev.advance(2, 0.5 * old.dx(1) / gas.c, mode="graph")
+    host_tags = dev.level0_tags()
+    pats = old.level_patches(0)
+    buf = torch.zeros(sum(p.nx * p.ny for p in pats), dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()  # the fill runs on torch's stream, the tags on the library's
+    tol = (ctypes.c_double * 3)(*gas.tag_tolerance)
+    refl = (ctypes.c_int * 4)(*dev.reflect)
+    _lib.check(dev.lib.enl_batch_tag_cells(dev.handle, 0, refl, 1, tol, ctypes.c_void_p(buf.data_ptr())))
+    _lib.check(dev.lib.enl_sync())
+    flat = buf.cpu().numpy()
+    off = 0
+    for p in pats:
+        t = flat[off:off + p.nx * p.ny].reshape(p.ny, p.nx)
+        assert np.array_equal(t, host_tags[p.box[2]:p.box[3] + 1, p.box[0]:p.box[1] + 1]), p.pid
+        off += p.nx * p.ny
+    tr = amr.make_transfer(old, old)
+    assert len(tr["copy"][0]) == 0 and len(tr["interp"][0]) == 0
+    same = dev.regrid(old, tr)
+    for (wa, pa, Ta), (wb, pb, Tb), sa, sb, p in zip(dev.states(), same.states(), dev.sources(), same.sources(),
+                                                      old.patches):
+        assert np.array_equal(wa, wb) and np.array_equal(pa, pb) and np.array_equal(Ta, Tb), p.pid
+        assert np.array_equal(sa, sb), p.pid
+    same.close()
     dev.close()
