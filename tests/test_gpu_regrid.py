@@ -12,7 +12,8 @@ regrid; main.C:489-502).
 * Regrid inside a run: two RK3 steps, tags of the evolved state (plus the
   same tags moved, so the boxes change), regrid, two more steps; the device
   follows its CPU twin to 1e-10 per field, as without the regrid (with a
-  floor of 16 ulps of the field's natural scale on quiet patches).
+  floor of 16 ulps of the field's natural scale on quiet patches); and at
+  the reference's cadence, a regrid every two steps for three cycles.
 """
 import numpy as np
 import pytest
@@ -195,4 +196,46 @@ def test_tags_into_device_memory_and_identity_transfer():
         assert np.array_equal(wa, wb) and np.array_equal(pa, pb) and np.array_equal(Ta, Tb), p.pid
         assert np.array_equal(sa, sb), p.pid
     same.close()
+    dev.close()
+
+
+def test_regrid_every_two_steps_follows_cpu_twin():
+    """The reference's cadence (regrid_step = 2, input.txt:7): three cycles
+    of two RK3 steps and a regrid on the evolved state's device tags (moved
+    by a few cells as well, so that boxes change); the device follows its
+    CPU twin to 1e-10 per field after every cycle."""
+    gas, h, _ = hierarchies()
+    dt = 0.5 * h.dx(1) / gas.c
+    states = [patch_initial_state(h, p, WL64, gas) for p in h.patches]
+    cpu = HostHierarchy(h, gas)
+    cpu.load_initial(states)
+    dev = D.Hierarchy(h, gas)
+    dev.load_initial(WL64, states=states)
+    natural = (gas.rho0, gas.rho0 * gas.c, gas.rho0 * gas.c, gas.rho0 * gas.R, gas.rho0 * gas.T0)
+    ulp = np.finfo(np.float64).eps
+    for cycle in range(3):
+        cpu.advance(2, dt, RK3)
+        dev.advance(2, dt, mode="graph")
+        # the evolved state's tags plus the same moved by a few cells, so
+        # that boxes change at every cycle
+        tags = dev.level0_tags()
+        tags = tags | np.roll(tags, (2 * cycle + 1, -cycle - 1), axis=(0, 1))
+        new = amr.build(WL64.n, 4, 4, 1.0, tags, tile0=16, previous=h)
+        tr = amr.make_transfer(new, h)
+        assert len(tr["copy"][0]) + len(tr["interp"][0]) > 0
+        print("cycle %d: %d fine patches, %d pairs, %d copies, %d refinements" % (
+            cycle, len(new.level_patches(1)), len(tr["same"][0]), len(tr["copy"][0]), len(tr["interp"][0])),
+            flush=True)
+        dnew = dev.regrid(new, tr)
+        cnew = HostHierarchy(new, gas)
+        cnew.transfer_from(cpu, tr)
+        dev.close()
+        dev, cpu, h = dnew, cnew, new
+        for (wc, pc, _), (wd, pd, _), p in zip(cpu.states(), dev.states(), h.patches):
+            g = p.g
+            for k in range(5):
+                a, b = wd[k, g:-g, g:-g], wc[k, g:-g, g:-g]
+                assert np.abs(a - b).max() <= max(1e-10 * np.abs(b).max(), 16 * ulp * natural[k]), (cycle, p.pid, k)
+            assert np.abs(pd[g:-g, g:-g] - pc[g:-g, g:-g]).max() <= \
+                1e-10 * np.abs(pc[g:-g, g:-g] - gas.p0).max() + 1e-9
     dev.close()
