@@ -241,7 +241,8 @@ class DeviceHierarchy:
         patches of each level, linear refinement from the old coarser level
         elsewhere).  This hierarchy
         keeps its interiors and stays usable; absorbing envelopes carry
-        over."""
+        over.  The FP64 state moves: after FP32-form steps call fp32_store()
+        first."""
         tr = amr.make_transfer(new_h, self.h) if transfer is None else transfer
         new = DeviceHierarchy(new_h, self.gas, self.reflect)
         try:

@@ -300,7 +300,9 @@ int enl_batch_set_absorb(void *handle, const double *grid_xlo, const double *gri
  *            the ghost fill's interp).
  * The old hierarchy's ghosts are refilled first (its interiors are not
  * written); afterwards the new hierarchy's source and state ghosts are
- * filled, so it is ready for enl_batch_advance.  Items that address cells
+ * filled, so it is ready for enl_batch_advance.  The FP64 state is what
+ * moves: after steps in the FP32-perturbation modes call
+ * enl_batch_fp32_store on the old batch first.  Items that address cells
  * outside their patches' ghosted boxes (or refinement stencils that leave
  * the coarse patch) and pairs whose level or box differ are rejected before
  * anything runs. */
