@@ -71,6 +71,7 @@ class DeviceHierarchy:
     def __init__(self, h, gas, reflect=(1, 1, 1, 1)):
         self.h, self.gas, self.reflect = h, gas, tuple(reflect)
         self.absorb_width = 0.0
+        self.tiles = {}  # level -> (bx, by) set by set_tiles / autotune; None: every level
         self.lib = _lib.load()
         n = len(h.patches)
         desc = (_lib.EnlPatchDesc * n)()
@@ -117,8 +118,10 @@ class DeviceHierarchy:
     def set_tiles(self, bx, by, level=None):
         if level is None:
             _lib.check(self.lib.enl_batch_set_tiles(self.handle, bx, by))
+            self.tiles = {None: (bx, by)}
         else:
             _lib.check(self.lib.enl_batch_set_level_tiles(self.handle, level, bx, by))
+            self.tiles[level] = (bx, by)
 
     def autotune(self, dt, states, wl, steps=2, variant="hybrid_high_llf", rk=RK3, mode="graph"):
         """Tile shape per level by coordinate descent over measured steps of
@@ -240,9 +243,9 @@ class DeviceHierarchy:
         amr.make_transfer: unchanged patches whole, copies from the old
         patches of each level, linear refinement from the old coarser level
         elsewhere).  This hierarchy
-        keeps its interiors and stays usable; absorbing envelopes carry
-        over.  The FP64 state moves: after FP32-form steps call fp32_store()
-        first."""
+        keeps its interiors and stays usable; absorbing envelopes and tile
+        shapes carry over.  The FP64 state moves: after FP32-form steps call
+        fp32_store() first."""
         tr = amr.make_transfer(new_h, self.h) if transfer is None else transfer
         new = DeviceHierarchy(new_h, self.gas, self.reflect)
         try:
@@ -253,6 +256,11 @@ class DeviceHierarchy:
                                                    len(c[0]), *P[2:6], len(it[0]), *P[6:12]))
             if self.absorb_width > 0:
                 new.set_absorb(self.absorb_width)
+            # the tile shapes chosen for this hierarchy (results do not
+            # depend on them)
+            for lev, (bx, by) in sorted(self.tiles.items(), key=lambda kv: -1 if kv[0] is None else kv[0]):
+                if lev is None or lev < new_h.nlevels:
+                    new.set_tiles(bx, by, level=lev)
         except Exception:
             new.close()
             raise

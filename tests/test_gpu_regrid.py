@@ -211,6 +211,7 @@ def test_regrid_every_two_steps_follows_cpu_twin():
     cpu.load_initial(states)
     dev = D.Hierarchy(h, gas)
     dev.load_initial(WL64, states=states)
+    dev.set_tiles(16, 8, level=1)  # carried over by every regrid
     natural = (gas.rho0, gas.rho0 * gas.c, gas.rho0 * gas.c, gas.rho0 * gas.R, gas.rho0 * gas.T0)
     ulp = np.finfo(np.float64).eps
     for cycle in range(3):
@@ -231,6 +232,8 @@ def test_regrid_every_two_steps_follows_cpu_twin():
         cnew.transfer_from(cpu, tr)
         dev.close()
         dev, cpu, h = dnew, cnew, new
+        if not D.DRY:
+            assert dev.tiles == {1: (16, 8)}
         for (wc, pc, _), (wd, pd, _), p in zip(cpu.states(), dev.states(), h.patches):
             g = p.g
             for k in range(5):
