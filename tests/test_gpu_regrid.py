@@ -200,16 +200,20 @@ def test_tags_into_device_memory_and_identity_transfer():
 
 
 def test_regrid_every_two_steps_follows_cpu_twin():
-    """The reference's cadence (regrid_step = 2, input.txt:7): three cycles
-    of two RK3 steps and a regrid on the evolved state's device tags (moved
-    by a few cells as well, so that boxes change); the device follows its
-    CPU twin to 1e-10 per field after every cycle."""
+    """The reference's cadence (regrid_step = 2, input.txt:7) with its
+    absorbing sides: three cycles of two RK3 steps and a regrid on the
+    evolved state's device tags (moved by a few cells as well, so that boxes
+    change); the device follows its CPU twin to 1e-10 per field after every
+    cycle, and the envelopes and tile shapes carry over."""
     gas, h, _ = hierarchies()
     dt = 0.5 * h.dx(1) / gas.c
     states = [patch_initial_state(h, p, WL64, gas) for p in h.patches]
-    cpu = HostHierarchy(h, gas)
+    reflect, width = (0, 0, 1, 0), 0.1  # input.txt:14-17: ABSORB left, right, top
+    cpu = HostHierarchy(h, gas, reflect)
+    cpu.absorb_width = width
     cpu.load_initial(states)
-    dev = D.Hierarchy(h, gas)
+    dev = D.Hierarchy(h, gas, reflect)
+    dev.set_absorb(width)  # carried over by every regrid
     dev.load_initial(WL64, states=states)
     dev.set_tiles(16, 8, level=1)  # carried over by every regrid
     natural = (gas.rho0, gas.rho0 * gas.c, gas.rho0 * gas.c, gas.rho0 * gas.R, gas.rho0 * gas.T0)
@@ -228,10 +232,12 @@ def test_regrid_every_two_steps_follows_cpu_twin():
             cycle, len(new.level_patches(1)), len(tr["same"][0]), len(tr["copy"][0]), len(tr["interp"][0])),
             flush=True)
         dnew = dev.regrid(new, tr)
-        cnew = HostHierarchy(new, gas)
+        cnew = HostHierarchy(new, gas, reflect)
+        cnew.absorb_width = width
         cnew.transfer_from(cpu, tr)
         dev.close()
         dev, cpu, h = dnew, cnew, new
+        assert dev.absorb_width == width
         if not D.DRY:
             assert dev.tiles == {1: (16, 8)}
         for (wc, pc, _), (wd, pd, _), p in zip(cpu.states(), dev.states(), h.patches):
