@@ -218,6 +218,24 @@ class DeviceHierarchy:
         _lib.check(self.lib.enl_batch_prepare(self.handle, refl))
         _lib.check(self.lib.enl_batch_fill(self.handle, refl))
 
+    def add_source(self, src, iteration, pulse_time0, dt):
+        """enlightning::addSource (enlightning.C:2140-2307) over the hierarchy:
+        enl_add_source writes every patch's source (src: a source.Source),
+        then enl_batch_prepare refills the source's ghosts and the interior
+        copy the drop-in mode reads.  Call it before each step, as the
+        reference does for time-dependent sources."""
+        st = src.to_struct()
+        D2 = ctypes.c_double * 2
+        for p in self.h.patches:
+            base = ctypes.c_void_p()
+            _lib.check(self.lib.enl_batch_patch_state(self.handle, p.pid, None, None, None, ctypes.byref(base)))
+            interior = ctypes.c_void_p(base.value + 8 * (p.g + p.g * p.Nx))
+            dx = self.h.dx(p.level)
+            _lib.check(self.lib.enl_add_source(p.box[0], p.box[1], p.box[2], p.box[3], D2(p.box[0] * dx, p.box[2] * dx),
+                                               D2(dx, dx), ctypes.byref(st), int(iteration), int(pulse_time0),
+                                               float(dt), interior, p.Nx))
+        _lib.check(self.lib.enl_batch_prepare(self.handle, self._reflect()))
+
     # ---- regrid -------------------------------------------------------------
     def level0_tags(self, refine_tag_val=1):
         """Tags of the current state on the base grid, (n0, n0) [j, i]: the

@@ -119,6 +119,14 @@ else:
         def set_absorb(self, width):
             self.absorb_width = width
 
+        def add_source(self, src, iteration, pulse_time0, dt):
+            for p in self.h.patches:
+                dx = self.h.dx(p.level)
+                tmp = np.zeros((p.ny, p.nx))
+                P.add_source(p.box, (p.box[0] * dx, p.box[2] * dx), (dx, dx), src, iteration, pulse_time0, dt, tmp)
+                self.src[p.pid][p.g:-p.g, p.g:-p.g] = tmp
+            self.fill_source()
+
         def level0_tags(self, refine_tag_val=1):
             self.fill_all()
             n0 = self.h.ncells(0)
