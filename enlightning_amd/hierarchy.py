@@ -236,6 +236,49 @@ class DeviceHierarchy:
                                                float(dt), interior, p.Nx))
         _lib.check(self.lib.enl_batch_prepare(self.handle, self._reflect()))
 
+    def sample_mics(self, mic_x, mic_y):
+        """recordAudio's pressure reads (enlightning.C:2392-2506) over the
+        hierarchy: for every level in order and every patch of it whose
+        bounds hold a microphone (inclusive, as there), the pressure of the
+        cell containing it (enl_sample_mics on the pool).  Returns, per
+        microphone, the (level, patch id, value) entries in the order the
+        reference appends them to its mic-*.txt file."""
+        import torch
+        mx_h = np.asarray(mic_x, dtype=np.float64)
+        my_h = np.asarray(mic_y, dtype=np.float64)
+        nm = len(mx_h)
+        D2 = ctypes.c_double * 2
+        todo = []
+        for lev in range(self.h.nlevels):
+            dx = self.h.dx(lev)
+            for p in self.h.level_patches(lev):
+                lo = (p.box[0] * dx, p.box[2] * dx)
+                hi = ((p.box[1] + 1) * dx, (p.box[3] + 1) * dx)
+                if np.any((mx_h >= lo[0]) & (mx_h <= hi[0]) & (my_h >= lo[1]) & (my_h <= hi[1])):
+                    todo.append((lev, p, lo, hi, dx))
+        res = [[] for _ in range(nm)]
+        if not todo or nm == 0:
+            return res
+        mx = torch.as_tensor(mx_h, device="cuda")
+        my = torch.as_tensor(my_h, device="cuda")
+        out = torch.zeros((len(todo), nm), dtype=torch.float64, device="cuda")
+        hit = torch.zeros((len(todo), nm), dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()  # the inputs are written on torch's stream
+        for t, (lev, p, lo, hi, dx) in enumerate(todo):
+            pr = ctypes.c_void_p()
+            _lib.check(self.lib.enl_batch_patch_state(self.handle, p.pid, None, ctypes.byref(pr), None, None))
+            interior = ctypes.c_void_p(pr.value + 8 * (p.g + p.g * p.Nx))
+            _lib.check(self.lib.enl_sample_mics(p.box[0], p.box[1], p.box[2], p.box[3], D2(*lo), D2(*hi), D2(dx, dx),
+                                                nm, ctypes.c_void_p(mx.data_ptr()), ctypes.c_void_p(my.data_ptr()),
+                                                interior, p.Nx, ctypes.c_void_p(out[t].data_ptr()),
+                                                ctypes.c_void_p(hit[t].data_ptr())))
+        _lib.check(self.lib.enl_sync())
+        out_h, hit_h = out.cpu().numpy(), hit.cpu().numpy()
+        for t, (lev, p, _, _, _) in enumerate(todo):
+            for k in np.nonzero(hit_h[t])[0]:
+                res[k].append((lev, p.pid, float(out_h[t, k])))
+        return res
+
     # ---- regrid -------------------------------------------------------------
     def level0_tags(self, refine_tag_val=1):
         """Tags of the current state on the base grid, (n0, n0) [j, i]: the

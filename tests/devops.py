@@ -127,6 +127,19 @@ else:
                 self.src[p.pid][p.g:-p.g, p.g:-p.g] = tmp
             self.fill_source()
 
+        def sample_mics(self, mic_x, mic_y):
+            res = [[] for _ in mic_x]
+            for lev in range(self.h.nlevels):
+                dx = self.h.dx(lev)
+                for p in self.h.level_patches(lev):
+                    lo = (p.box[0] * dx, p.box[2] * dx)
+                    hi = ((p.box[1] + 1) * dx, (p.box[3] + 1) * dx)
+                    pr = np.ascontiguousarray(self.view(p.pid)[1][p.g:-p.g, p.g:-p.g])
+                    v, hit = P.sample_mics(p.box, lo, hi, (dx, dx), mic_x, mic_y, pr)
+                    for k in np.nonzero(hit)[0]:
+                        res[k].append((lev, p.pid, float(v[k])))
+            return res
+
         def level0_tags(self, refine_tag_val=1):
             self.fill_all()
             n0 = self.h.ncells(0)

@@ -216,3 +216,35 @@ def test_time_dependent_source_batched():
     assert any(np.abs(s).max() > 0 for s in cpu.src)
     for d in devs.values():
         d.close()
+
+
+def test_microphones_over_the_hierarchy():
+    """recordAudio's reads over every level and patch (enlightning.C:2392-2506,
+    bounds inclusive): a microphone inside a fine patch is read on level 0
+    and then on level 1; one on the edge between two base patches is read on
+    both; one outside the domain nowhere.  Values equal the oracle's read of
+    the same pressure arrays."""
+    gas, h = make(WL64, 16)
+    d = D.Hierarchy(h, gas)
+    d.load_initial(WL64)
+    d.advance(2, dt_of(h, gas), mode="graph")
+    fine = h.level_patches(1)[0]
+    dx0, dx1 = h.dx(0), h.dx(1)
+    mic_x = [(fine.box[0] + 2.5) * dx1, 16 * dx0, 1.5]
+    mic_y = [(fine.box[2] + 3.5) * dx1, 5.5 * dx0, 0.5]
+    got = d.sample_mics(mic_x, mic_y)
+    states = d.states()
+    want = [[] for _ in mic_x]
+    for lev in range(h.nlevels):
+        dx = h.dx(lev)
+        for p in h.level_patches(lev):
+            pr = np.ascontiguousarray(states[p.pid][1][p.g:-p.g, p.g:-p.g])
+            v, hit = oracle.sample_mics(p.box, (p.box[0] * dx, p.box[2] * dx), ((p.box[1] + 1) * dx, (p.box[3] + 1) * dx),
+                                        (dx, dx), mic_x, mic_y, pr)
+            for k in np.nonzero(hit)[0]:
+                want[k].append((lev, p.pid, float(v[k])))
+    assert got == want
+    assert [e[0] for e in got[0]] == [0, 1]
+    assert [e[0] for e in got[1]].count(0) == 2
+    assert got[2] == []
+    d.close()
