@@ -104,7 +104,9 @@ def amr_bench(args):
     on the uniform grid until the pulses have spread, then one level refined
     by 4 over tag_cells' tags.  The drop-in entry points patch by patch
     (plain) against the batched fused stages replayed from a hipGraph, with
-    the stage kernels' tile shape picked by timing each candidate."""
+    the stage kernels' tile shape picked by timing each candidate; then one
+    regrid of the evolved state (device tags, host clustering and schedules,
+    the new batch and the transfer on the device), timed by part."""
     import numpy as np
     import torch
     from enlightning_amd import amr
