@@ -61,7 +61,9 @@ def reflect_from_scalars(d):
 
 def save_restart(path, hier, iteration=0, loop_time=0.0, **extra):
     """Checkpoint `hier` (a DeviceHierarchy, or anything with .h, .gas,
-    .reflect, .absorb_width, states() and sources()) into `path` (.npz)."""
+    .reflect, .absorb_width, states() and sources()) into `path` (.npz).
+    The FP64 state is what is saved: after FP32-form steps call
+    hier.fp32_store() first."""
     h = hier.h
     arrays = {
         "grid": np.array([h.n0, h.ratio, h.g], np.int64),
