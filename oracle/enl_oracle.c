@@ -57,6 +57,29 @@
 /* Fortran REAL literal -> double, as flang/gfortran fold it. */
 #define F(x) ((double)(float)(x))
 
+/* Every single-precision-folded literal of the Fortran kernels, in one table
+ * (SURVEY Appendix A order).  The functions below read their constants from
+ * here and orc_literals() exports this same table, so the pin test of the
+ * table checks the values the kernels actually use. */
+enum {
+  L_13_12, L_0_1, L_0_6, L_0_3, L_EP5, L_1_6, L_1_3, L_2_3, L_4_3,
+  L_A1, L_A2, L_A3, L_0_78, L_0_9, L_0_9999, L_EP3, L_COUNT
+};
+static const double LIT[L_COUNT] = {
+  F(13.0f / 12.0f), /* wenop/wenom beta, rhs_hybrid_high_llf.m4:375-380 */
+  F(0.1f), F(0.6f), F(0.3f), /* WENO5 ideal weights, :382-384, 417-419 */
+  F(10.0e-6f),      /* WENO5 epsilon, :373, 408 */
+  F(1.0f / 6.0f),   /* WENO5 candidates, :392-394, 427-429 */
+  F(1.0f / 3.0f), F(2.0f / 3.0f), /* WENO3 weights (rhs_hybrid_low_llf.m4:360-361,
+                                     386-387) and viscous phi (rhs_hybrid_high_llf.m4:174-175) */
+  F(4.0f / 3.0f),   /* viscous phi */
+  F(0.79926643f), F(-0.18941314f), F(0.02651995f), /* DRP a_1..a_3, :117-119 */
+  F(0.78f),         /* c_vn, :217 */
+  F(0.9f),          /* sensor / tag epsilon, :115, tag_cells.m4:65 */
+  F(0.9999f),       /* sensor / tag threshold, :247, tag_cells.m4:72 */
+  F(10e-6f),        /* WENO3 epsilon, rhs_hybrid_low_llf.m4:355, 381 */
+};
+
 static double sq(double x) { return x * x; }
 
 /* Strided view of one cell-centred array.  base points at element (i0,j0,0). */
@@ -91,46 +114,46 @@ static view_t patch_view(const double *p, int if0, int il0, int if1, int il1, in
 
 /* rhs_hybrid_high_llf.m4:365-398 (identical copy in every *_high* file). */
 double orc_wenom5(double fgm2, double fgm1, double fg0, double fgp1, double fgp2) {
-  const double ep = F(10.0e-6f);
-  double beta0 = F(13.0f / 12.0f) * sq(fg0 - 2 * fgp1 + fgp2) + 0.25 * sq(3 * fg0 - 4 * fgp1 + fgp2);
-  double beta1 = F(13.0f / 12.0f) * sq(fgm1 - 2 * fg0 + fgp1) + 0.25 * sq(fgm1 - fgp1);
-  double beta2 = F(13.0f / 12.0f) * sq(fgm2 - 2 * fgm1 + fg0) + 0.25 * sq(fgm2 - 4 * fgm1 + 3 * fg0);
-  double zeta0 = F(0.1f) / sq(ep + beta0);
-  double zeta1 = F(0.6f) / sq(ep + beta1);
-  double zeta2 = F(0.3f) / sq(ep + beta2);
+  const double ep = LIT[L_EP5];
+  double beta0 = LIT[L_13_12] * sq(fg0 - 2 * fgp1 + fgp2) + 0.25 * sq(3 * fg0 - 4 * fgp1 + fgp2);
+  double beta1 = LIT[L_13_12] * sq(fgm1 - 2 * fg0 + fgp1) + 0.25 * sq(fgm1 - fgp1);
+  double beta2 = LIT[L_13_12] * sq(fgm2 - 2 * fgm1 + fg0) + 0.25 * sq(fgm2 - 4 * fgm1 + 3 * fg0);
+  double zeta0 = LIT[L_0_1] / sq(ep + beta0);
+  double zeta1 = LIT[L_0_6] / sq(ep + beta1);
+  double zeta2 = LIT[L_0_3] / sq(ep + beta2);
   double sigma = zeta0 + zeta1 + zeta2;
   double omega0 = zeta0 / sigma, omega1 = zeta1 / sigma, omega2 = zeta2 / sigma;
-  double f0 = F(1.0f / 6.0f) * (11 * fg0 - 7 * fgp1 + 2 * fgp2);
-  double f1 = F(1.0f / 6.0f) * (2 * fgm1 + 5 * fg0 - fgp1);
-  double f2 = F(1.0f / 6.0f) * (-fgm2 + 5 * fgm1 + 2 * fg0);
+  double f0 = LIT[L_1_6] * (11 * fg0 - 7 * fgp1 + 2 * fgp2);
+  double f1 = LIT[L_1_6] * (2 * fgm1 + 5 * fg0 - fgp1);
+  double f2 = LIT[L_1_6] * (-fgm2 + 5 * fgm1 + 2 * fg0);
   return omega0 * f0 + omega1 * f1 + omega2 * f2;
 }
 
 /* rhs_hybrid_high_llf.m4:400-433. */
 double orc_wenop5(double fgm2, double fgm1, double fg0, double fgp1, double fgp2) {
-  const double ep = F(10.0e-6f);
-  double beta0 = F(13.0f / 12.0f) * sq(fg0 - 2 * fgp1 + fgp2) + 0.25 * sq(3 * fg0 - 4 * fgp1 + fgp2);
-  double beta1 = F(13.0f / 12.0f) * sq(fgm1 - 2 * fg0 + fgp1) + 0.25 * sq(fgm1 - fgp1);
-  double beta2 = F(13.0f / 12.0f) * sq(fgm2 - 2 * fgm1 + fg0) + 0.25 * sq(fgm2 - 4 * fgm1 + 3 * fg0);
-  double zeta0 = F(0.3f) / sq(ep + beta0);
-  double zeta1 = F(0.6f) / sq(ep + beta1);
-  double zeta2 = F(0.1f) / sq(ep + beta2);
+  const double ep = LIT[L_EP5];
+  double beta0 = LIT[L_13_12] * sq(fg0 - 2 * fgp1 + fgp2) + 0.25 * sq(3 * fg0 - 4 * fgp1 + fgp2);
+  double beta1 = LIT[L_13_12] * sq(fgm1 - 2 * fg0 + fgp1) + 0.25 * sq(fgm1 - fgp1);
+  double beta2 = LIT[L_13_12] * sq(fgm2 - 2 * fgm1 + fg0) + 0.25 * sq(fgm2 - 4 * fgm1 + 3 * fg0);
+  double zeta0 = LIT[L_0_3] / sq(ep + beta0);
+  double zeta1 = LIT[L_0_6] / sq(ep + beta1);
+  double zeta2 = LIT[L_0_1] / sq(ep + beta2);
   double sigma = zeta0 + zeta1 + zeta2;
   double omega0 = zeta0 / sigma, omega1 = zeta1 / sigma, omega2 = zeta2 / sigma;
-  double f0 = F(1.0f / 6.0f) * (2 * fg0 + 5 * fgp1 - fgp2);
-  double f1 = F(1.0f / 6.0f) * (-fgm1 + 5 * fg0 + 2 * fgp1);
-  double f2 = F(1.0f / 6.0f) * (2 * fgm2 - 7 * fgm1 + 11 * fg0);
+  double f0 = LIT[L_1_6] * (2 * fg0 + 5 * fgp1 - fgp2);
+  double f1 = LIT[L_1_6] * (-fgm1 + 5 * fg0 + 2 * fgp1);
+  double f2 = LIT[L_1_6] * (2 * fgm2 - 7 * fgm1 + 11 * fg0);
   return omega0 * f0 + omega1 * f1 + omega2 * f2;
 }
 
 /* rhs_hybrid_low_llf.m4:349-373 (ep literal 10e-6 there, 10.0e-6 in
  * rhs_weno_low*.m4:313,339 -- both fold to the same float). */
 double orc_wenom3(double fgm1, double fg0, double fgp1) {
-  const double ep = F(10e-6f);
+  const double ep = LIT[L_EP3];
   double beta0 = sq(fgp1 - fg0);
   double beta1 = sq(fg0 - fgm1);
-  double zeta0 = F(1.0f / 3.0f) / sq(ep + beta0);
-  double zeta1 = F(2.0f / 3.0f) / sq(ep + beta1);
+  double zeta0 = LIT[L_1_3] / sq(ep + beta0);
+  double zeta1 = LIT[L_2_3] / sq(ep + beta1);
   double sigma = zeta0 + zeta1;
   double omega0 = zeta0 / sigma, omega1 = zeta1 / sigma;
   double f0 = 0.5 * (3 * fg0 - fgp1);
@@ -140,11 +163,11 @@ double orc_wenom3(double fgm1, double fg0, double fgp1) {
 
 /* rhs_hybrid_low_llf.m4:375-399. */
 double orc_wenop3(double fgm1, double fg0, double fgp1) {
-  const double ep = F(10e-6f);
+  const double ep = LIT[L_EP3];
   double beta0 = sq(fgp1 - fg0);
   double beta1 = sq(fg0 - fgm1);
-  double zeta0 = F(2.0f / 3.0f) / sq(ep + beta0);
-  double zeta1 = F(1.0f / 3.0f) / sq(ep + beta1);
+  double zeta0 = LIT[L_2_3] / sq(ep + beta0);
+  double zeta1 = LIT[L_1_3] / sq(ep + beta1);
   double sigma = zeta0 + zeta1;
   double omega0 = zeta0 / sigma, omega1 = zeta1 / sigma;
   double f0 = 0.5 * (fg0 + fgp1);
@@ -162,7 +185,7 @@ static inline double smooth_ratio(double tmp1, double tmp2, double ep) {
  * (rhs_hybrid_high_llf.m4:114-115, tag_cells.m4:64-65). */
 double orc_sensor_eps(double tol) {
   double xi = 1;
-  return F(0.9f) * tol * sq(xi) / (1 - F(0.9f) * sq(tol));
+  return LIT[L_0_9] * tol * sq(xi) / (1 - LIT[L_0_9] * sq(tol));
 }
 
 /* ------------------------------------------------------------------------ */
@@ -222,7 +245,7 @@ static void cell_source(const view_t *w, const view_t *T, double H0, int i, int 
   double u_dprime_x = (U(i - 1, j) - 2 * U(i, j) + U(i + 1, j)) / sq(dx);
   double u_dprime_y = (U(i, j - 1) - 2 * U(i, j) + U(i, j + 1)) / sq(dy);
   double v_dprime_xy = (V(i + 1, j + 1) - V(i + 1, j - 1) - V(i - 1, j + 1) + V(i - 1, j - 1)) / (4 * dx * dy);
-  double phi_xx_prime_x = F(4.0f / 3.0f) * u_dprime_x - F(2.0f / 3.0f) * v_dprime_xy;
+  double phi_xx_prime_x = LIT[L_4_3] * u_dprime_x - LIT[L_2_3] * v_dprime_xy;
   double phi_xy_prime_y = u_dprime_y + v_dprime_xy;
   H[1] = P->muB * (u_dprime_x + v_dprime_xy) + P->mu * (phi_xx_prime_x + phi_xy_prime_y);
 
@@ -230,7 +253,7 @@ static void cell_source(const view_t *w, const view_t *T, double H0, int i, int 
   double v_dprime_x = (V(i - 1, j) - 2 * V(i, j) + V(i + 1, j)) / sq(dx);
   double u_dprime_yx = (U(i + 1, j + 1) - U(i + 1, j - 1) - U(i - 1, j + 1) + U(i - 1, j - 1)) / (4 * dx * dy);
   double phi_yx_prime_x = u_dprime_yx + v_dprime_x;
-  double phi_yy_prime_y = F(4.0f / 3.0f) * v_dprime_y - F(2.0f / 3.0f) * u_dprime_yx;
+  double phi_yy_prime_y = LIT[L_4_3] * v_dprime_y - LIT[L_2_3] * u_dprime_yx;
   H[2] = P->muB * (v_dprime_y + u_dprime_yx) + P->mu * (phi_yx_prime_x + phi_yy_prime_y);
 
   double u_prime_x = (U(i + 1, j) - U(i - 1, j)) / (2 * dx);
@@ -246,11 +269,11 @@ static void cell_source(const view_t *w, const view_t *T, double H0, int i, int 
   double T_n = at(w, i, j, 4) / w0;
   double Mn = (temp - T_n) / P->tau_n;
   double tmp1 = P->T_star_n / (T_n);
-  double c_vn = F(0.78f) * P->R * sq(tmp1) * exp(-tmp1);
+  double c_vn = LIT[L_0_78] * P->R * sq(tmp1) * exp(-tmp1);
   double A_n = ((temp / T_n) - 1) * c_vn;
-  double phi_xx = F(4.0f / 3.0f) * u_prime_x - F(2.0f / 3.0f) * v_prime_y;
+  double phi_xx = LIT[L_4_3] * u_prime_x - LIT[L_2_3] * v_prime_y;
   double phi_xy_yx = u_prime_y + v_prime_x;
-  double phi_yy = F(4.0f / 3.0f) * v_prime_y - F(2.0f / 3.0f) * u_prime_x;
+  double phi_yy = LIT[L_4_3] * v_prime_y - LIT[L_2_3] * u_prime_x;
   double sigma = (P->muB / temp) * sq(u_prime_x + v_prime_y) +
                  (P->mu / (2 * temp)) * (sq(phi_xx) + 2 * sq(phi_xy_yx) + sq(phi_yy)) +
                  (P->kappa / sq(temp)) * (sq(t_prime_x) + sq(t_prime_y)) +
@@ -299,11 +322,11 @@ static int cell_sensor(const view_t *w, int i, int j, double ep, int level) {
       double tmp1 = at(w, i + q, j, 0) - at(w, i - 1 + q, j, 0);
       double tmp2 = at(w, i + 1 + q, j, 0) - at(w, i + q, j, 0);
       double rx = smooth_ratio(tmp1, tmp2, ep);
-      if (rx < F(0.9999f)) tag = 1;
+      if (rx < LIT[L_0_9999]) tag = 1;
       tmp1 = at(w, i, j + q, 0) - at(w, i, j - 1 + q, 0);
       tmp2 = at(w, i, j + 1 + q, 0) - at(w, i, j + q, 0);
       double ry = smooth_ratio(tmp1, tmp2, ep);
-      if (ry < F(0.9999f)) tag = 1;
+      if (ry < LIT[L_0_9999]) tag = 1;
     }
   }
   return tag;
@@ -324,7 +347,7 @@ static int rhs_cell(const scheme_t *s, const view_t *w, const view_t *p, const v
 
   if (tag == 0) {
     /* DRP branch: rhs_hybrid_high_llf.m4:258-268, rhs_drp.m4:227-236 */
-    const double a_1 = F(0.79926643f), a_2 = F(-0.18941314f), a_3 = F(0.02651995f);
+    const double a_1 = LIT[L_A1], a_2 = LIT[L_A2], a_3 = LIT[L_A3];
     for (int k = 0; k < 5; k++) {
       *ref(KK, i, j, k) =
           (-a_3 * flux_x(w, p, i - 3, j, k) - a_2 * flux_x(w, p, i - 2, j, k) - a_1 * flux_x(w, p, i - 1, j, k) +
@@ -505,23 +528,23 @@ void orc_tag_cells(const int *ifirst0, const int *ilast0, const int *ifirst1, co
   for (int j = if1; j <= il1; j++)
     for (int i = if0; i <= il0; i++) {
       double rx = smooth_ratio(at(&h, i, j, 0) - at(&h, i - 1, j, 0), at(&h, i + 1, j, 0) - at(&h, i, j, 0), ep);
-      if (rx < F(0.9999f)) TAG(i, j) = *refine_tag_val;
+      if (rx < LIT[L_0_9999]) TAG(i, j) = *refine_tag_val;
     }
   for (int j = if1; j <= il1; j++)
     for (int i = if0; i <= il0; i++) {
       double ry = smooth_ratio(at(&h, i, j, 0) - at(&h, i, j - 1, 0), at(&h, i, j + 1, 0) - at(&h, i, j, 0), ep);
-      if (ry < F(0.9999f)) TAG(i, j) = *refine_tag_val;
+      if (ry < LIT[L_0_9999]) TAG(i, j) = *refine_tag_val;
     }
   ep = orc_sensor_eps(tolerance[1]);
   for (int j = if1; j <= il1; j++)
     for (int i = if0; i <= il0; i++) {
       double rx = smooth_ratio(at(&w, i, j, 0) - at(&w, i - 1, j, 0), at(&w, i + 1, j, 0) - at(&w, i, j, 0), ep);
-      if (rx < F(0.9999f)) TAG(i, j) = *refine_tag_val;
+      if (rx < LIT[L_0_9999]) TAG(i, j) = *refine_tag_val;
     }
   for (int j = if1; j <= il1; j++)
     for (int i = if0; i <= il0; i++) {
       double ry = smooth_ratio(at(&w, i, j, 0) - at(&w, i, j - 1, 0), at(&w, i, j + 1, 0) - at(&w, i, j, 0), ep);
-      if (ry < F(0.9999f)) TAG(i, j) = *refine_tag_val;
+      if (ry < LIT[L_0_9999]) TAG(i, j) = *refine_tag_val;
     }
 #undef TAG
 }
@@ -860,22 +883,7 @@ int orc_advance_patch(int variant, int if0, int il0, int if1, int il1, int g, co
   return 0;
 }
 
-/* Constants of the single-precision-folded literals, for the pin tests. */
-void orc_literals(double *out /* 16 */) {
-  out[0] = F(13.0f / 12.0f);
-  out[1] = F(0.1f);
-  out[2] = F(0.6f);
-  out[3] = F(0.3f);
-  out[4] = F(10.0e-6f);
-  out[5] = F(1.0f / 6.0f);
-  out[6] = F(1.0f / 3.0f);
-  out[7] = F(2.0f / 3.0f);
-  out[8] = F(4.0f / 3.0f);
-  out[9] = F(0.79926643f);
-  out[10] = F(-0.18941314f);
-  out[11] = F(0.02651995f);
-  out[12] = F(0.78f);
-  out[13] = F(0.9f);
-  out[14] = F(0.9999f);
-  out[15] = F(10e-6f);
+/* The literal table, for the pin tests. */
+void orc_literals(double *out /* L_COUNT = 16 */) {
+  for (int k = 0; k < L_COUNT; k++) out[k] = LIT[k];
 }

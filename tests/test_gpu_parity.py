@@ -9,6 +9,8 @@ Bar (per field, relative L-infinity against the field's max |value|):
   tag_cells      bit-exact;
   physical BC    bit-exact.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -90,6 +92,69 @@ def test_rhs_ragged_and_offset_boxes(shape, g, box0, variant):
     Kg, smg = gpu_rhs(variant, box, g, xlo, dx, w, p, T, src, gas, 0)
     Kc, smc = cpu_rhs(variant, box, g, xlo, dx, w, p, T, src, gas, 0)
     assert_rhs_parity(Kg, smg, Kc, smc, str(shape))
+
+
+def _step_ratio(k, ep):
+    """The sensor's ratio for the triple (0, dd) of a density step from 1 to
+    1 + k 2^-52 (dd = k 2^-52 exactly), in the reference's double
+    operations: (2|0*dd| + ep) / (0^2 + dd^2 + ep)."""
+    dd = np.float64(k) * np.float64(2.0) ** -52
+    return float(np.float64(ep) / (np.float64(0.0) * np.float64(0.0) + dd * dd + np.float64(ep)))
+
+
+def _threshold_cases(ep):
+    """Step sizes k (in units of 2^-52) whose ratio r lands (1) on each of the
+    double values next to float(0.9999), where only the exact quotient can
+    decide, and (2) across the +-1e-6 relative band around it in which the
+    HIP sensor leaves its single-precision estimate for the exact quotient,
+    and just outside that band."""
+    thr = float(np.float32(0.9999))
+    d0 = math.sqrt(ep / thr - ep)
+    k0 = int(round(d0 / 2.0 ** -52))
+    # r decreases with k: find the last k with r >= thr
+    k = k0 - 64
+    while _step_ratio(k + 1, ep) >= thr:
+        k += 1
+    assert _step_ratio(k, ep) >= thr > _step_ratio(k + 1, ep)
+    ks = list(range(k - 3, k + 5))
+    for rel in (-2e-6, -1.1e-6, -9e-7, -5e-7, -1e-7, 1e-7, 5e-7, 9e-7, 1.1e-6, 2e-6):
+        ks.append(int(round(math.sqrt(ep / (thr * (1 + rel)) - ep) / 2.0 ** -52)))
+    return thr, sorted(set(ks))
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_sensor_near_threshold_bit_exact(axis):
+    """Switch maps at level 0 from the HIP split kernel against the oracle
+    (and the closed form: the step is flagged on the four cells whose triples
+    hold it iff r < float(0.9999), rhs_hybrid_high_llf.m4:244-254) for steps
+    whose ratio sits within a few ulps of the threshold and inside the band
+    where the device decides by the exact quotient."""
+    gas = Gas()
+    ep = oracle.sensor_eps(gas.tag_tolerance[2])
+    thr, ks = _threshold_cases(ep)
+    n, g, J = 16, 4, 8
+    below = 0
+    for k in ks:
+        r = _step_ratio(k, ep)
+        w, p, T, src = random_state(n, n, g, seed=40)
+        hi = 1.0 + k * 2.0 ** -52
+        prof = np.where(np.arange(n + 2 * g) - g >= J, hi, 1.0)
+        w[0] = prof[None, :] if axis == 0 else prof[:, None]
+        w[1] = w[2] = 0.0
+        box, xlo, dx = (0, n - 1, 0, n - 1), (0.0, 0.0), (1.0 / 64, 1.0 / 64)
+        Kg, smg = gpu_rhs("hybrid_high_llf", box, g, xlo, dx, w, p, T, src, gas, 0)
+        Kc, smc = cpu_rhs("hybrid_high_llf", box, g, xlo, dx, w, p, T, src, gas, 0)
+        expect = np.zeros((n, n), np.int32)
+        if r < thr:
+            if axis == 0:
+                expect[:, J - 2:J + 2] = 1
+            else:
+                expect[J - 2:J + 2, :] = 1
+        assert np.array_equal(smc, expect), (k, r)
+        assert np.array_equal(smg, smc), (k, r, r / thr - 1)
+        assert_rhs_parity(Kg, smg, Kc, smc, str(k))
+        below += r < thr
+    assert 0 < below < len(ks)  # both decisions, on either side of the threshold
 
 
 def test_rhs_host_pointer_staging_matches_device_path():

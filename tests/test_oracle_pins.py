@@ -23,6 +23,7 @@ sign) fails at least one check:
   closed form, physical BC closed form.
 """
 import math
+from fractions import Fraction as Q
 
 import numpy as np
 import pytest
@@ -134,6 +135,128 @@ def test_weno3_convergence_order(orc):
     e = [_weno_derivative_error(orc.wenop3, 3, 3, h, 1e-3) for h in hs]
     orders = [math.log2(e[i] / e[i + 1]) for i in range(2)]
     assert min(orders) > 2.7, (e, orders)
+
+
+# Textbook Jiang-Shu reconstructions (p = 2) in exact rational arithmetic.
+# Candidates are named by their stencils; d are the linear weights, eps the
+# regulariser.  Inputs, weights and eps are exact rationals, so the only
+# difference from the oracle is the oracle's own rounding.
+
+def _js_exact(cands, betas, d, eps):
+    a = [dk / (eps + bk) ** 2 for dk, bk in zip(d, betas)]
+    s = sum(a)
+    return sum(ak / s * qk for ak, qk in zip(a, cands))
+
+
+def _weno3_exact(fm1, f0, fp1, plus, eps, third, two_thirds):
+    fm1, f0, fp1 = Q(fm1), Q(f0), Q(fp1)
+    # stencil {i-1, i} and stencil {i, i+1}, interface i+1/2 (plus) or i-1/2
+    if plus:
+        left, right = (3 * f0 - fm1) / 2, (f0 + fp1) / 2
+        d = (third, two_thirds)
+    else:
+        left, right = (fm1 + f0) / 2, (3 * f0 - fp1) / 2
+        d = (two_thirds, third)
+    return _js_exact((left, right), ((f0 - fm1) ** 2, (fp1 - f0) ** 2), d, eps)
+
+
+def _weno5_exact(f, plus, eps, w_lo, w_mid, w_hi, sixth, c1312):
+    fm2, fm1, f0, fp1, fp2 = map(Q, f)
+    bl = c1312 * (fm2 - 2 * fm1 + f0) ** 2 + Q(1, 4) * (fm2 - 4 * fm1 + 3 * f0) ** 2
+    bc = c1312 * (fm1 - 2 * f0 + fp1) ** 2 + Q(1, 4) * (fm1 - fp1) ** 2
+    br = c1312 * (f0 - 2 * fp1 + fp2) ** 2 + Q(1, 4) * (3 * f0 - 4 * fp1 + fp2) ** 2
+    if plus:   # value at i+1/2 from stencils {i-2..i}, {i-1..i+1}, {i..i+2}
+        cands = (sixth * (2 * fm2 - 7 * fm1 + 11 * f0), sixth * (-fm1 + 5 * f0 + 2 * fp1),
+                 sixth * (2 * f0 + 5 * fp1 - fp2))
+        d = (w_lo, w_mid, w_hi)
+    else:      # value at i-1/2
+        cands = (sixth * (-fm2 + 5 * fm1 + 2 * f0), sixth * (2 * fm1 + 5 * f0 - fp1),
+                 sixth * (11 * f0 - 7 * fp1 + 2 * fp2))
+        d = (w_hi, w_mid, w_lo)
+    return _js_exact(cands, (bl, bc, br), d, eps)
+
+
+def _near_eps_inputs(n, npts, seed):
+    # differences ~ sqrt(eps): the smoothness indicators are comparable to
+    # eps, where the weights depend on eps at first order
+    rng = np.random.default_rng(seed)
+    return [np.cumsum(rng.normal(scale=3e-3, size=npts)) + rng.uniform(-1e-2, 1e-2) for _ in range(n)]
+
+
+def test_weno3_exact_rational_with_float_eps(orc):
+    """WENO3 against the textbook formula with the float-folded eps = 10e-6
+    and weights (1.0/3.0), (2.0/3.0) of rhs_hybrid_low_llf.m4:355-361,
+    381-387.  A double-precision eps moves the result by ~1e6 ulps here."""
+    eps = Q(f32("10e-6"))
+    third = Q(float(np.float32(1) / np.float32(3)))
+    two = Q(float(np.float32(2) / np.float32(3)))
+    worst, moved = 0.0, []
+    for f in _near_eps_inputs(300, 3, 21):
+        for plus, fn in ((True, orc.wenop3), (False, orc.wenom3)):
+            ex = _weno3_exact(*f, plus, eps, third, two)
+            ulp = np.spacing(float(abs(ex)) + float(np.abs(f).max()))
+            worst = max(worst, abs(fn(*f) - float(ex)) / ulp)
+            moved.append(abs(float(_weno3_exact(*f, plus, Q(1, 100000), third, two) - ex)) / ulp)
+    assert worst <= 8, worst
+    assert np.median(moved) > 1e3, np.median(moved)
+
+
+def test_weno5_exact_rational_with_float_literals(orc):
+    """WENO5 against the textbook formula with the float-folded literals of
+    rhs_hybrid_high_llf.m4:365-433 (eps 10.0e-6, 13.0/12.0, weights 0.1, 0.6,
+    0.3 and the candidates' 1.0/6.0)."""
+    L = dict(eps=Q(f32("10.0e-6")), w_lo=Q(f32("0.1")), w_mid=Q(f32("0.6")), w_hi=Q(f32("0.3")),
+             sixth=Q(float(np.float32(1) / np.float32(6))), c1312=Q(float(np.float32(13) / np.float32(12))))
+    worst, moved_eps, moved_w = 0.0, [], []
+    for f in _near_eps_inputs(200, 5, 22):
+        for plus, fn in ((True, orc.wenop5), (False, orc.wenom5)):
+            ex = _weno5_exact(f, plus, **L)
+            ulp = np.spacing(float(abs(ex)) + float(np.abs(f).max()))
+            worst = max(worst, abs(fn(*f) - float(ex)) / ulp)
+            moved_eps.append(abs(float(_weno5_exact(f, plus, **dict(L, eps=Q(1, 100000))) - ex)) / ulp)
+            dbl = dict(L, w_lo=Q(0.1), w_mid=Q(0.6), w_hi=Q(0.3))
+            moved_w.append(abs(float(_weno5_exact(f, plus, **dbl) - ex)) / ulp)
+    assert worst <= 16, worst
+    assert np.median(moved_eps) > 1e3 and np.median(moved_w) > 1e3, (np.median(moved_eps), np.median(moved_w))
+
+
+def _fortran_weno3(fm1, f0, fp1, plus, lit):
+    """The Fortran statements of wenom / wenop (rhs_hybrid_low_llf.m4:349-399)
+    in IEEE double arithmetic, one operation at a time, with `lit` holding the
+    literals (1.0/3.0), (2.0/3.0) and 10e-6 as the compiler folds them."""
+    third, two, ep = lit
+    beta0 = (fp1 - f0) * (fp1 - f0)
+    beta1 = (f0 - fm1) * (f0 - fm1)
+    zeta0 = (two if plus else third) / ((ep + beta0) * (ep + beta0))
+    zeta1 = (third if plus else two) / ((ep + beta1) * (ep + beta1))
+    sigma = zeta0 + zeta1
+    omega0, omega1 = zeta0 / sigma, zeta1 / sigma
+    if plus:
+        c0, c1 = 0.5 * (f0 + fp1), 0.5 * (-fm1 + 3 * f0)
+    else:
+        c0, c1 = 0.5 * (3 * f0 - fp1), 0.5 * (fm1 + f0)
+    return omega0 * c0 + omega1 * c1
+
+
+def test_weno3_weight_literals_fold_in_single_precision(orc):
+    """float(2/3) = 2 float(1/3) exactly, as for the doubles, so the folded
+    and the double weights give the same normalised weights in exact
+    arithmetic; which one the Fortran uses shows only in the rounding of zeta
+    and sigma.  This pins it at the bit level: the oracle must give the bits
+    of the Fortran statements evaluated in IEEE double with the REAL literals
+    folded in single precision (numpy's float32 parse), and the double
+    literals must give other bits on some inputs."""
+    single = (float(np.float32(1) / np.float32(3)), float(np.float32(2) / np.float32(3)), f32("10e-6"))
+    double = (1.0 / 3.0, 2.0 / 3.0, 10e-6)
+    differs = 0
+    for f in _near_eps_inputs(2000, 3, 23):
+        f = [float(x) for x in f]
+        for plus, fn in ((True, orc.wenop3), (False, orc.wenom3)):
+            got = fn(*f)
+            assert got == _fortran_weno3(*f, plus, single), (f, plus)
+            weights_only = (double[0], double[1], single[2])
+            differs += got != _fortran_weno3(*f, plus, weights_only)
+    assert differs > 100, differs
 
 
 # ---------------------------------------------------------------- sensor / switch map
@@ -375,6 +498,82 @@ def test_llf_alpha_window(orc):
         window[j0, i0 - r:i0 + r + 1] = True
         assert not differ[~window].any(), variant
         assert differ[window].all(), variant
+
+
+def _height(xlo1, dy, jj):
+    """Cell-centre height of row jj (relative to ifirst1), as the LLF loop of
+    rhs_hybrid_high_llf.m4:273, 279 forms it."""
+    return xlo1 + dy * (float(jj) + 0.5)
+
+
+def _layered(n, g, axis, seed):
+    """u = v = 0; rho, rho*s, rho*T_n, p and T vary along `axis` only (0: x,
+    1: y), with a density and pressure jump in the middle, so the LLF
+    dissipation alpha*(w jump) is O(1) next to it."""
+    rng = np.random.default_rng(seed)
+    N = n + 2 * g
+    t = np.arange(N) - g
+    jump = (t >= n // 2).astype(float)
+    rho = 1.2 + 0.3 * jump + 0.01 * np.sin(0.7 * t + rng.uniform(0, 6))
+    p = 101325.0 * (1 + 0.2 * jump) + 500.0 * np.cos(0.5 * t)
+    T = 294.0 + 3.0 * np.sin(0.3 * t)
+    s = 5.0 * np.cos(0.4 * t)
+    prof = [rho, np.zeros(N), np.zeros(N), rho * s, rho * (T + 1.0)]
+    shape = (N, N)
+    bcast = (lambda a: np.broadcast_to(a[None, :], shape)) if axis == 0 else (lambda a: np.broadcast_to(a[:, None], shape))
+    w = np.ascontiguousarray(np.stack([bcast(a) for a in prof]))
+    return w, np.ascontiguousarray(bcast(p)), np.ascontiguousarray(bcast(T)), np.zeros((n, n))
+
+
+def _natural_scale(w, p, alpha, dx):
+    # size of each component's flux divergence: alpha*w (+ p for momenta) / dx
+    s = np.array([alpha * np.abs(w[k]).max() for k in range(5)])
+    s[1] += np.abs(p).max()
+    s[2] += np.abs(p).max()
+    return s / dx
+
+
+@pytest.mark.parametrize("llf,fixed,reach,level", [
+    ("weno_high_llf", "weno_high", 3, 0), ("weno_low_llf", "weno_low", 2, 0),
+    ("hybrid_high_llf", "hybrid_high", 3, 1), ("hybrid_low_llf", "hybrid_low", 2, 1)])
+@pytest.mark.parametrize("c_lr", [2000.0, -600.0])
+def test_llf_height_term(orc, llf, fixed, reach, level, c_lr):
+    """rhs_hybrid_high_llf.m4:270-284 with u = v = 0: the x-window's alpha is
+    c + c_lr*y_j for every q (its height has no q), the y-window's is the
+    largest c + c_lr*y_{j+q}, i.e. y_{j+reach} for c_lr > 0 and y_{j-reach}
+    for c_lr < 0.  Row j of the LLF result must equal the fixed-alpha twin
+    run with that alpha, and differ from the twins with the other heights.
+    (The direction without variation contributes a rounding residual only,
+    so the match is to 1e-13 of the natural scale, not bitwise.)"""
+    from enlightning_amd.gas import Gas
+    n, g = 16, 4
+    dx = 1.0 / 64
+    xlo = (0.25, 0.125)
+    for axis in (0, 1):
+        w, p, T, src = _layered(n, g, axis, seed=31 + axis)
+        gas = Gas(c_lr=c_lr)
+        K, _ = _rhs(orc, gas, llf, w, p, T, src, g, dx=(dx, dx), level=level, xlo=xlo)
+        side = reach if c_lr > 0 else -reach
+        # the height each alpha takes, per row j: right one first
+        if axis == 1:
+            choices = [lambda j: j + side, lambda j: j, lambda j: j - side]
+        else:
+            choices = [lambda j: j, lambda j: j + side, lambda j: j - side]
+        scale = _natural_scale(w, p, gas.c + abs(c_lr), dx)
+        twins = {}
+        for ci, hj in enumerate(choices):
+            for j in range(n):
+                alpha = gas.c + c_lr * _height(xlo[1], dx, hj(j))
+                if alpha not in twins:
+                    twins[alpha] = _rhs(orc, Gas(c_lr=c_lr, weno_alpha=alpha), fixed, w, p, T, src, g,
+                                        dx=(dx, dx), level=level, xlo=xlo)[0]
+                err = np.abs(K[:, j, :] - twins[alpha][:, j, :]).max(axis=1) / scale
+                if ci == 0:
+                    assert err.max() <= 1e-13, (axis, j, err)
+                elif axis == 0 or abs(j - n // 2) <= reach:
+                    # next to the jump alpha changes the result far above
+                    # the 1e-13 match
+                    assert err.max() >= 1e-9, (axis, ci, j, err)
 
 
 # ---------------------------------------------------------------- sympy convergence of -div F
