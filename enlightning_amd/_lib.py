@@ -96,6 +96,7 @@ PROTOTYPES = {
     "enl_batch_patch_state": ([_P, _I] + [_P] * 4, _I),
     "enl_batch_copy_state": ([_P, _I, _I] + [_P] * 4, _I),
     "enl_batch_prepare": ([_P, _P], _I),
+    "enl_batch_zero_source": ([_P], _I),
     "enl_batch_fill": ([_P, _P], _I),
     "enl_batch_update_state": ([_P, ctypes.POINTER(EnlGas)], _I),
     "enl_batch_advance": ([_P, _I, _I, ctypes.POINTER(EnlGas), _P, _I, _P, _P, _P, _D, _I], _I),

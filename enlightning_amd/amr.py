@@ -4,13 +4,18 @@ This stands in for the parts of SAMRAI the reference relies on around the
 per-patch kernels (not vendored, so none of it is parity-pinned):
 
 * level 0: the base grid cut into patches of at most `tile0` cells a side
-  (input.txt:114-116, largest_patch_size level_0 = 64);
+  (input.txt:114-116, largest_patch_size level_0 = 64); a remainder
+  narrower than the ghost width joins the tile before it;
 * level 1: refine ratio r; tags from tag_cells on level 0, buffered by
   `tag_buffer` cells (input.txt:8), clustered into boxes whose fine sides lie
   in [8, 64]: the domain is cut into blocks of 64/r coarse cells and each
-  block's tagged cells become its bounding box (grown to 8/r coarse cells);
-  boxes within one coarse cell of the domain edge are extended to touch it,
-  as SAMRAI's gridding does so that ghost strips never straddle the edge;
+  block's tagged cells become its bounding box (grown to 8/r coarse cells
+  inside the block; a last block narrower than that keeps its width);
+  boxes within one coarse cell of the domain edge are extended to touch it
+  when their block reaches the edge, as SAMRAI's gridding does so that
+  ghost strips never straddle the edge (boxes never leave their block, so
+  they never overlap; with the ghost width at most r, a one-coarse-cell gap
+  to the edge holds a whole corner block);
 * ghost fill for every patch, per level, in three passes:
     1. ghost cells inside the domain: copy from the patch of the same level
        that holds the cell, else linear refinement of level-0 data (cell-
@@ -95,11 +100,22 @@ class Hierarchy:
         return int(sum(p.nx * p.ny for p in self.patches if level is None or p.level == level))
 
 
-def base_boxes(n0, tile0):
+def _cuts(n0, tile0, g):
+    """Tile starts along one axis; a last tile narrower than the ghost width
+    g joins the one before it (a patch within g cells of the domain edge
+    but not on it would have corner ghost blocks partly outside the domain)."""
+    starts = list(range(0, n0, tile0))
+    if len(starts) > 1 and n0 - starts[-1] < g:
+        starts.pop()
+    return starts + [n0]
+
+
+def base_boxes(n0, tile0, g=0):
+    cuts = _cuts(n0, tile0, g)
     boxes = []
-    for j0 in range(0, n0, tile0):
-        for i0 in range(0, n0, tile0):
-            boxes.append((i0, min(i0 + tile0, n0) - 1, j0, min(j0 + tile0, n0) - 1))
+    for j0, j1 in zip(cuts[:-1], cuts[1:]):
+        for i0, i1 in zip(cuts[:-1], cuts[1:]):
+            boxes.append((i0, i1 - 1, j0, j1 - 1))
     return boxes
 
 
@@ -139,27 +155,30 @@ def cluster_fine_boxes(tags, ratio, fine_max=64, fine_min=8):
         bj, bi = int(kj) * blk, int(ki) * blk
         i0, i1 = bi + int(ilo[kj, ki]), bi + int(ihi[kj, ki])
         j0, j1 = bj + int(jlo[kj, ki]), bj + int(jhi[kj, ki])
-        # at least cmin coarse cells a side, kept inside the block
+        # at least cmin coarse cells a side where the block allows it; a box
+        # never leaves its block, so boxes of different blocks cannot overlap
+        # (the last block of a row is narrower than cmin when n0 % blk is
+        # smaller: its box is then the block's width)
         bi1, bj1 = min(bi + blk, n0) - 1, min(bj + blk, n0) - 1
-        while i1 - i0 + 1 < cmin:
+        while i1 - i0 + 1 < cmin and (i1 < bi1 or i0 > bi):
             if i1 < bi1:
                 i1 += 1
             else:
                 i0 -= 1
-        while j1 - j0 + 1 < cmin:
+        while j1 - j0 + 1 < cmin and (j1 < bj1 or j0 > bj):
             if j1 < bj1:
                 j1 += 1
             else:
                 j0 -= 1
-        # touch the domain edge when within one coarse cell of it (the
-        # block edge coincides with the domain edge there)
-        if i0 == 1:
+        # touch the domain edge when within one coarse cell of it, where the
+        # block edge is the domain edge
+        if i0 == 1 and bi == 0:
             i0 = 0
-        if j0 == 1:
+        if j0 == 1 and bj == 0:
             j0 = 0
-        if i1 == n0 - 2:
+        if i1 == n0 - 2 and bi1 == n0 - 1:
             i1 = n0 - 1
-        if j1 == n0 - 2:
+        if j1 == n0 - 2 and bj1 == n0 - 1:
             j1 = n0 - 1
         boxes.append((i0 * ratio, (i1 + 1) * ratio - 1, j0 * ratio, (j1 + 1) * ratio - 1))
     return boxes
@@ -196,7 +215,7 @@ def build(n0, ratio, g, L, coarse_tags, tile0=64, tag_buffer=2, fine_max=64, fin
     At a regrid, `previous` (the hierarchy being replaced) lends its level-0
     fill schedule when its base patches are the same."""
     h = Hierarchy(n0=n0, ratio=ratio, g=g, L=L)
-    for b in base_boxes(n0, tile0):
+    for b in base_boxes(n0, tile0, g):
         h.patches.append(Patch(len(h.patches), 0, b, g))
     if coarse_tags is not None and np.any(coarse_tags):
         tb = buffer_tags(coarse_tags, tag_buffer)

@@ -27,6 +27,7 @@
 // hierarchy through the drop-in entry points, patch by patch, for comparison.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -712,6 +713,31 @@ int enl_batch_create(int npatch, const enl_patch_desc_t* patches, int gcw, int r
   return 0;
 }
 
+namespace {
+// Host checks of schedule items before they reach a kernel.  A (patch,
+// local) pair must name a cell of the patch's ghosted box; a refinement
+// source also needs its +-1 cell and +-1 row neighbours in that box (a
+// local in the first or last column passes a flat-index bound but its i+-1
+// reads would wrap into the neighbouring row).
+bool cell_ok(const std::vector<PatchDev>& hp, int p, long l) {
+  return p >= 0 && p < (int)hp.size() && l >= 0 && l < hp[p].sk;
+}
+bool stencil_ok(const PatchDev& S, long l) {
+  if (l < 0 || l >= S.sk) return false;
+  const long col = l % S.sy, row = l / S.sy, rows = S.sk / S.sy;
+  return col >= 1 && col <= S.sy - 2 && row >= 1 && row <= rows - 2;
+}
+// the r x r fine block whose lower-left cell is l lies in the fine box
+bool block_ok(const PatchDev& F, long l, int r) {
+  if (l < 0 || l >= F.sk) return false;
+  const long col = l % F.sy, row = l / F.sy, rows = F.sk / F.sy;
+  return col + r <= F.sy && row + r <= rows;
+}
+bool null_list(long n, const void* a, const void* b, const void* c, const void* d) {
+  return n > 0 && !(a && b && c && d);
+}
+}  // namespace
+
 int enl_batch_set_fill(void* handle, int level, int ncopy, const int* c_dp, const int* c_dl, const int* c_sp,
                        const int* c_sl, int ninterp, const int* i_dp, const int* i_dl, const int* i_sp,
                        const int* i_sl, const double* i_fx, const double* i_fy, int nedge, const int* e_p,
@@ -719,6 +745,27 @@ int enl_batch_set_fill(void* handle, int level, int ncopy, const int* c_dp, cons
                        const int* k_s, const int* k_n) {
   Batch* b = (Batch*)handle;
   if (!b || level < 0 || level >= 8) return set_error(-2, "enl_batch_set_fill: bad level");
+  if (ncopy < 0 || ninterp < 0 || nedge < 0 || ncorner < 0 || null_list(ncopy, c_dp, c_dl, c_sp, c_sl) ||
+      null_list(ninterp, i_dp, i_dl, i_sp, i_sl) || (ninterp > 0 && !(i_fx && i_fy)) ||
+      null_list(nedge, e_p, e_d, e_s, e_n) || null_list(ncorner, k_p, k_d, k_s, k_n))
+    return set_error(-3, "enl_batch_set_fill: bad item lists");
+  const std::vector<PatchDev>& hp = b->hp;
+  for (int q = 0; q < ncopy; q++)
+    if (!cell_ok(hp, c_dp[q], c_dl[q]) || !cell_ok(hp, c_sp[q], c_sl[q]) || hp[c_dp[q]].level != level ||
+        hp[c_sp[q]].level != level)
+      return set_error(-2, "enl_batch_set_fill: level %d copy item %d out of range", level, q);
+  for (int q = 0; q < ninterp; q++)
+    if (!cell_ok(hp, i_dp[q], i_dl[q]) || !cell_ok(hp, i_sp[q], i_sl[q]) || hp[i_dp[q]].level != level ||
+        hp[i_sp[q]].level != level - 1 || !stencil_ok(hp[i_sp[q]], i_sl[q]))
+      return set_error(-2, "enl_batch_set_fill: level %d refine item %d out of range", level, q);
+  for (int q = 0; q < nedge + ncorner; q++) {
+    const bool edge = q < nedge;
+    const int r = edge ? q : q - nedge;
+    const int pp = edge ? e_p[r] : k_p[r], side = edge ? e_n[r] : k_n[r];
+    if (!cell_ok(hp, pp, edge ? e_d[r] : k_d[r]) || !cell_ok(hp, pp, edge ? e_s[r] : k_s[r]) || side < 1 ||
+        side > 4 || hp[pp].level != level)
+      return set_error(-2, "enl_batch_set_fill: level %d %s item %d out of range", level, edge ? "edge" : "corner", r);
+  }
   LevelFill& L = b->fill[level];
   L.ncopy = ncopy;
   L.ninterp = ninterp;
@@ -749,6 +796,11 @@ int enl_batch_set_fill(void* handle, int level, int ncopy, const int* c_dp, cons
 int enl_batch_set_coarsen(void* handle, int n, const int* cp, const int* cl, const int* fp, const int* fl) {
   Batch* b = (Batch*)handle;
   if (!b) return set_error(-2, "null batch");
+  if (n < 0 || null_list(n, cp, cl, fp, fl)) return set_error(-3, "enl_batch_set_coarsen: bad item lists");
+  for (int q = 0; q < n; q++)
+    if (!cell_ok(b->hp, cp[q], cl[q]) || !cell_ok(b->hp, fp[q], fl[q]) ||
+        b->hp[fp[q]].level != b->hp[cp[q]].level + 1 || !block_ok(b->hp[fp[q]], fl[q], b->ratio))
+      return set_error(-2, "enl_batch_set_coarsen: item %d out of range", q);
   b->ncoarsen = n;
   int rc = 0;
   rc = rc ? rc : upload(b, &b->co_cp, cp, n);
@@ -841,6 +893,30 @@ int enl_batch_prepare(void* handle, const int* reflect) {
     if (e != hipSuccess) return check_hip(e, "source interior copy");
   }
   return check_hip(hipGetLastError(), "enl_batch_prepare");
+}
+
+namespace {
+// Zeroes, for patch blockIdx.y, its ghosted source and the interior copy
+// that follows it in the pool (and the alignment padding between them).
+__global__ void __launch_bounds__(256) zero_source_kernel(double* pool, const PatchDev* __restrict__ dp) {
+  const PatchDev pd = dp[blockIdx.y];
+  const long b = pd.src, e = pd.src_int + (long)pd.nx * pd.ny;
+  for (long q = b + blockIdx.x * 256L + threadIdx.x; q < e; q += (long)gridDim.x * 256L) pool[q] = 0.0;
+}
+}  // namespace
+
+int enl_batch_zero_source(void* handle) {
+  // enlightning::setSourceToZero (enlightning.C:2310-2329), called by the
+  // main loop after a step that added a source (main.C:476-480).  A kernel,
+  // not a memset, so the call is also safe inside a captured graph.
+  Batch* b = (Batch*)handle;
+  if (!b) return set_error(-2, "null batch");
+  if (b->npatch > 65535) return set_error(-2, "enl_batch_zero_source: more than 65535 patches");
+  long most = 0;
+  for (const PatchDev& pd : b->hp) most = std::max(most, pd.src_int + (long)pd.nx * pd.ny - pd.src);
+  const unsigned bx = (unsigned)std::min<long>((most + 255) / 256, 64);
+  zero_source_kernel<<<dim3(bx, b->npatch), 256, 0, lib_stream()>>>(b->pool, b->dp);
+  return check_hip(hipGetLastError(), "enl_batch_zero_source");
 }
 
 int enl_batch_fill(void* handle, const int* reflect) {
@@ -965,6 +1041,9 @@ int enl_batch_set_absorb(void* handle, const double* grid_xlo, const double* gri
   if (!b) return set_error(-2, "null batch");
   // queued steps may still read the old tables
   if (b->ab_rects || b->ab_tab) (void)hipStreamSynchronize(lib_stream());
+  // a captured step holds the old table pointers and per-side grid sizes:
+  // it is re-captured on the next graph advance
+  drop_graph(b);
   if (b->ab_rects) {
     (void)hipFree(b->ab_rects);
     b->ab_rects = nullptr;
@@ -1053,16 +1132,15 @@ int enl_batch_transfer(void* new_handle, void* old_handle, const int* reflect, i
   for (int q = 0; q < ncopy; q++) {
     if (c_dp[q] < 0 || c_dp[q] >= nb->npatch || c_sp[q] < 0 || c_sp[q] >= ob->npatch)
       return set_error(-2, "enl_batch_transfer: copy item %d names a bad patch", q);
-    const PatchDev &D = nb->hp[c_dp[q]], &S = ob->hp[c_sp[q]];
-    if (D.level != S.level || c_dl[q] < 0 || c_dl[q] >= D.sk || c_sl[q] < 0 || c_sl[q] >= S.sk)
+    if (nb->hp[c_dp[q]].level != ob->hp[c_sp[q]].level || !cell_ok(nb->hp, c_dp[q], c_dl[q]) ||
+        !cell_ok(ob->hp, c_sp[q], c_sl[q]))
       return set_error(-2, "enl_batch_transfer: copy item %d out of range", q);
   }
   for (int q = 0; q < ninterp; q++) {
     if (i_dp[q] < 0 || i_dp[q] >= nb->npatch || i_sp[q] < 0 || i_sp[q] >= ob->npatch)
       return set_error(-2, "enl_batch_transfer: refine item %d names a bad patch", q);
     const PatchDev &D = nb->hp[i_dp[q]], &S = ob->hp[i_sp[q]];
-    const long sl = i_sl[q];
-    if (S.level != D.level - 1 || i_dl[q] < 0 || i_dl[q] >= D.sk || sl - S.sy - 1 < 0 || sl + S.sy + 1 >= S.sk)
+    if (S.level != D.level - 1 || !cell_ok(nb->hp, i_dp[q], i_dl[q]) || !stencil_ok(S, i_sl[q]))
       return set_error(-2, "enl_batch_transfer: refine item %d out of range", q);
   }
   hipStream_t s = lib_stream();

@@ -223,7 +223,12 @@ class DeviceHierarchy:
         enl_add_source writes every patch's source (src: a source.Source),
         then enl_batch_prepare refills the source's ghosts and the interior
         copy the drop-in mode reads.  Call it before each step, as the
-        reference does for time-dependent sources."""
+        reference does for time-dependent sources.
+
+        The reference's order per iteration (main.C:424-480): if
+        source.should_add_source(...) then add_source, advance one step,
+        then zero_source; a PULSE source is therefore felt for one step only
+        and a LIGHTNING channel merges into a zeroed source each time."""
         st = src.to_struct()
         D2 = ctypes.c_double * 2
         for p in self.h.patches:
@@ -235,6 +240,12 @@ class DeviceHierarchy:
                                                D2(dx, dx), ctypes.byref(st), int(iteration), int(pulse_time0),
                                                float(dt), interior, p.Nx))
         _lib.check(self.lib.enl_batch_prepare(self.handle, self._reflect()))
+
+    def zero_source(self):
+        """enlightning::setSourceToZero (enlightning.C:2310-2329): every
+        patch's source, ghosts and interior copy, set to 0 on the device
+        (enl_batch_zero_source), after a step that added a source."""
+        _lib.check(self.lib.enl_batch_zero_source(self.handle))
 
     def sample_mics(self, mic_x, mic_y):
         """recordAudio's pressure reads (enlightning.C:2392-2506) over the

@@ -44,6 +44,18 @@ class Source:
         return s
 
 
+def should_add_source(src, iteration, pulse_times):
+    """enlightning::shouldAddSource (enlightning.C:2118-2135), evaluated
+    before the main loop increments the iteration number: a DRIVEN source is
+    drawn on every iteration from its first pulse time on, a PULSE source on
+    the iterations with iteration + 1 equal to one of its pulse times."""
+    if src.mode == DRIVEN:
+        return iteration >= pulse_times[0]
+    if src.mode == PULSE:
+        return any(iteration + 1 == t for t in pulse_times)
+    return False
+
+
 def add_source(box, patch_xlo, dx, src, iteration, pulse_time0, dt, source, sy=None):
     """Writes the patch's mass source into the device array `source` (interior
     cells, row stride sy; default the patch width)."""

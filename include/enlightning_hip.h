@@ -269,6 +269,11 @@ int enl_batch_copy_state(void *handle, int pid, int to_device, double *w_n, doub
                          double *source);
 /* Fills the source ghosts (constant over a run) and its interior copy. */
 int enl_batch_prepare(void *handle, const int *reflect);
+/* Zeroes every patch's source (ghosts and interior copy), as
+ * enlightning::setSourceToZero (enlightning.C:2310-2329) does after a step
+ * that added a source (main.C:476-480).  A fill kernel on the library
+ * stream (no memset node), so it may also be captured into a graph. */
+int enl_batch_zero_source(void *handle);
 /* Ghost fill of the current state of every level. */
 int enl_batch_fill(void *handle, const int *reflect);
 /* pres, temp of every patch interior from w_n (update_state.m4). */

@@ -82,6 +82,12 @@ class HostHierarchy:
         for k, s in enumerate(self.src):
             s[...] = flat[off[k]:off[k] + s.size].reshape(s.shape)
 
+    def zero_source(self):
+        """enlightning::setSourceToZero (enlightning.C:2310-2329): the whole
+        source of every patch set to 0."""
+        for s in self.src:
+            s[...] = 0.0
+
     def fill_source(self):
         """Ghost fill of the source (one component, nothing negated at the
         walls), level by level, with the state's schedule."""

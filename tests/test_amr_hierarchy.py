@@ -50,6 +50,45 @@ def test_fine_boxes(hier):
     assert (cover[4 * jj, 4 * ii] == 1).all()
 
 
+@pytest.mark.parametrize("n0,tile0", [(65, 64), (65, 16), (66, 16), (67, 16), (97, 32), (70, 64)])
+def test_odd_base_sizes_build_without_overlap(n0, tile0):
+    """Base sizes that leave a remainder block: n0 = 65 with tags at coarse
+    columns 62 and 64 (blocks of 16 coarse cells end at 63, the last block
+    is column 64 alone), plus random tag maps.  The fine boxes stay inside
+    their blocks (no overlap), lie in the domain, and build() succeeds (its
+    schedule asserts that no corner block straddles the domain edge)."""
+    g, r = 4, 4
+    rng = np.random.default_rng(n0 * 100 + tile0)
+    maps = []
+    t = np.zeros((n0, n0), np.int32)
+    t[40:42, 62] = 1
+    t[40:42, 64] = 1
+    maps.append((t, 0))
+    for _ in range(3):
+        maps.append(((rng.random((n0, n0)) < 0.01).astype(np.int32), int(rng.integers(0, 3))))
+    t = np.zeros((n0, n0), np.int32)
+    t[n0 - 2:, :] = 1  # along the top edge and the last row of blocks
+    t[:, n0 - 1] = 1
+    maps.append((t, 1))
+    for tags, buf in maps:
+        h = amr.build(n0, r, g, 1.0, tags, tile0=tile0, tag_buffer=buf)
+        n1 = h.ncells(1) if h.nlevels > 1 else 0
+        cover = np.zeros((n1, n1), np.int32)
+        for p in h.level_patches(1):
+            i0, i1, j0, j1 = p.box
+            assert 0 <= i0 <= i1 < n1 and 0 <= j0 <= j1 < n1
+            cover[j0:j1 + 1, i0:i1 + 1] += 1
+        assert cover.max(initial=0) <= 1
+        if buf == 0 and h.nlevels > 1:
+            jj, ii = np.nonzero(tags)
+            assert (cover[r * jj, r * ii] == 1).all()
+        base = np.zeros((n0, n0), np.int32)
+        for p in h.level_patches(0):
+            assert p.nx >= g and p.ny >= g
+            base[p.box[2]:p.box[3] + 1, p.box[0]:p.box[1] + 1] += 1
+        assert (base == 1).all()
+
+
 def _linear_states(h, a, bx, by):
     """Per patch: w_k = a_k + bx_k*x + by_k*y on all ghosted cell centres."""
     states = []

@@ -175,6 +175,90 @@ def test_absorbing_sides_batched():
             assert np.abs(a - b).max() <= 1e-10 * max(np.abs(b).max(), 1e-6 * natural[k]), (p.pid, k)
 
 
+def test_set_absorb_again_recaptures_the_graph():
+    """enl_batch_set_absorb with a new width drops the captured step (its
+    nodes hold the old envelope tables): graph steps after the change equal
+    fused steps with the new width from the same state, bit for bit."""
+    gas, h = make(WL64, 16)
+    dt = dt_of(h, gas)
+    refl = (0, 0, 1, 0)
+    states = [patch_initial_state(h, p, WL64, gas) for p in h.patches]
+    a = D.Hierarchy(h, gas, reflect=refl)
+    a.set_absorb(0.08)
+    a.load_initial(WL64, states=states)
+    a.advance(2, dt, mode="graph")
+    mid, mid_src = a.states(), a.sources()
+    a.set_absorb(0.2)
+    a.advance(2, dt, mode="graph")
+    b = D.Hierarchy(h, gas, reflect=refl)
+    b.set_absorb(0.2)
+    b.upload_all(mid, mid_src)
+    b.advance(2, dt, mode="fused")
+    assert same(a.states(), b.states(), h)
+    c = D.Hierarchy(h, gas, reflect=refl)  # the old width gives another state
+    c.set_absorb(0.08)
+    c.upload_all(mid, mid_src)
+    c.advance(2, dt, mode="fused")
+    assert not same(a.states(), c.states(), h)
+    for d in (a, b, c):
+        d.close()
+
+
+def test_pulse_source_zeroed_after_its_step():
+    """The reference's source sequence (main.C:424-480): on the iterations
+    shouldAddSource picks (PULSE: iteration + 1 in the pulse times,
+    enlightning.C:2118-2135) addSource writes the source, one step runs, and
+    setSourceToZero clears it.  A PULSE LIGHTNING channel over six
+    iterations (pulses at 1 and 4) on the device (add_source, graph step,
+    zero_source) follows the CPU twin doing the same to 1e-10 per field, the
+    source is zero after every step, and without the zeroing the state
+    differs (the channel would keep injecting mass)."""
+    from enlightning_amd.source import LIGHTNING, PULSE, Source, should_add_source
+    gas, h = make(WL64, 16)
+    dt = dt_of(h, gas)
+    xs = np.linspace(0.3, 0.7, 6)
+    ys = 0.5 + 0.05 * np.sin(7 * xs)
+    coords = np.stack([xs[:-1], ys[:-1], xs[1:], ys[1:]], axis=1)
+    src = Source(mode=PULSE, type=LIGHTNING, amp=3.0e3, spread=0.02, coords=coords)
+    pulses = (1, 4)
+    states = [patch_initial_state(h, p, WL64, gas) for p in h.patches]
+    cpu = HostHierarchy(h, gas)
+    cpu.load_initial(states)
+    dev, keep = D.Hierarchy(h, gas), D.Hierarchy(h, gas)
+    dev.load_initial(WL64, states=states)
+    keep.load_initial(WL64, states=states)
+    natural = (gas.rho0, gas.rho0 * gas.c, gas.rho0 * gas.c, gas.rho0 * gas.R, gas.rho0 * gas.T0)
+    added = 0
+    for it in range(6):
+        add = should_add_source(src, it, pulses)
+        added += add
+        if add:
+            for k, p in enumerate(h.patches):
+                dx = h.dx(p.level)
+                tmp = np.ascontiguousarray(cpu.src[k][p.g:-p.g, p.g:-p.g])
+                oracle.add_source(p.box, (p.box[0] * dx, p.box[2] * dx), (dx, dx), src, it, 0, dt, tmp)
+                cpu.src[k][p.g:-p.g, p.g:-p.g] = tmp
+            cpu.fill_source()
+            dev.add_source(src, it, 0, dt)
+            keep.add_source(src, it, 0, dt)
+        cpu.advance(1, dt, RK3)
+        dev.advance(1, dt, mode="graph")
+        keep.advance(1, dt, mode="graph")
+        if add:
+            cpu.zero_source()
+            dev.zero_source()
+            assert all(not np.any(s) for s in dev.sources())
+        for (wc, _, _), (wd, _, _), p in zip(cpu.states(), dev.states(), h.patches):
+            g = p.g
+            for k in range(5):
+                a, b = wd[k, g:-g, g:-g], wc[k, g:-g, g:-g]
+                assert np.abs(a - b).max() <= 1e-10 * max(np.abs(b).max(), 1e-6 * natural[k]), (it, p.pid, k)
+    assert added == 2
+    assert not same(dev.states(), keep.states(), h)
+    dev.close()
+    keep.close()
+
+
 def test_time_dependent_source_batched():
     """A DRIVEN point source (enlightning.C:2140-2307) written into every
     patch before each step (DeviceHierarchy.add_source: enl_add_source on the
