@@ -57,6 +57,7 @@ PROTOTYPES = {
     "rhs_weno_low_": (_RHS_ARGS, None),
     "rhs_drp_": (_RHS_ARGS, None),
     "enl_rhs_variant": ([_I] + _RHS_ARGS + [_P], _I),
+    "enl_rhs_variant_noalloc": ([_I] + _RHS_ARGS + [_P], _I),
     "runge_kutta_": ([_P] * 14, None),
     "update_state_": ([_P] * 23, None),
     "tag_cells_": ([_P] * 12, None),

@@ -16,6 +16,14 @@
 
 #pragma clang fp contract(off)
 
+// The WENO list of enl_rhs_variant_noalloc, which allocates nothing:
+// kNoAllocMaxCells list entries, then the list counter in the next slot.
+// Every GPU holds its own copy until the process ends.  In host code the
+// name `scratch` is a host-side stand-in, not a device address: host code
+// uses only the address hipGetSymbolAddress returns for the current device.
+static constexpr long kNoAllocMaxCells = 1L << 22;  // 32 MiB: a 2048 x 2048 patch
+static __device__ __align__(256) unsigned long long scratch[kNoAllocMaxCells + 32];
+
 namespace enl {
 
 namespace {
@@ -93,6 +101,27 @@ int rhs_workspace(long cells, unsigned long long** list, unsigned** count) {
 }
 
 namespace {
+// Per-device address of `scratch`, looked up on the first call on a device.
+std::mutex g_na_mu;
+void* g_na_ptr[64] = {nullptr};
+
+int noalloc_workspace(long cells, unsigned long long** list, unsigned** count) {
+  if (cells > kNoAllocMaxCells)
+    return set_error(-2, "enl_rhs_variant_noalloc: %ld cells exceed its fixed workspace of %ld cells", cells,
+                     kNoAllocMaxCells);
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return set_error(-5, "hipGetDevice");
+  std::lock_guard<std::mutex> lock(g_na_mu);
+  if (!g_na_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return check_hip(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+    g_na_ptr[dev] = p;
+  }
+  *list = (unsigned long long*)g_na_ptr[dev];
+  *count = (unsigned*)(*list + kNoAllocMaxCells);
+  return 0;
+}
 
 // Device mirrors for host-pointer calls.  Grow-only buffers, one per argument
 // slot, guarded by one mutex (host-pointer calls are serialized; device-pointer
@@ -191,16 +220,22 @@ int do_rhs(int variant, const int* ifirst0, const int* ilast0, const int* ifirst
            const double* source, double* rhs, const double* pres, const double* temp, const double* weno_alpha,
            const double* kappa, const double* mu, const double* muB, const double* R, const double* T_star_n,
            const double* T_star_o, const double* tau_n, const double* tau_o, const double* tag_tolerance,
-           const int* level, const double* c, const double* c_lr, const int* nequ, int* switch_map) {
+           const int* level, const double* c, const double* c_lr, const int* nequ, int* switch_map,
+           bool noalloc = false) {
   (void)T_star_o;
   (void)tau_o;
   if (variant < 0 || variant >= kNumVariants) return set_error(-1, "unknown rhs variant %d", variant);
   int rc = check_box(*ifirst0, *ilast0, *ifirst1, *ilast1, *gcw0, *gcw1, ghost_need(variant), *nequ);
   if (rc) return rc;
   Geom G = make_geom(*ifirst0, *ilast0, *ifirst1, *ilast1, *gcw0, *gcw1);
+  if (noalloc && (long)G.nx * G.ny > kNoAllocMaxCells)
+    return set_error(-2, "enl_rhs_variant_noalloc: %ld cells exceed its fixed workspace of %ld cells",
+                     (long)G.nx * G.ny, kNoAllocMaxCells);
   if (G.nx == 0 || G.ny == 0) return 0;
   size_t ng = (size_t)G.sk * sizeof(double), ni = (size_t)G.nx * G.ny * sizeof(double);
   const void* ptrs[6] = {w_n, source, rhs, pres, temp, switch_map};
+  if (noalloc && any_host(ptrs, 6))
+    return set_error(-3, "enl_rhs_variant_noalloc: every array must be device memory (host arrays need staging)");
   std::unique_lock<std::mutex> lock(staging().mu, std::defer_lock);
   if (any_host(ptrs, 6)) lock.lock();
   Arg args[6];
@@ -235,7 +270,10 @@ int do_rhs(int variant, const int* ifirst0, const int* ilast0, const int* ifirst
   a.level = *level;
   a.list = nullptr;
   a.count = nullptr;
-  if (rhs_needs_list(variant, a.level) && (rc = rhs_workspace((long)G.nx * G.ny, &a.list, &a.count))) return rc;
+  if (rhs_needs_list(variant, a.level) &&
+      (rc = noalloc ? noalloc_workspace((long)G.nx * G.ny, &a.list, &a.count)
+                    : rhs_workspace((long)G.nx * G.ny, &a.list, &a.count)))
+    return rc;
   if ((rc = check_hip(launch_rhs(variant, a, g_stream), "rhs launch"))) return rc;
   return finish(args, switch_map ? 6 : 5);
 }
@@ -271,6 +309,10 @@ extern "C" {
 
 int enl_rhs_variant(int variant, ENL_RHS_ARGS, int* switch_map) {
   return do_rhs(variant, ENL_RHS_FWD, switch_map);
+}
+
+int enl_rhs_variant_noalloc(int variant, ENL_RHS_ARGS, int* switch_map) {
+  return do_rhs(variant, ENL_RHS_FWD, switch_map, true);
 }
 
 void rhs_(ENL_RHS_ARGS) { (void)do_rhs(default_variant(), ENL_RHS_FWD, nullptr); }

@@ -83,6 +83,17 @@ void rhs_drp_(ENL_RHS_ARGS);             /* fortran/rhs_drp.m4:26-49 */
  * (rhs_hybrid_high_llf.m4:239-258).  Returns 0 or a negative error code. */
 int enl_rhs_variant(int variant, ENL_RHS_ARGS, int *switch_map);
 
+/* enl_rhs_variant without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * array must be device memory (host arrays would need staging buffers).
+ * The level-0 hybrid variants' WENO list lives in a fixed per-GPU array of
+ * 2^22 cells (32 MiB, held until the process ends); a patch of more interior
+ * cells is rejected with an error.  Same kernels, same results as
+ * enl_rhs_variant, bit for bit.  All calls on one GPU share its list: calls
+ * that overlap on one GPU (other streams, other host threads) race. */
+int enl_rhs_variant_noalloc(int variant, ENL_RHS_ARGS, int *switch_map);
+
 /* runge_kutta: replaces SAMRAI_F77_FUNC(runge_kutta,RUNGE_KUTTA),
  * enlightningfort.h:51-64, called at enlightning.C:538-551.
  * w_n(interior) = alpha_1*w + alpha_2*w_n + beta*dt*rhs; ghosts untouched. */
