@@ -45,6 +45,9 @@ struct RhsArgs {
   double dx, dy, ylo;
   double alpha, kappa, mu, muB, R, T_star_n, tau_n, c, c_lr;
   double ep;  // sensor epsilon from tag_tolerance(2)
+  // 5e-5 * ep when ep > 0, else -1: a cell whose eight sensor differences
+  // have squares summing below it is DRP (filled by fill_rhs_constants)
+  double ep_pre;
   int level;
   // WENO-cell list of the split hybrid path (nx*ny ints) and its length;
   // needed when rhs_needs_list(variant, level)

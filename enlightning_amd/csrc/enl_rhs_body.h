@@ -71,6 +71,7 @@ __host__ __device__ inline void fill_rhs_constants(RhsArgs& a) {
   a.fast_div = den_ok(a.tau_n) && num_ok(a.T_star_n) && num_ok(a.mu) && num_ok(a.muB) && num_ok(a.kappa);
   a.eos_k1 = (a.gamma1 - 1) / (2 * a.rho0);
   a.eos_ok = num_ok(a.rho0) && den_ok(a.c_p) && den_ok(a.c_v);
+  a.ep_pre = a.ep > 0 ? 5e-5 * a.ep : -1.0;
 }
 
 // a / c for a grid constant c; a * (1/c) when 1/c is exact (POW2).
@@ -167,9 +168,9 @@ __device__ inline void cell_output(const RhsArgs& a, int i, int j, const double 
 
 // Physical source H, rhs_hybrid_high_llf.m4:159-237.  The thirteen
 // quotients by tau_n, T_n, temp, 2*temp and temp^2 are formed as n / d
-// (NR false) or through one reciprocal per denominator (NR true: rcp_nr /
-// div_nr, the same bits where every operand passes den_ok / num_ok); the
-// NR form returns false, its H to be discarded, where one does not.
+// (NR false) or as n times one reciprocal per denominator (NR true: n *
+// rcp_nr(d), within 1.5 ulp of n / d where d passes den_ok); the NR form
+// returns false, its H to be discarded, where T_n or temp^2 does not.
 template <int S, bool P, bool NR>
 __device__ inline bool cell_source_q(const Tile& t, int e, double H0, const RhsArgs& a, double H[5]) {
 #pragma clang fp contract(fast)
@@ -211,15 +212,16 @@ __device__ inline bool cell_source_q(const Tile& t, int e, double H0, const RhsA
   bool ok = true;
   double rtau = 0, rTn = 0, rT = 0, r2T = 0, rT2 = 0;
   if constexpr (NR) {
-    ok = den_ok(temp2) && den_ok(T_n) && num_ok(w0) && num_ok(temp - T_n) && num_ok(nx1) && num_ok(nx2) &&
-         num_ok(ny1) && num_ok(ny2);
+    // |temp^2| in [2^-50, 2^50] puts temp and 2*temp in range too; tau_n
+    // passed den_ok on the host (fast_div)
+    ok = den_ok(temp2) && den_ok(T_n);
     rtau = rcp_nr(a.tau_n);
     rTn = rcp_nr(T_n);
     rT = rcp_nr(temp);
     r2T = rcp_nr(2 * temp);
     rT2 = rcp_nr(temp2);
   }
-  auto dv = [](double n, double d, double r) { return NR ? div_nr(n, d, r) : n / d; };
+  auto dv = [](double n, double d, double r) { return NR ? n * r : n / d; };
   double Mn = dv(temp - T_n, a.tau_n, rtau);
   double tmp1 = dv(a.T_star_n, T_n, rTn);
   double c_vn = kC078 * a.R * sq(tmp1) * exp(-tmp1);
@@ -260,17 +262,35 @@ __device__ inline bool ratio_below(double tmp1, double tmp2, double ep) {
   return num / den < kThresh;
 }
 
-// Hybrid density sensor, rhs_hybrid_high_llf.m4:239-256.  1 = WENO.
+// Hybrid density sensor, rhs_hybrid_high_llf.m4:239-256.  1 = WENO.  The
+// triples of q = -1, 0, 1 share four density differences per direction:
+// d[m] = rho(i-1+m) - rho(i-2+m), the triple of q being (d[q+1], d[q+2]),
+// the reference's tmp1, tmp2 (same subtraction, same bits).  When the
+// squares of all eight sum below ep_pre = 5e-5 * ep (ep > 0), every ratio
+// (2|t1 t2| + ep) / (t1^2 + t2^2 + ep) is at least 1 / (1 + 5e-5), rounding
+// included, far above float(0.9999): the cell is DRP and the six ratios
+// need not be formed.  Otherwise they decide as in the reference.
 template <int S>
-__device__ inline int cell_sensor(const double* r, int e, double ep, int level) {
-  int tag = 1;
-  if (level == 0) {
-    tag = 0;
+__device__ inline int cell_sensor(const double* r, int e, double ep, double ep_pre, int level) {
+  if (level != 0) return 1;
+  double dx[4], dy[4];
 #pragma unroll
-    for (int q = -1; q <= 1; q++) {
-      if (ratio_below(r[e + q] - r[e - 1 + q], r[e + 1 + q] - r[e + q], ep)) tag = 1;
-      if (ratio_below(r[e + q * S] - r[e + (q - 1) * S], r[e + (q + 1) * S] - r[e + q * S], ep)) tag = 1;
-    }
+  for (int m = 0; m < 4; m++) {
+    dx[m] = r[e - 1 + m] - r[e - 2 + m];
+    dy[m] = r[e + (m - 1) * S] - r[e + (m - 2) * S];
+  }
+  {
+#pragma clang fp contract(fast)
+    double s = 0;
+#pragma unroll
+    for (int m = 0; m < 4; m++) s += dx[m] * dx[m] + dy[m] * dy[m];
+    if (s < ep_pre) return 0;
+  }
+  int tag = 0;
+#pragma unroll
+  for (int q = -1; q <= 1; q++) {
+    if (ratio_below(dx[q + 1], dx[q + 2], ep)) tag = 1;
+    if (ratio_below(dy[q + 1], dy[q + 2], ep)) tag = 1;
   }
   return tag;
 }
@@ -346,10 +366,11 @@ __device__ inline void llf_alpha(const double* u, const double* v, int j, const 
   }
 }
 
-// WENO branch of one cell out of an LDS tile that holds the raw w_k.
+// WENO branch of one cell out of an LDS tile that holds the raw w_k.  hk
+// holds the physical source H on entry and the cell's rhs on return (the
+// source is added last, as in the reference's sum).
 template <int V, int S, bool P>
-__device__ inline void cell_weno_tile(const Tile& t, int e, int j, const RhsArgs& a, const double H[5],
-                                      double out[5]) {
+__device__ inline void cell_weno_tile(const Tile& t, int e, int j, const RhsArgs& a, double hk[5]) {
 #pragma clang fp contract(fast)
   using Sc = Scheme<V>;
   constexpr int R = Sc::order == 5 ? 3 : 2;
@@ -366,63 +387,99 @@ __device__ inline void cell_weno_tile(const Tile& t, int e, int j, const RhsArgs
     af = a.alpha;
     ag = a.alpha;
   }
-#pragma unroll
+  // one component at a time (a rolled loop: k is the same for every lane,
+  // so the flux switch and the selects below are scalar), the x direction
+  // finished before the y direction's loads: only one direction's stencil
+  // and split fluxes are live at once
+#pragma unroll 1
   for (int k = 0; k < 5; k++) {
-    double F[7], Wx[7], G[7], Wy[7];
+    const double* wk = k == 0 ? t.w[0] : k == 1 ? t.w[1] : k == 2 ? t.w[2] : k == 3 ? t.w[3] : t.w[4];
+    double x, y;
+    {
+      double F[7], W[7];
 #pragma unroll
-    for (int m = -R; m <= R; m++) {
-      F[m + 3] = tflux_f(t, e + m, k);
-      Wx[m + 3] = t.w[k][e + m];
-      G[m + 3] = tflux_g(t, e + m * S, k);
-      Wy[m + 3] = t.w[k][e + m * S];
+      for (int m = -R; m <= R; m++) {
+        F[m + 3] = tflux_f(t, e + m, k);
+        W[m + 3] = wk[e + m];
+      }
+      x = weno_dir<Sc::order>(F, W, af);
     }
-    double x = weno_dir<Sc::order>(F, Wx, af);
-    double y = weno_dir<Sc::order>(G, Wy, ag);
-    out[k] = divc<P>(x, a.dx, a.inv_dx) + divc<P>(y, a.dy, a.inv_dy) + H[k];
+    __builtin_amdgcn_sched_barrier(0);
+    {
+      double G[7], W[7];
+#pragma unroll
+      for (int m = -R; m <= R; m++) {
+        G[m + 3] = tflux_g(t, e + m * S, k);
+        W[m + 3] = wk[e + m * S];
+      }
+      y = weno_dir<Sc::order>(G, W, ag);
+    }
+    const double h = k == 0 ? hk[0] : k == 1 ? hk[1] : k == 2 ? hk[2] : k == 3 ? hk[3] : hk[4];
+    const double o = divc<P>(x, a.dx, a.inv_dx) + divc<P>(y, a.dy, a.inv_dy) + h;
+    hk[0] = k == 0 ? o : hk[0];
+    hk[1] = k == 1 ? o : hk[1];
+    hk[2] = k == 2 ? o : hk[2];
+    hk[3] = k == 3 ? o : hk[3];
+    hk[4] = k == 4 ? o : hk[4];
   }
 }
 
-// Stage rho, p, T and the quotients (and the raw w_k if RAW) of the tile +
-// halo.  Cells outside the ghosted array are skipped; no interior cell of
-// the patch reads them.
-template <int BX, int BY, bool RAW>
-__device__ inline void stage_tile(const RhsArgs& a, int i0, int j0, TileLds<BX, BY>& L) {
+// One thread's share of the staging loads of a tile + halo (stage_load),
+// held in registers until stage_store writes them to LDS.
+template <int BX, int BY>
+struct StageRegs {
+  static constexpr int kIt = (TileShape<BX, BY>::TN + TileShape<BX, BY>::threads - 1) / TileShape<BX, BY>::threads;
+  double v[kIt][7];  // w_n(5), p, T
+  bool in[kIt];
+};
+
+// The global loads of the tile + halo at (i0, j0), all issued before any is
+// used.  A cell outside the tile or the ghost box reads element 0 (always in
+// the arrays) and is marked not to be stored; no interior cell of the patch
+// reads it.
+// Loads slots [IT0, IT1) of the thread's share (all of them by default).
+template <int BX, int BY, int IT0 = 0, int IT1 = StageRegs<BX, BY>::kIt>
+__device__ inline void stage_load(const RhsArgs& a, int i0, int j0, StageRegs<BX, BY>& R) {
   using Sh = TileShape<BX, BY>;
-  constexpr int kIt = (Sh::TN + Sh::threads - 1) / Sh::threads;
   const Geom& G = a.geom;
   const int tid = threadIdx.x + threadIdx.y * BX;
-  double *s0 = L.w[0], *sp = L.p, *sT = L.T, *su = L.u, *sv = L.v, *ss = L.s, *stn = L.tn;
-  // Every load of the thread's staged cells is issued before the first is
-  // used: a cell outside the tile or the ghost box reads element 0 (always
-  // in the arrays) and is not stored.
-  double v[kIt][7];
-  bool in[kIt];
   // the whole tile and halo inside the ghost box (every tile of a grid
   // whose sides are multiples of the tile's, with g >= halo): no per-cell
   // bounds test
   const bool inside = i0 - Sh::halo >= -G.g0 && i0 + Sh::TW - Sh::halo <= G.nx + G.g0 &&
                       j0 - Sh::halo >= -G.g1 && j0 + Sh::TH - Sh::halo <= G.ny + G.g1;
 #pragma unroll
-  for (int it = 0; it < kIt; it++) {
+  for (int it = IT0; it < IT1; it++) {
     const int e = tid + it * Sh::threads;
     const int gi = i0 + (e % Sh::TW) - Sh::halo, gj = j0 + (e / Sh::TW) - Sh::halo;
-    in[it] = e < Sh::TN && (inside || (gi >= -G.g0 && gi < G.nx + G.g0 && gj >= -G.g1 && gj < G.ny + G.g1));
-    const long gidx = in[it] ? (long)(gi + G.g0) + (long)(gj + G.g1) * G.sy : 0;
+    R.in[it] = e < Sh::TN && (inside || (gi >= -G.g0 && gi < G.nx + G.g0 && gj >= -G.g1 && gj < G.ny + G.g1));
+    const long gidx = R.in[it] ? (long)(gi + G.g0) + (long)(gj + G.g1) * G.sy : 0;
 #pragma unroll
-    for (int k = 0; k < 5; k++) v[it][k] = a.w[gidx + k * G.sk];
-    v[it][5] = a.p[gidx];
-    v[it][6] = a.T[gidx];
+    for (int k = 0; k < 5; k++) R.v[it][k] = a.w[gidx + k * G.sk];
+    R.v[it][5] = a.p[gidx];
+    R.v[it][6] = a.T[gidx];
   }
+}
+
+// rho, p, T and the quotients (and the raw w_k if RAW) of the loaded cells
+// into the LDS tile.
+template <int BX, int BY, bool RAW>
+__device__ inline void stage_store(const StageRegs<BX, BY>& R, TileLds<BX, BY>& L) {
+  using Sh = TileShape<BX, BY>;
+  const int tid = threadIdx.x + threadIdx.y * BX;
+  double *s0 = L.w[0], *sp = L.p, *sT = L.T, *su = L.u, *sv = L.v, *ss = L.s, *stn = L.tn;
+  const auto& v = R.v;
 #pragma unroll
-  for (int it = 0; it < kIt; it++) {
-    if (!in[it]) continue;
+  for (int it = 0; it < R.kIt; it++) {
+    if (!R.in[it]) continue;
     const int e = tid + it * Sh::threads;
     const double w0 = v[it][0], w1 = v[it][1], w2 = v[it][2], w3 = v[it][3], w4 = v[it][4];
     s0[e] = w0;
     sp[e] = v[it][5];
     sT[e] = v[it][6];
-    // one division per staged cell: the four quotients through 1/w0
-    const double iw0 = 1.0 / w0;
+    // the four quotients through one reciprocal of w0 (two Newton steps,
+    // within an ulp of 1/w0; the IEEE reciprocal outside den_ok's range)
+    const double iw0 = den_ok(w0) ? rcp_nr(w0) : 1.0 / w0;
     su[e] = w1 * iw0;
     sv[e] = w2 * iw0;
     ss[e] = w3 * iw0;
@@ -436,20 +493,56 @@ __device__ inline void stage_tile(const RhsArgs& a, int i0, int j0, TileLds<BX, 
   }
 }
 
+// Stage rho, p, T and the quotients (and the raw w_k if RAW) of the tile +
+// halo.  Cells outside the ghosted array are skipped.
+template <int BX, int BY, bool RAW>
+__device__ inline void stage_tile(const RhsArgs& a, int i0, int j0, TileLds<BX, BY>& L) {
+  StageRegs<BX, BY> R;
+  stage_load(a, i0, j0, R);
+  stage_store<BX, BY, RAW>(R, L);
+}
+
+// The per-cell inputs of a split tile that come from global memory besides
+// the staged tile: the source H0 and, where a switch was decided elsewhere,
+// its decision (2: the sensor decides).  Loaded apart so that a kernel can
+// issue them before prefetching its next tile.
+struct SplitIn {
+  double h0;
+  int d;
+};
+template <int BX, int BY>
+__device__ inline SplitIn split_inputs(const RhsArgs& a, int i0, int j0) {
+  const Geom& G = a.geom;
+  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+  SplitIn s = {0.0, 2};
+  if (i < G.nx && j < G.ny) {
+    s.h0 = a.H0[(long)i + (long)j * a.h0_sy];
+    if (a.decide) s.d = (int)a.decide[(long)i + (long)j * G.nx];
+  }
+  return s;
+}
 
 // Split tile: source H and sensor for every cell, DRP cells finished here,
 // WENO cells (sensor on level 0; every cell on finer levels, where the
 // reference switches the sensor off, and for rhs_weno_*) keep H in their rhs
 // slots and are appended to the list as (patch << 32 | cell): one atomic per
-// wave.
+// wave.  split_compute runs on a staged tile.
+template <int V, bool P, bool FUSED, int BX, int BY>
+__device__ inline void split_compute(const RhsArgs& a, int i0, int j0, unsigned pid, TileLds<BX, BY>& L,
+                                     const SplitIn& in);
 template <int V, bool P, bool FUSED, int BX, int BY>
 __device__ inline void split_body(const RhsArgs& a, int i0, int j0, unsigned pid, TileLds<BX, BY>& L) {
+  stage_tile<BX, BY, false>(a, i0, j0, L);
+  const SplitIn in = split_inputs<BX, BY>(a, i0, j0);
+  __syncthreads();
+  split_compute<V, P, FUSED, BX, BY>(a, i0, j0, pid, L, in);
+}
+template <int V, bool P, bool FUSED, int BX, int BY>
+__device__ inline void split_compute(const RhsArgs& a, int i0, int j0, unsigned pid, TileLds<BX, BY>& L,
+                                     const SplitIn& in) {
   using Sc = Scheme<V>;
   using Sh = TileShape<BX, BY>;
   const Geom& G = a.geom;
-  stage_tile<BX, BY, false>(a, i0, j0, L);
-  __syncthreads();
-
   const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
   const bool valid = i < G.nx && j < G.ny;
   int tag = 0;
@@ -466,21 +559,19 @@ __device__ inline void split_body(const RhsArgs& a, int i0, int j0, unsigned pid
     const long icell = (long)i + (long)j * G.nx;
     const long ik = (long)G.nx * G.ny;
     if constexpr (Sc::hybrid) {
-      const int d = a.decide ? (int)a.decide[icell] : 2;
-      tag = d == 2 ? cell_sensor<Sh::TW>(t.w0, e, a.ep, a.level) : d;
+      tag = in.d == 2 ? cell_sensor<Sh::TW>(t.w0, e, a.ep, a.ep_pre, a.level) : in.d;
     } else
       tag = Sc::drp ? 0 : 1;
     // Hybrid DRP cells: the stencil part first, so that H is not live
     // across it (the DRP-only variant keeps source then stencil)
     double out[5] = {0, 0, 0, 0, 0};
     double H[5];
-    const double* h0 = &a.H0[(long)i + (long)j * a.h0_sy];
     if constexpr (Sc::hybrid) {
       if (tag == 0) cell_drp<Sh::TW, P>(t, e, a, out);
       __builtin_amdgcn_sched_barrier(0);
-      cell_source<Sh::TW, P>(t, e, *h0, a, H);
+      cell_source<Sh::TW, P>(t, e, in.h0, a, H);
     } else {
-      cell_source<Sh::TW, P>(t, e, *h0, a, H);
+      cell_source<Sh::TW, P>(t, e, in.h0, a, H);
       if (tag == 0) cell_drp<Sh::TW, P>(t, e, a, out);
     }
     if (tag == 0) {
@@ -709,10 +800,10 @@ __device__ inline void dense_body(const RhsArgs& a, int i0, int j0, TileLds<BX, 
   t.s = L.s;
   t.tn = L.tn;
   const int e = (threadIdx.y + Sh::halo) * Sh::TW + threadIdx.x + Sh::halo;
-  double H[5], out[5];
-  cell_source<Sh::TW, P>(t, e, a.H0[(long)i + (long)j * a.h0_sy], a, H);
-  cell_weno_tile<V, Sh::TW, P>(t, e, j, a, H, out);
-  cell_output<FUSED>(a, i, j, out);
+  double K[5];
+  cell_source<Sh::TW, P>(t, e, a.H0[(long)i + (long)j * a.h0_sy], a, K);
+  cell_weno_tile<V, Sh::TW, P>(t, e, j, a, K);
+  cell_output<FUSED>(a, i, j, K);
   if (a.switch_map) a.switch_map[(long)i + (long)j * G.nx] = 1;
 }
 
