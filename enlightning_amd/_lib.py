@@ -87,12 +87,14 @@ PROTOTYPES = {
     "enl_switch_train_step": ([_P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float, _P], _I),
     "enl_switch_decide": ([_P, _I, _I, _I, _I, _I, _P, _P, ctypes.POINTER(EnlGas), ctypes.c_float, ctypes.c_float, _P,
                            _P], _I),
-    "enl_advance_patch_learned": ([_P, ctypes.c_float, ctypes.c_float, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P,
+    "enl_advance_patch_learned": ([_P, ctypes.c_float, ctypes.c_float, _I, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P,
                                    ctypes.POINTER(EnlGas), _P, _I, _P, _P, _P, ctypes.c_double, _I, _P, _P, _P, _P,
                                    _P, ctypes.c_size_t, _D], _I),
     "enl_batch_set_ambient": ([_P, _I, _I, _I, _P, _P, _P, ctypes.POINTER(EnlGas)], _I),
     "enl_batch_fp32_prepare": ([_P, _I, ctypes.POINTER(EnlGas), _P], _I),
     "enl_batch_fp32_load": ([_P], _I),
+    "enl_batch_fp32_set_switch": ([_P, _P, ctypes.c_float, ctypes.c_float, _I, _P], _I),
+    "enl_batch_fp32_decisions": ([_P, _I, _P], _I),
     "enl_batch_fp32_store": ([_P], _I),
     "enl_batch_patch_state": ([_P, _I] + [_P] * 4, _I),
     "enl_batch_copy_state": ([_P, _I, _I] + [_P] * 4, _I),

@@ -28,13 +28,16 @@
 // the destination row), so they are the FP64 form's operators.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <vector>
 
 #include "enl_batch.h"
 #include "enl_device.h"
 #include "enl_rhs_body.h"
+#include "enl_switch.h"
 
 #pragma clang fp contract(off)
 
@@ -46,6 +49,7 @@ constexpr int kRec = 8;  // ambient record: rho, rho*s, rho*T_n, p, T, s, T_n, (
 
 struct Patch32 {
   long blk[3];  // FP32 stage blocks (floats from pool32): w'(5), p', T', 7*sk each
+  long res[3];  // their rounding residuals (bytes from res8), interior w'(5), 5*nx*ny each
   long dk;      // FP32 dK (floats from pool32), 5*nx*ny
   long wc;      // FP64 carried perturbation (doubles from wc), 5*nx*ny
   long amb;     // ambient record of ghosted row 0
@@ -79,12 +83,16 @@ __host__ __device__ inline double amb_of(const double* rec, int k) {
 struct Args32 {
   const float* in;   // stage input block
   float* out;        // stage output block
+  const signed char* rin;  // rounding residuals of the input block's interior w'
+  signed char* rout;       // and of the output block's
   double* wc;        // carried FP64 perturbation, interior
   const float* dk;   // ambient correction, interior (nullptr in eval mode)
   float* kout;       // eval mode: K32 stored here
   const double* h0;  // source, ghosted
   const double* amb; // record of ghosted row 0
   const RowF* rows;  // row table entry of ghosted row 0
+  const unsigned char* decide;  // level 0 with the learned switch: 0 DRP, 1 WENO, 2 sensor
+  unsigned char* amb_weno;      // eval mode, level 0: the sensor's choice on the ambient alone
   int nx, ny, g, last;
   long sy, sk;
   float ylo, dy;
@@ -96,6 +104,25 @@ struct Args32 {
 
 __device__ inline float sqf(float x) { return x * x; }
 __device__ inline float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Stage states are stored in FP32, but the RK update combines a stage's
+// input (alpha_2 w') with FP64 terms: rounding w' to FP32 there would add a
+// fresh error of up to 2^-24 of the perturbation at every stage.  Each
+// interior value of w' therefore carries its rounding residual, the FP64
+// value less the stored float, as a signed byte in units of 1/254 of the
+// float's ulp; float + residual stands for the FP64 value to within
+// 2^-24 / 508 of it.
+__device__ inline double ulp32(float f) {
+  const unsigned e = __float_as_uint(f) & 0x7f800000u;  // 2^E, E the float's exponent
+  return e ? (double)__uint_as_float(e) * 0x1p-23 : 0x1p-149;
+}
+__device__ inline signed char res_enc(double v, float f) {
+  if (!__builtin_isfinite(f)) return 0;
+  double r = rint((v - (double)f) * (254.0 / ulp32(f)));
+  r = r > 127.0 ? 127.0 : (r < -127.0 ? -127.0 : r);
+  return (signed char)r;
+}
+__device__ inline double res_dec(float f, signed char r) { return (double)f + (double)r * (ulp32(f) * (1.0 / 254.0)); }
 
 // WENO5 reconstructions in FP32, the formulas of rhs_hybrid_high_llf.m4:365-433
 // (float literals as there), weights through reciprocals.
@@ -126,19 +153,26 @@ __device__ inline float wenop5f(float fgm2, float fgm1, float fg0, float fgp1, f
   return (z0 * is) * f0 + (z1 * is) * f1 + (z2 * is) * f2;
 }
 
-// pf1 + mf1 - pf2 - mf2 of one direction (rhs_hybrid_high_llf.m4:286-354).
-__device__ inline float weno_dir32(const float* F, const float* W, float alpha) {
+// pf1 + mf1 - pf2 - mf2 of one direction (rhs_hybrid_high_llf.m4:286-354),
+// from the stencil's differences to its centre point,
+// dF[m] = F[m] - F[3] and dW[m] = W[m] - W[3].  The weights see only
+// differences and every candidate's coefficients sum to 6 x (1.0/6.0), so
+// shifting the seven values by one constant shifts pf1 and pf2 (mf1 and
+// mf2) alike and leaves pf1 - pf2 + mf1 - mf2 as it was; with the small
+// shifted values FP32 rounds at the size of the differences, not of the
+// fluxes.
+__device__ inline float weno_dir32s(const float* dF, const float* dW, float alpha) {
   float fp[7], fm[7];
 #pragma unroll
   for (int m = 0; m < 7; m++) {
-    fp[m] = 0.5f * (F[m] + alpha * W[m]);
-    fm[m] = 0.5f * (F[m] - alpha * W[m]);
+    fp[m] = 0.5f * (dF[m] + alpha * dW[m]);
+    fm[m] = 0.5f * (dF[m] - alpha * dW[m]);
   }
   const float p1 = wenop5f(fp[0], fp[1], fp[2], fp[3], fp[4]);
   const float m1 = wenom5f(fm[1], fm[2], fm[3], fm[4], fm[5]);
   const float p2 = wenop5f(fp[1], fp[2], fp[3], fp[4], fp[5]);
   const float m2 = wenom5f(fm[2], fm[3], fm[4], fm[5], fm[6]);
-  return p1 + m1 - p2 - m2;
+  return (p1 - p2) + (m1 - m2);
 }
 
 // FP64 WENO5 with reciprocals refined by one Newton step, for the
@@ -197,17 +231,6 @@ struct Lds32 {
 };
 
 template <int BX, int BY>
-__device__ inline float flux_x(const Lds32<BX, BY>& L, int e, int r, int k) {
-  const float ru = L.rho[e] * L.u[e];
-  switch (k) {
-    case 0: return ru;
-    case 1: return ru * L.u[e] + L.p[e];
-    case 2: return ru * L.v[e];
-    case 3: return ru * (L.row[r].aS + L.s[e]);
-    default: return ru * (L.row[r].aTn + L.tn[e]);
-  }
-}
-template <int BX, int BY>
 __device__ inline float flux_y(const Lds32<BX, BY>& L, int e, int r, int k) {
   const float rv = L.rho[e] * L.v[e];
   switch (k) {
@@ -219,8 +242,43 @@ __device__ inline float flux_y(const Lds32<BX, BY>& L, int e, int r, int k) {
   }
 }
 
+// F(e1) - F(e2) of two cells of one row (x) and G(e1) - G(e2) of two cells
+// of one column (y, rows r1 and r2), formed difference first: the parts
+// that are stored values (the momenta for the mass flux, p' in the normal
+// momentum flux, the ambient entropy and T_n of a row times the momentum)
+// are differenced before the products of perturbations are added, so a
+// smooth field's nearly cancelling fluxes lose no more than the rounding of
+// their difference.  The fluxes are those of rhs_hybrid_high_llf.m4:125-156
+// on full values (ambient + perturbation); the mass flux is the stored
+// momentum rather than rho times its quotient u.
+template <int BX, int BY>
+__device__ inline float dflux_x(const Lds32<BX, BY>& L, int e1, int e2, int r, int k) {
+  const float m1 = L.w[1][e1], m2 = L.w[1][e2];
+  switch (k) {
+    case 0: return m1 - m2;
+    case 1: return (L.p[e1] - L.p[e2]) + (m1 * L.u[e1] - m2 * L.u[e2]);
+    case 2: return m1 * L.v[e1] - m2 * L.v[e2];
+    case 3: return L.row[r].aS * (m1 - m2) + (m1 * L.s[e1] - m2 * L.s[e2]);
+    default: return L.row[r].aTn * (m1 - m2) + (m1 * L.tn[e1] - m2 * L.tn[e2]);
+  }
+}
+template <int BX, int BY>
+__device__ inline float dflux_y(const Lds32<BX, BY>& L, int e1, int e2, int r1, int r2, int k) {
+  const float m1 = L.w[2][e1], m2 = L.w[2][e2];
+  switch (k) {
+    case 0: return m1 - m2;
+    case 1: return m1 * L.u[e1] - m2 * L.u[e2];
+    case 2: return ((L.p[e1] - L.p[e2]) + (L.row[r1].pA - L.row[r2].pA)) + (m1 * L.v[e1] - m2 * L.v[e2]);
+    case 3:
+      return (L.row[r1].aS * (m1 - m2) + (L.row[r1].aS - L.row[r2].aS) * m2) + (m1 * L.s[e1] - m2 * L.s[e2]);
+    default:
+      return (L.row[r1].aTn * (m1 - m2) + (L.row[r1].aTn - L.row[r2].aTn) * m2) + (m1 * L.tn[e1] - m2 * L.tn[e2]);
+  }
+}
+
 // One tile of one patch: rhs in FP32, FP64 RK accumulation, FP32 EOS.
-// SENSOR: level 0 of a hybrid variant (DRP where the density sensor allows).
+// SENSOR: level 0 of a hybrid variant (DRP where the density sensor allows,
+// or as the learned switch's decision map says where one is given).
 // EVAL: store K32 into kout instead of stepping.
 template <int BX, int BY, bool SENSOR, bool EVAL>
 __global__ void __launch_bounds__(BX* BY) stage32_kernel(const int4* __restrict__ tiles, int ntiles,
@@ -333,10 +391,11 @@ __global__ void __launch_bounds__(BX* BY) stage32_kernel(const int4* __restrict_
   // ---- switch (rhs_hybrid_high_llf.m4:239-256): density differences
   bool weno = true;
   if constexpr (SENSOR) {
-    weno = false;
+    const unsigned char d = a.decide ? a.decide[(long)i + (long)j * a.nx] : 2;
+    weno = d == 1;
     const float* R0 = L.w[0];
 #pragma unroll
-    for (int q = -1; q <= 1; q++) {
+    for (int q = -1; q <= 1 && d == 2; q++) {
       float t1 = R0[e + q] - R0[e - 1 + q], t2 = R0[e + 1 + q] - R0[e + q];
       float num = 2 * fabsf(t1 * t2) + a.ep, den = sqf(t1) + sqf(t2) + a.ep;
       if (num / den < 0.9999f) weno = true;
@@ -349,15 +408,16 @@ __global__ void __launch_bounds__(BX* BY) stage32_kernel(const int4* __restrict_
   }
   float K[5];
   if (!weno) {
-    // DRP (rhs_hybrid_high_llf.m4:258-268)
+    // DRP (rhs_hybrid_high_llf.m4:258-268) as a1 (F1 - F-1) + a2 (F2 - F-2)
+    // + a3 (F3 - F-3), each pair differenced first (dflux_x / dflux_y)
     const float a1 = 0.79926643f, a2 = -0.18941314f, a3 = 0.02651995f;
 #pragma unroll
     for (int k = 0; k < 5; k++) {
-      const float fx = -a3 * flux_x(L, e - 3, r, k) - a2 * flux_x(L, e - 2, r, k) - a1 * flux_x(L, e - 1, r, k) +
-                       a1 * flux_x(L, e + 1, r, k) + a2 * flux_x(L, e + 2, r, k) + a3 * flux_x(L, e + 3, r, k);
-      const float gy = -a3 * flux_y(L, e - 3 * S, r - 3, k) - a2 * flux_y(L, e - 2 * S, r - 2, k) -
-                       a1 * flux_y(L, e - S, r - 1, k) + a1 * flux_y(L, e + S, r + 1, k) +
-                       a2 * flux_y(L, e + 2 * S, r + 2, k) + a3 * flux_y(L, e + 3 * S, r + 3, k);
+      const float fx = a1 * dflux_x(L, e + 1, e - 1, r, k) + a2 * dflux_x(L, e + 2, e - 2, r, k) +
+                       a3 * dflux_x(L, e + 3, e - 3, r, k);
+      const float gy = a1 * dflux_y(L, e + S, e - S, r + 1, r - 1, k) +
+                       a2 * dflux_y(L, e + 2 * S, e - 2 * S, r + 2, r - 2, k) +
+                       a3 * dflux_y(L, e + 3 * S, e - 3 * S, r + 3, r - 3, k);
       K[k] = fx * a.inv_mdx + gy * a.inv_mdy + H[k];
     }
   } else {
@@ -376,19 +436,19 @@ __global__ void __launch_bounds__(BX* BY) stage32_kernel(const int4* __restrict_
       float F[7], W[7];
 #pragma unroll
       for (int m = 0; m < 7; m++) {
-        F[m] = flux_x(L, e + m - 3, r, k);
-        W[m] = L.w[k][e + m - 3];
+        F[m] = dflux_x(L, e + m - 3, e, r, k);
+        W[m] = L.w[k][e + m - 3] - L.w[k][e];
       }
-      const float x = weno_dir32(F, W, af);
+      const float x = weno_dir32s(F, W, af);
       float y;
       if (k == 1 || k == 2) {
         // momenta: no ambient, FP32
 #pragma unroll
         for (int m = 0; m < 7; m++) {
-          F[m] = flux_y(L, e + (m - 3) * S, r + m - 3, k);
-          W[m] = L.w[k][e + (m - 3) * S];
+          F[m] = dflux_y(L, e + (m - 3) * S, e, r + m - 3, r, k);
+          W[m] = L.w[k][e + (m - 3) * S] - L.w[k][e];
         }
-        y = weno_dir32(F, W, ag);
+        y = weno_dir32s(F, W, ag);
       } else {
         // full values: FP64 ambient row + FP32 perturbation
         const int c = k == 0 ? 0 : k - 2;
@@ -412,22 +472,30 @@ __global__ void __launch_bounds__(BX* BY) stage32_kernel(const int4* __restrict_
   if constexpr (EVAL) {
 #pragma unroll
     for (int k = 0; k < 5; k++) a.kout[ic + k * nn] = K[k];
+    if (SENSOR && a.amb_weno) a.amb_weno[ic] = weno ? 1 : 0;
     return;
   } else {
     // ---- FP64 RK accumulation (runge_kutta.m4:60-63) of the perturbation
     // the loads before the first store, so that they are in flight together
-    double wn[5], wcv[5];
+    double wn[5], wcv[5], win[5];
     float dkv[5];
+    signed char rv[5] = {0, 0, 0, 0, 0};
+    const bool use_in = a.alpha_2 != 0.0;  // uniform over the launch
 #pragma unroll
     for (int k = 0; k < 5; k++) {
       dkv[k] = a.dk[ic + k * nn];
       wcv[k] = a.wc[ic + k * nn];
+      if (use_in) rv[k] = a.rin[ic + k * nn];
     }
 #pragma unroll
     for (int k = 0; k < 5; k++) {
+      // the stage input at the FP64 value its RK update formed
+      win[k] = res_dec(L.w[k][e], rv[k]);
       const double kd = (double)K[k] + (double)dkv[k];
-      wn[k] = a.alpha_1 * wcv[k] + a.alpha_2 * (double)L.w[k][e] + a.bdt * kd;
-      a.out[gx + k * a.sk] = (float)wn[k];
+      wn[k] = a.alpha_1 * wcv[k] + a.alpha_2 * win[k] + a.bdt * kd;
+      const float wf = (float)wn[k];
+      a.out[gx + k * a.sk] = wf;
+      a.rout[ic + k * nn] = res_enc(wn[k], wf);
       if (a.last) a.wc[ic + k * nn] = wn[k];
     }
     // ---- update_state (update_state.m4:87-107) in perturbation form
@@ -520,8 +588,9 @@ __global__ void __launch_bounds__(256) fill32_corner_kernel(Fill32 f, int n, con
 }
 
 // Fine-to-coarse average of block 0 and the carried state, on full values.
-__global__ void __launch_bounds__(256) coarsen32_kernel(Fill32 f, double* wc, int r, int n, const int* cp,
-                                                         const int* cl, const int* fp, const int* fl) {
+__global__ void __launch_bounds__(256) coarsen32_kernel(Fill32 f, double* wc, signed char* res8, int r, int n,
+                                                         const int* cp, const int* cl, const int* fp,
+                                                         const int* fl) {
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
     const Patch32& C = f.patches[cp[q]];
     const Patch32& F = f.patches[fp[q]];
@@ -543,16 +612,21 @@ __global__ void __launch_bounds__(256) coarsen32_kernel(Fill32 f, double* wc, in
           s += A + pv;
         }
       const double v = s * inv - amb_of(rc, k);
-      if (k < 5) wc[C.wc + ci + k * nc] = v;
-      cb[k * C.sk] = (float)v;
+      const float vf = (float)v;
+      if (k < 5) {
+        wc[C.wc + ci + k * nc] = v;
+        res8[C.res[0] + ci + k * nc] = res_enc(v, vf);
+      }
+      cb[k * C.sk] = vf;
     }
   }
 }
 
-// FP64 block -> FP32 block (+ carried state, at the patch's offset p.wc of
-// the carried-state pool), and back.
-__global__ void __launch_bounds__(256) to32_kernel(const double* w64, float* w32, double* wc, const double* amb,
-                                                    Patch32 p) {
+// FP64 block -> FP32 block 0 (+ carried state, at the patch's offset p.wc
+// of the carried-state pool, and the interior's rounding residuals), and
+// back.
+__global__ void __launch_bounds__(256) to32_kernel(const double* w64, float* w32, double* wc, signed char* res8,
+                                                    const double* amb, Patch32 p) {
   const long nall = p.sk;
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < nall; q += (long)gridDim.x * blockDim.x) {
     const double* rec = amb + (p.amb + q / p.sy) * kRec;
@@ -560,8 +634,13 @@ __global__ void __launch_bounds__(256) to32_kernel(const double* w64, float* w32
     const bool inner = ii >= 0 && ii < p.nx && jj >= 0 && jj < p.ny;
     for (int k = 0; k < 7; k++) {
       const double v = w64[q + k * p.sk] - amb_of(rec, k);
-      w32[q + k * p.sk] = (float)v;
-      if (inner && k < 5) wc[p.wc + ii + (long)jj * p.nx + k * (long)p.nx * p.ny] = v;
+      const float vf = (float)v;
+      w32[q + k * p.sk] = vf;
+      if (inner && k < 5) {
+        const long c = ii + (long)jj * p.nx + k * (long)p.nx * p.ny;
+        wc[p.wc + c] = v;
+        res8[p.res[0] + c] = res_enc(v, vf);
+      }
     }
   }
 }
@@ -609,6 +688,7 @@ __global__ void __launch_bounds__(256) dk_kernel(float* dk, const double* k64, P
 
 struct Fp32Form {
   float* pool32 = nullptr;
+  signed char* res8 = nullptr;
   double* wc = nullptr;
   double* amb = nullptr;
   RowF* rows = nullptr;
@@ -625,6 +705,19 @@ struct Fp32Form {
   hipGraphExec_t exec = nullptr;
   std::vector<unsigned char> key;
   std::vector<void*> allocs;
+  // the learned switch on level 0 (enl_batch_fp32_set_switch)
+  Switch* sw = nullptr;
+  float lo = 0, hi = 1;
+  int every_stage = 1;
+  unsigned long long* totals = nullptr;
+  unsigned char* dmap = nullptr;  // decision maps of the level-0 patches
+  unsigned char* ambmap = nullptr;  // per level-0 cell: the sensor's choice on the ambient alone
+  std::vector<long> dmap_off;     // per patch, -1 above level 0
+  SwPatch* swtab = nullptr;       // 3 x n0 entries: the level-0 patches per stage block
+  int n0 = 0, max_side0 = 0;
+  long cells0 = 0;
+  double sw_thr = 0;
+  float sw_inv_r = 0, sw_inv_p = 0;
 };
 
 namespace {
@@ -740,7 +833,7 @@ int setup32(Batch* b) {
   drop_graph32(F);
   F->ready = false;
   F->hp.assign(b->npatch, Patch32());
-  long off32 = 0, off64 = 0, amb_total = 0;
+  long off32 = 0, off64 = 0, off8 = 0, amb_total = 0;
   long amb_base[kMaxLevels];
   for (int l = 0; l < kMaxLevels; l++) {
     amb_base[l] = amb_total;
@@ -760,6 +853,9 @@ int setup32(Batch* b) {
       q.blk[k] = off32;
       off32 += 7 * pd.sk;
       off32 = (off32 + 63) & ~63L;
+      q.res[k] = off8;
+      off8 += 5L * pd.nx * pd.ny;
+      off8 = (off8 + 255) & ~255L;
     }
     q.dk = off32;
     off32 += 5L * pd.nx * pd.ny;
@@ -774,6 +870,8 @@ int setup32(Batch* b) {
     q.amb = amb_base[pd.level] + r0;
   }
   int rc = alloc(F, &F->pool32, (size_t)off32, "hipMalloc(FP32 pool)");
+  if (!rc) rc = alloc(F, &F->res8, (size_t)off8, "hipMalloc(FP32 rounding residuals)");
+  if (!rc) rc = check_hip(hipMemset(F->res8, 0, (size_t)off8), "FP32 rounding residuals clear");
   if (!rc) rc = alloc(F, &F->wc, (size_t)off64, "hipMalloc(FP32 carried state)");
   if (!rc) rc = alloc(F, &F->amb, (size_t)amb_total * kRec, "hipMalloc(ambient)");
   if (!rc) rc = alloc(F, &F->rows, (size_t)amb_total, "hipMalloc(ambient rows)");
@@ -818,6 +916,52 @@ int setup32(Batch* b) {
   if (!rc)
     rc = check_hip(hipMemcpy(F->dp, F->hp.data(), sizeof(Patch32) * F->hp.size(), hipMemcpyHostToDevice),
                    "FP32 patch table upload");
+  // decision maps of the level-0 patches, and per stage block the table
+  // the learned switch reads them from
+  F->dmap_off.assign((size_t)b->npatch, -1L);
+  F->n0 = 0;
+  F->max_side0 = 0;
+  F->cells0 = 0;
+  long offd = 0;
+  for (int p = 0; p < b->npatch; p++) {
+    const Patch32& q = F->hp[p];
+    if (q.level != 0) continue;
+    F->dmap_off[p] = offd;
+    offd += (long)q.nx * q.ny;
+    offd = (offd + 255) & ~255L;
+    F->n0++;
+    F->cells0 += (long)q.nx * q.ny;
+    F->max_side0 = std::max(F->max_side0, std::max(q.nx, q.ny));
+  }
+  if (!rc) rc = alloc(F, &F->dmap, (size_t)offd, "hipMalloc(decision maps)");
+  if (!rc) rc = alloc(F, &F->ambmap, (size_t)offd, "hipMalloc(ambient sensor maps)");
+  if (!rc) rc = alloc(F, &F->swtab, (size_t)3 * F->n0, "hipMalloc(switch tables)");
+  if (!rc) {
+    std::vector<SwPatch> t((size_t)3 * F->n0);
+    int q0 = 0;
+    for (int p = 0; p < b->npatch; p++) {
+      if (F->dmap_off[p] < 0) continue;
+      const Patch32& q = F->hp[p];
+      const char* row0 = (const char*)(F->rows + q.amb);
+      for (int k = 0; k < 3; k++) {
+        SwPatch& S = t[(size_t)k * F->n0 + q0];
+        S.rho = F->pool32 + q.blk[k];
+        S.p = F->pool32 + q.blk[k] + 5 * q.sk;
+        S.drho = (const float*)(row0 + offsetof(RowF, d1Rho));
+        S.decide = F->dmap + F->dmap_off[p];
+        S.amb_weno = F->ambmap + F->dmap_off[p];
+        S.nx = q.nx;
+        S.ny = q.ny;
+        S.g = q.g;
+        S.rs = (int)(sizeof(RowF) / sizeof(float));
+        S.sy = q.sy;
+      }
+      q0++;
+    }
+    rc = check_hip(hipMemcpy(F->swtab, t.data(), sizeof(SwPatch) * t.size(), hipMemcpyHostToDevice),
+                   "switch table upload");
+  }
+  if (!rc && F->sw) rc = switch_reserve(F->sw, F->cells0);
   F->h_args.clear();
   return rc;
 }
@@ -827,7 +971,7 @@ int load32(Batch* b, hipStream_t s) {
   for (int p = 0; p < b->npatch; p++) {
     const Patch32& q = F->hp[p];
     hipLaunchKernelGGL(to32_kernel, dim3(grid_for(q.sk)), dim3(256), 0, s, b->pool + b->hp[p].state[0],
-                       F->pool32 + q.blk[0], F->wc, F->amb, q);
+                       F->pool32 + q.blk[0], F->wc, F->res8, F->amb, q);
   }
   return check_hip(hipGetLastError(), "FP32 load");
 }
@@ -842,12 +986,21 @@ int prepare_args32(Batch* b, const enl_gas_t* gas, int order, const double* a1, 
       const Patch32& q = F->hp[p];
       a.in = F->pool32 + q.blk[stage_in(st, order)];
       a.out = F->pool32 + q.blk[stage_out(st, order)];
+      a.rin = F->res8 + q.res[stage_in(st, order)];
+      a.rout = F->res8 + q.res[stage_out(st, order)];
       a.alpha_1 = a1[st];
       a.alpha_2 = a2[st];
       a.bdt = be[st] * dt;
       a.last = st == order - 1;
+      if (F->sw && F->dmap_off[p] >= 0) a.decide = F->dmap + F->dmap_off[p];
       t[(size_t)st * b->npatch + p] = a;
     }
+  if (F->sw) {
+    const double ep = sensor_eps(gas->tag_tolerance[2]);
+    F->sw_thr = 5e-5 * (double)(float)ep;  // of the FP32 sensor's epsilon
+    F->sw_inv_r = (float)(1.0 / std::sqrt(ep));
+    F->sw_inv_p = (float)(1.0 / (gas->c * gas->c * std::sqrt(ep)));
+  }
   const size_t bytes = sizeof(Args32) * t.size();
   if (F->h_args.size() == t.size() && memcmp(F->h_args.data(), t.data(), bytes) == 0) return 0;
   F->h_args.swap(t);
@@ -865,6 +1018,13 @@ int enqueue32(Batch* b, const int* reflect, int order, int nsteps, hipStream_t s
       for (int lev = 0; lev < b->nlevels; lev++) {
         const bool refreshed = st > 0 && lev == 0 && b->nlevels > 1;
         if (!refreshed) fill32_level(b, lev, in, out, reflect, s);
+        if (lev == 0 && F->sw && (F->every_stage || st == 0)) {
+          // the learned switch decides level 0's cells from this stage's input
+          int rc = switch_decide_batch(F->sw, F->swtab + (size_t)in * F->n0, F->n0, F->max_side0, F->cells0,
+                                       F->sw_thr, F->sw_inv_r, F->sw_inv_p, F->lo, F->hi, s);
+          if (!rc && F->totals) rc = switch_accumulate(F->sw, F->totals, s);
+          if (rc) return rc;
+        }
         int rc = stage32<false>(b, args, lev, s);
         if (rc) return rc;
         if (lev + 1 < b->nlevels) fill32_level(b, lev, out, out, reflect, s);
@@ -878,7 +1038,7 @@ int enqueue32(Batch* b, const int* reflect, int order, int nsteps, hipStream_t s
       f.buf = 0;
       f.cbuf = 0;
       for (int k = 0; k < 4; k++) f.reflect[k] = reflect[k];
-      hipLaunchKernelGGL(coarsen32_kernel, dim3(grid_for(b->ncoarsen)), dim3(256), 0, s, f, F->wc, b->ratio,
+      hipLaunchKernelGGL(coarsen32_kernel, dim3(grid_for(b->ncoarsen)), dim3(256), 0, s, f, F->wc, F->res8, b->ratio,
                          b->ncoarsen, b->co_cp, b->co_cl, b->co_fp, b->co_fl);
     }
   }
@@ -912,6 +1072,12 @@ int fp32_advance(Batch* b, int mode, const enl_gas_t* gas, const int* reflect, i
   put(reflect, 4 * sizeof(int));
   put(b->tile_bx, sizeof b->tile_bx);
   put(b->tile_by, sizeof b->tile_by);
+  put(&F->sw, sizeof F->sw);
+  put(&F->lo, sizeof F->lo);
+  put(&F->hi, sizeof F->hi);
+  put(&F->every_stage, sizeof F->every_stage);
+  put(&F->totals, sizeof F->totals);
+  put(&F->sw_thr, sizeof F->sw_thr);
   if (!(F->exec && key == F->key)) {
     drop_graph32(F);
     hipStream_t cs;
@@ -1017,6 +1183,7 @@ int enl_batch_fp32_prepare(void* handle, int variant, const enl_gas_t* gas, cons
     a.in = F->pool32 + F->hp[p].blk[2];
     a.out = nullptr;
     a.kout = F->pool32 + F->hp[p].dk;
+    if (F->dmap_off[p] >= 0) a.amb_weno = F->ambmap + F->dmap_off[p];
     ev[p] = a;
   }
   rc = check_hip(hipMemcpy(F->d_eval, ev.data(), sizeof(Args32) * ev.size(), hipMemcpyHostToDevice),
@@ -1033,6 +1200,35 @@ int enl_batch_fp32_prepare(void* handle, int variant, const enl_gas_t* gas, cons
   if (!rc) rc = check_hip(hipStreamSynchronize(s), "enl_batch_fp32_prepare");
   F->ready = rc == 0;
   return rc;
+}
+
+int enl_batch_fp32_set_switch(void* handle, void* sw, float lo, float hi, int every_stage,
+                              unsigned long long* totals) {
+  Batch* b = (Batch*)handle;
+  if (!b) return set_error(-2, "enl_batch_fp32_set_switch: no batch");
+  if (sw && !(lo <= hi)) return set_error(-2, "enl_batch_fp32_set_switch: lo > hi");
+  if (totals && !is_device_ptr(totals)) return set_error(-4, "enl_batch_fp32_set_switch: totals must be device memory");
+  Fp32Form* F = form(b);
+  F->sw = (Switch*)sw;
+  F->lo = sw ? lo : 0.0f;
+  F->hi = sw ? hi : 1.0f;
+  F->every_stage = every_stage ? 1 : 0;
+  F->totals = sw ? totals : nullptr;
+  if (F->sw && F->ready) return switch_reserve(F->sw, F->cells0);
+  return 0;
+}
+
+int enl_batch_fp32_decisions(void* handle, int pid, unsigned char* out) {
+  Batch* b = (Batch*)handle;
+  if (!b || !b->f32 || !b->f32->ready || !out) return set_error(-2, "enl_batch_fp32_decisions: bad arguments");
+  Fp32Form* F = b->f32;
+  if (pid < 0 || pid >= b->npatch || F->dmap_off[pid] < 0)
+    return set_error(-2, "enl_batch_fp32_decisions: patch %d is not on level 0", pid);
+  const Patch32& q = F->hp[pid];
+  hipStream_t s = lib_stream();
+  hipError_t e = hipMemcpyAsync(out, F->dmap + F->dmap_off[pid], (size_t)q.nx * q.ny, hipMemcpyDefault, s);
+  if (e != hipSuccess) return check_hip(e, "enl_batch_fp32_decisions copy");
+  return check_hip(hipStreamSynchronize(s), "enl_batch_fp32_decisions sync");
 }
 
 int enl_batch_fp32_load(void* handle) {

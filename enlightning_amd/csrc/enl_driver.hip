@@ -235,6 +235,7 @@ namespace {
 struct LearnedSwitch {
   Switch* sw = nullptr;
   float lo = 0, hi = 1;
+  bool every_stage = true;
   unsigned char* decide = nullptr;
   unsigned long long* totals = nullptr;
 };
@@ -256,7 +257,7 @@ extern "C" int enl_advance_patch_fused(int variant, int ifirst0, int ilast0, int
                        absorb_width);
 }
 
-extern "C" int enl_advance_patch_learned(void* sw, float lo, float hi, unsigned char* decide,
+extern "C" int enl_advance_patch_learned(void* sw, float lo, float hi, int every_stage, unsigned char* decide,
                                          unsigned long long* totals, int variant, int ifirst0, int ilast0,
                                          int ifirst1, int ilast1, int gcw, const double* xlo, const double* dx,
                                          const enl_gas_t* gas, const int* reflect, int order, const double* alpha_1,
@@ -273,6 +274,7 @@ extern "C" int enl_advance_patch_learned(void* sw, float lo, float hi, unsigned 
   L.sw = (Switch*)sw;
   L.lo = lo;
   L.hi = hi;
+  L.every_stage = every_stage != 0;
   L.decide = decide;
   L.totals = totals;
   return advance_fused(L, variant, ifirst0, ilast0, ifirst1, ilast1, gcw, xlo, dx, gas, reflect, order, alpha_1,
@@ -384,9 +386,12 @@ int advance_fused(const LearnedSwitch& L, int variant, int ifirst0, int ilast0, 
       ra.bdt = beta[stage] * dt;
       if (L.sw) {
         // the learned switch decides level 0's cells from the stage input
-        int rc = switch_decide(L.sw, G, in.w, in.p, gas, L.lo, L.hi, L.decide, s);
-        if (!rc && L.totals) rc = switch_accumulate(L.sw, L.totals, s);
-        if (rc) return rc;
+        // (every stage, or the step's first and its map kept)
+        if (L.every_stage || stage == 0) {
+          int rc = switch_decide(L.sw, G, in.w, in.p, gas, L.lo, L.hi, L.decide, s);
+          if (!rc && L.totals) rc = switch_accumulate(L.sw, L.totals, s);
+          if (rc) return rc;
+        }
         ra.decide = L.decide;
       }
       t = g_timer.begin(ENL_KERNEL_STAGE, s);

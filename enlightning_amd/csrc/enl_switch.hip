@@ -3,10 +3,13 @@
 // on level 0.
 //
 // Per RK stage and patch:
-//   1. a prefilter marks as DRP every cell whose sensor pairs all satisfy
-//      t1^2 + t2^2 < 5e-5*eps: there 1 - r <= (t1^2+t2^2)/(t1^2+t2^2+eps)
-//      < 5e-5, so r > 0.9999f and the sensor itself says DRP.  The other
-//      cells (candidates) are compacted into a list;
+//   1. a prefilter marks as DRP every cell whose six sensor pairs (t1, t2)
+//      are all provably smooth: 1 - r = (|t1| - |t2|)^2 / (t1^2 + t2^2 +
+//      eps), and (|t1| - |t2|)^2 is at most t1^2 + t2^2 and at most
+//      (t1 - t2)^2, so where either is below 5e-5*eps, 1 - r < 5e-5, r >
+//      0.9999f and the sensor itself says DRP (the second form covers
+//      steep but straight profiles, a stratified ambient among them).  The
+//      other cells (candidates) are compacted into a list;
 //   2. the network runs on the candidates only: a 5x5 window of four
 //      normalised differences (x and y differences of rho and p, each
 //      asinh(d / scale) with scale sqrt(eps) for rho and c^2 sqrt(eps) for
@@ -26,6 +29,7 @@
 
 #include "enl_device.h"
 #include "enl_kernels.h"
+#include "enl_switch.h"
 #include "enlightning_hip.h"
 
 // The switch is not part of the reference arithmetic: multiply-adds fuse.
@@ -186,11 +190,22 @@ struct DecideArgs {
   unsigned char* decide;
   unsigned* cand;    // candidate cells (interior index)
   unsigned* ncand;   // [0] candidates, [1] WENO, [2] undecided
-  float lo, hi;
 };
 
+// n[0] network candidates (the list length), n[1] network WENO, n[2]
+// network undecided, n[3] cells the prefilter itself leaves to the sensor.
 __global__ void decide_reset_kernel(unsigned* n) {
-  if (threadIdx.x < 3) n[threadIdx.x] = 0;
+  if (threadIdx.x < 4) n[threadIdx.x] = 0;
+}
+
+// A sensor pair provably on the DRP side (see the top of this file):
+// t1^2 + t2^2 or (t1 - t2)^2 below thr = 5e-5 eps, each product and sum
+// rounded on its own.
+template <class T>
+__device__ inline bool pair_smooth(T t1, T t2, T thr) {
+#pragma clang fp contract(off)
+  const T d = t1 - t2;
+  return t1 * t1 + t2 * t2 < thr || d * d < thr;
 }
 
 // Prefilter: DRP where the sensor provably says DRP, else a candidate.  A
@@ -229,10 +244,10 @@ __global__ void __launch_bounds__(kPBX * kPBY) prefilter_kernel(DecideArgs a) {
 #pragma unroll
     for (int q = -1; q <= 1; q++) {
       double t1 = rt[e + q] - rt[e - 1 + q], t2 = rt[e + 1 + q] - rt[e + q];
-      if (t1 * t1 + t2 * t2 >= a.thr) cand = true;
+      if (!pair_smooth(t1, t2, a.thr)) cand = true;
       t1 = rt[e + q * kPW] - rt[e + (q - 1) * kPW];
       t2 = rt[e + (q + 1) * kPW] - rt[e + q * kPW];
-      if (t1 * t1 + t2 * t2 >= a.thr) cand = true;
+      if (!pair_smooth(t1, t2, a.thr)) cand = true;
     }
     if (!cand) a.decide[c] = 0;
   }
@@ -255,6 +270,96 @@ __global__ void __launch_bounds__(kPBX * kPBY) prefilter_kernel(DecideArgs a) {
   if (cand) a.cand[wbase[wave] + __popcll(m & __lanemask_lt())] = (unsigned)c;
 }
 
+// The same prefilter over the level-0 patches of an FP32-perturbation
+// hierarchy: blockIdx.y is the patch, blockIdx.x a 32 x 8 tile of it (tiles
+// past the patch's own count leave at once).  The pairs are formed as the
+// FP32 stage kernel forms them (x: differences of rho'; y: differences of
+// rho' plus the ambient's row difference), so a cell marked DRP here is one
+// the FP32 sensor marks DRP: with t1^2 + t2^2 or (t1 - t2)^2 below 5e-5 eps
+// (each rounded within 2^-22 of itself), 1 - r < 5.0001e-5 and the float
+// quotient stays above 0.99994 > 0.9999f.  The network reads the
+// perturbation alone and is trusted only where the ambient plays no part:
+// a cell whose pairs of rho' alone are provably smooth (a candidate only
+// through the ambient's profile), or where the sensor calls the ambient
+// alone WENO (next to the reflecting walls, whose ghost copies kink a
+// stratified ambient; the FP32 form's ambient correction holds the
+// ambient's tendency under the sensor's choice there), gets no network call
+// and is left to the sensor (2).
+__global__ void __launch_bounds__(kPBX * kPBY) prefilter32_kernel(const SwPatch* __restrict__ pt, float thr,
+                                                                 unsigned long long* cand, unsigned* ncand) {
+  __shared__ float rt[kPH * kPW];
+  __shared__ float dr[kPH];
+  __shared__ unsigned wcount[kPWaves], wbase[kPWaves];
+  const int pid = blockIdx.y;
+  const SwPatch& P = pt[pid];
+  const int nx = P.nx, ny = P.ny, g = P.g;
+  const int nbx = (nx + kPBX - 1) / kPBX, nby = (ny + kPBY - 1) / kPBY;
+  if ((int)blockIdx.x >= nbx * nby) return;  // the same for the whole block
+  const int i0 = ((int)blockIdx.x % nbx) * kPBX, j0 = ((int)blockIdx.x / nbx) * kPBY;
+  const int tid = threadIdx.x;
+  constexpr int kIt = (kPH * kPW + kPBX * kPBY - 1) / (kPBX * kPBY);
+  float v[kIt];
+  bool in[kIt];
+#pragma unroll
+  for (int it = 0; it < kIt; it++) {
+    const int e = tid + it * kPBX * kPBY;
+    const int gi = i0 + e % kPW - 2, gj = j0 + e / kPW - 2;
+    in[it] = e < kPH * kPW && gi >= -g && gi < nx + g && gj >= -g && gj < ny + g;
+    v[it] = P.rho[in[it] ? (long)(gi + g) + (long)(gj + g) * P.sy : 0];
+  }
+  float dv = 0.0f;
+  const bool din = tid < kPH && j0 + tid - 2 >= -g && j0 + tid - 2 < ny + g;
+  if (din) dv = P.drho[(long)(j0 + tid - 2 + g) * P.rs];
+#pragma unroll
+  for (int it = 0; it < kIt; it++)
+    if (in[it]) rt[tid + it * kPBX * kPBY] = v[it];
+  if (din) dr[tid] = dv;
+  __syncthreads();
+  const int tx = tid % kPBX, ty = tid / kPBX;
+  const int i = i0 + tx, j = j0 + ty;
+  bool c = false, amb_only = false;
+  if (i < nx && j < ny) {
+    const int e = (ty + 2) * kPW + tx + 2;
+    bool full = false, pert = false;
+#pragma unroll
+    for (int q = -1; q <= 1; q++) {
+      float t1 = rt[e + q] - rt[e - 1 + q], t2 = rt[e + 1 + q] - rt[e + q];
+      if (!pair_smooth(t1, t2, thr)) full = pert = true;
+      const float p1 = rt[e + q * kPW] - rt[e + (q - 1) * kPW];
+      const float p2 = rt[e + (q + 1) * kPW] - rt[e + q * kPW];
+      if (!pair_smooth(p1 + dr[ty + q + 1], p2 + dr[ty + q + 2], thr)) full = true;
+      if (!pair_smooth(p1, p2, thr)) pert = true;
+    }
+    const bool amb = P.amb_weno[(long)i + (long)j * nx] != 0;
+    c = full && pert && !amb;
+    amb_only = full && !c;
+    if (!full || amb_only) P.decide[(long)i + (long)j * nx] = amb_only ? 2 : 0;
+  }
+  const unsigned long long m = __ballot(c);
+  const unsigned long long ma = __ballot(amb_only);
+  const int wave = tid >> 6;
+  if ((tid & 63) == 0) {
+    wcount[wave] = (unsigned)__popcll(m);
+    if (ma) atomicAdd(ncand + 3, (unsigned)__popcll(ma));
+  }
+  __syncthreads();
+  if (tid == 0) {
+    unsigned tot = 0;
+#pragma unroll
+    for (int w = 0; w < kPWaves; w++) tot += wcount[w];
+    unsigned b = tot ? atomicAdd(ncand, tot) : 0u;
+#pragma unroll
+    for (int w = 0; w < kPWaves; w++) {
+      wbase[w] = b;
+      b += wcount[w];
+    }
+  }
+  __syncthreads();
+  if (c)
+    cand[wbase[wave] + __popcll(m & __lanemask_lt())] =
+        ((unsigned long long)pid << 32) | (unsigned)(i + j * nx);
+}
+
 // Inference on the matrix cores: each wave takes 16 candidates at a time
 // (M = 16) through v_mfma_f32_16x16x16_bf16.  Lane l of a 16x16x16 MFMA holds
 // A[l%16][4(l/16)+v], B[4(l/16)+v][l%16] and D[4(l/16)+v][l%16], v = 0..3,
@@ -269,7 +374,67 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kWaves = 4;
 constexpr int kH1Row = kH1 + 4;  // LDS row of a cell's layer-1 output (bf16)
 
-__global__ void __launch_bounds__(64 * kWaves) infer_kernel(DecideArgs a, const unsigned short* params) {
+// A candidate source gives, per candidate q of its list, the location of its
+// cell (locate), the four features of window point (di, dj) from there
+// (features), and stores the decision (put).
+//
+// One FP64 patch state (enl_switch_decide, the learned fused advance).
+struct PatchSrc {
+  Feat F;
+  int nx, g;
+  const unsigned* cand;
+  unsigned char* decide;
+  struct Loc {
+    long g;
+  };
+  __device__ Loc locate(unsigned q) const {
+    const unsigned c = cand[q];
+    const int i = (int)(c % (unsigned)nx), j = (int)(c / (unsigned)nx);
+    return Loc{(long)(i + g) + (long)(j + g) * F.sy};
+  }
+  __device__ void features(const Loc& L, int di, int dj, float f[4]) const { point_features(F, L.g, di, dj, f); }
+  __device__ void put(unsigned q, unsigned char d) const { decide[cand[q]] = d; }
+};
+
+// Level-0 patches of an FP32-perturbation hierarchy: a candidate is (patch
+// << 32) | interior cell.  The features are the differences of the
+// perturbations rho' and p' alone: the network sees the deviation from the
+// known height-dependent ambient, as it saw the whole field on the uniform
+// ambient it was trained on (where the two are the same).  The ambient's own
+// vertical gradient is smooth, and the sensor calls it DRP.
+struct Batch32Src {
+  const SwPatch* pt;
+  const unsigned long long* cand;
+  float inv_r, inv_p;
+  struct Loc {
+    const float *r, *p;
+    long sy;
+  };
+  __device__ Loc locate(unsigned q) const {
+    const unsigned long long c = cand[q];
+    const SwPatch& P = pt[c >> 32];
+    const unsigned cell = (unsigned)c;
+    const int i = (int)(cell % (unsigned)P.nx), j = (int)(cell / (unsigned)P.nx);
+    const long gx = (long)(i + P.g) + (long)(j + P.g) * P.sy;
+    return Loc{P.rho + gx, P.p + gx, P.sy};
+  }
+  __device__ void features(const Loc& L, int di, int dj, float f[4]) const {
+    const long q = di + (long)dj * L.sy;
+    const float r0 = L.r[q], p0 = L.p[q];
+    f[0] = bf2f(f2bf(asinh_fast((L.r[q + 1] - r0) * inv_r)));
+    f[1] = bf2f(f2bf(asinh_fast((L.r[q + L.sy] - r0) * inv_r)));
+    f[2] = bf2f(f2bf(asinh_fast((L.p[q + 1] - p0) * inv_p)));
+    f[3] = bf2f(f2bf(asinh_fast((L.p[q + L.sy] - p0) * inv_p)));
+  }
+  __device__ void put(unsigned q, unsigned char d) const {
+    const unsigned long long c = cand[q];
+    pt[c >> 32].decide[(unsigned)c] = d;
+  }
+};
+
+template <class Src>
+__global__ void __launch_bounds__(64 * kWaves) infer_kernel(Src src, unsigned* ncand, float lo, float hi,
+                                                            const unsigned short* params) {
   __shared__ unsigned short h1s[kWaves][16][kH1Row];
   const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int row = l & 15, quad = l >> 4;
@@ -300,19 +465,14 @@ __global__ void __launch_bounds__(64 * kWaves) infer_kernel(DecideArgs a, const 
   }
   const float b3 = bf2f(params[oB3]);
 
-  const unsigned n = a.ncand[0];
+  const unsigned n = ncand[0];
   const unsigned groups = (n + 15) / 16;
   unsigned nweno = 0, nfall = 0;
   for (unsigned grp = blockIdx.x * kWaves + wv; grp < groups; grp += gridDim.x * kWaves) {
     const unsigned q = grp * 16 + row;
     const bool live = q < n;
-    unsigned c = 0;
-    long g = 0;
-    if (live) {
-      c = a.cand[q];
-      const int i = (int)(c % (unsigned)a.nx), j = (int)(c / (unsigned)a.nx);
-      g = (long)(i + a.g) + (long)(j + a.g) * a.F.sy;
-    }
+    typename Src::Loc loc{};
+    if (live) loc = src.locate(q);
     // layer 1: features of this lane's points straight into the A operands
     f32x4 acc1[4];
 #pragma unroll
@@ -323,7 +483,7 @@ __global__ void __launch_bounds__(64 * kWaves) infer_kernel(DecideArgs a, const 
       bf16x4 av = {0, 0, 0, 0};
       if (live && pt < 25) {
         float f[4];
-        point_features(a.F, g, pt % 5 - 2, pt / 5 - 2, f);
+        src.features(loc, pt % 5 - 2, pt / 5 - 2, f);
 #pragma unroll
         for (int v = 0; v < 4; v++) av[v] = (short)f2bf(f[v]);
       }
@@ -372,8 +532,8 @@ __global__ void __launch_bounds__(64 * kWaves) infer_kernel(DecideArgs a, const 
         const unsigned qq = grp * 16 + 4 * quad + v;
         if (qq < n) {
           const float p = sigmoid(z[v]);
-          const unsigned char d = p > a.hi ? 1 : (p < a.lo ? 0 : 2);
-          a.decide[a.cand[qq]] = d;
+          const unsigned char d = p > hi ? 1 : (p < lo ? 0 : 2);
+          src.put(qq, d);
           nweno += d == 1;
           nfall += d == 2;
         }
@@ -385,8 +545,8 @@ __global__ void __launch_bounds__(64 * kWaves) infer_kernel(DecideArgs a, const 
     nfall += __shfl_down(nfall, off);
   }
   if (l == 0) {
-    if (nweno) atomicAdd(a.ncand + 1, nweno);
-    if (nfall) atomicAdd(a.ncand + 2, nfall);
+    if (nweno) atomicAdd(ncand + 1, nweno);
+    if (nfall) atomicAdd(ncand + 2, nfall);
   }
 }
 
@@ -514,8 +674,11 @@ __global__ void __launch_bounds__(kLanes) forward_kernel(const unsigned short* p
   }
 }
 
+// totals: candidates (cells not provably DRP), WENO, left to the sensor.
 __global__ void accumulate_kernel(const unsigned* n, unsigned long long* totals) {
-  if (threadIdx.x < 3) totals[threadIdx.x] += n[threadIdx.x];
+  if (threadIdx.x == 0) totals[0] += (unsigned long long)n[0] + n[3];
+  if (threadIdx.x == 1) totals[1] += n[1];
+  if (threadIdx.x == 2) totals[2] += (unsigned long long)n[2] + n[3];
 }
 
 unsigned blocks_for(long n, long cap = 2048) {
@@ -532,12 +695,23 @@ struct Switch {
   long step = 0;
   TrainBuf t{};
   int cap = 0;               // samples the training buffers hold
-  unsigned* cand = nullptr;  // candidate list of the decisions
+  void* cand = nullptr;  // candidate list of the decisions
   unsigned* ncand = nullptr;
-  long cand_cap = 0;
+  size_t cand_bytes = 0;
 };
 
 namespace {
+
+int grow_cand(Switch* s, size_t bytes) {
+  if (bytes <= s->cand_bytes) return 0;
+  if (s->cand) (void)hipFree(s->cand);
+  s->cand = nullptr;
+  s->cand_bytes = 0;
+  hipError_t e = hipMalloc(&s->cand, bytes);
+  if (e != hipSuccess) return check_hip(e, "hipMalloc(switch candidates)");
+  s->cand_bytes = bytes;
+  return 0;
+}
 
 int grow_train(Switch* s, int n) {
   if (n <= s->cap) return 0;
@@ -571,12 +745,8 @@ Feat make_feat(const Geom& G, const double* w_n, const double* pres, const enl_g
 int switch_decide(Switch* sw, const Geom& G, const double* w_n, const double* pres, const enl_gas_t* gas, float lo,
                   float hi, unsigned char* decide, hipStream_t s) {
   const long ncell = (long)G.nx * G.ny;
-  if (ncell > sw->cand_cap) {
-    if (sw->cand) (void)hipFree(sw->cand);
-    hipError_t e = hipMalloc((void**)&sw->cand, sizeof(unsigned) * (size_t)ncell);
-    if (e != hipSuccess) return check_hip(e, "hipMalloc(switch candidates)");
-    sw->cand_cap = ncell;
-  }
+  int rc = grow_cand(sw, sizeof(unsigned) * (size_t)ncell);
+  if (rc) return rc;
   DecideArgs a;
   a.F = make_feat(G, w_n, pres, gas);
   a.nx = G.nx;
@@ -584,15 +754,41 @@ int switch_decide(Switch* sw, const Geom& G, const double* w_n, const double* pr
   a.g = G.g0;
   a.thr = 5e-5 * sensor_eps(gas->tag_tolerance[2]);
   a.decide = decide;
-  a.cand = sw->cand;
+  a.cand = (unsigned*)sw->cand;
   a.ncand = sw->ncand;
-  a.lo = lo;
-  a.hi = hi;
+  PatchSrc src;
+  src.F = a.F;
+  src.nx = G.nx;
+  src.g = G.g0;
+  src.cand = a.cand;
+  src.decide = decide;
   hipLaunchKernelGGL(decide_reset_kernel, dim3(1), dim3(64), 0, s, sw->ncand);
   const unsigned ntile = (unsigned)(((G.nx + kPBX - 1) / kPBX) * ((G.ny + kPBY - 1) / kPBY));
   hipLaunchKernelGGL(prefilter_kernel, dim3(ntile), dim3(kPBX * kPBY), 0, s, a);
-  hipLaunchKernelGGL(infer_kernel, dim3(1024), dim3(64 * kWaves), 0, s, a, sw->wb);
+  hipLaunchKernelGGL(infer_kernel<PatchSrc>, dim3(1024), dim3(64 * kWaves), 0, s, src, sw->ncand, lo, hi, sw->wb);
   return check_hip(hipGetLastError(), "switch decisions");
+}
+
+int switch_reserve(Switch* sw, long cells) {
+  return grow_cand(sw, sizeof(unsigned long long) * (size_t)(cells > 0 ? cells : 1));
+}
+
+int switch_decide_batch(Switch* sw, const SwPatch* patches, int npatch, int max_side, long cells, double thr,
+                        float inv_r, float inv_p, float lo, float hi, hipStream_t s) {
+  if (npatch <= 0) return 0;
+  if (!sw->cand || max_side <= 0 || sw->cand_bytes < sizeof(unsigned long long) * (size_t)cells)
+    return set_error(-2, "switch_decide_batch: candidate list not reserved for %ld cells", cells);
+  const int nt = ((max_side + kPBX - 1) / kPBX) * ((max_side + kPBY - 1) / kPBY);
+  Batch32Src src;
+  src.pt = patches;
+  src.cand = (const unsigned long long*)sw->cand;
+  src.inv_r = inv_r;
+  src.inv_p = inv_p;
+  hipLaunchKernelGGL(decide_reset_kernel, dim3(1), dim3(64), 0, s, sw->ncand);
+  hipLaunchKernelGGL(prefilter32_kernel, dim3(nt, npatch), dim3(kPBX * kPBY), 0, s, patches, (float)thr,
+                     (unsigned long long*)sw->cand, sw->ncand);
+  hipLaunchKernelGGL(infer_kernel<Batch32Src>, dim3(1024), dim3(64 * kWaves), 0, s, src, sw->ncand, lo, hi, sw->wb);
+  return check_hip(hipGetLastError(), "batched switch decisions");
 }
 
 // Adds the last decisions' counts (candidates, WENO, undecided) to totals.

@@ -67,6 +67,90 @@ def refine_from_base(h, patch, wb):
     return np.ascontiguousarray(out)
 
 
+def evolved_amr_workload(wl, gas, tile0=64, ratio=4):
+    """The two-level hierarchy of an AMR workload on the device: the base
+    grid (one patch) advanced wl.presteps fused steps from the analytic
+    initial state, tag_cells on that state, one level refined by `ratio`
+    over the clustered tags, and every patch's state from the evolved base
+    (refine_from_base).  Returns (h, states, tags)."""
+    import torch
+    from . import patch as P
+    from .workloads import initial_fields
+    n, g = wl.n, wl.g
+    base = P.DevicePatch(wl, gas)
+    base.init_state(*initial_fields(wl, gas))
+    if wl.presteps:
+        base.advance_fused(wl.presteps, wl.dt(gas))
+    P.fill_physical_bc(base.box, g, (1, 1, 1, 1), base.reflect, base.w, base.p, base.T, base.src)
+    torch.cuda.synchronize()
+    wb = base.w.cpu().numpy()
+    del base
+    tags = np.zeros((n, n), np.int32)
+    P.tag_cells((0, n - 1, 0, n - 1), g, tags, wb, np.zeros((n + 2 * g, n + 2 * g)), 1, gas.tag_tolerance)
+    h = amr.build(n, ratio, g, 1.0, tags, tile0=tile0)
+    states = [refine_from_base(h, p, wb) for p in h.patches]
+    return h, states, tags
+
+
+def weak_acoustic_workload(wl, gas, tags, tile0=64, ratio=4):
+    """The FP32-perturbation workload on an AMR workload's boxes: the
+    hierarchy clustered from `tags` on the wl.L domain, per patch the
+    stratified state carrying wl's pulses and its ambient rows (rho, rho s,
+    rho T_n at each ghosted row), and the per-level ambient function that
+    DeviceHierarchy.set_ambient takes.  Returns (h, states, ambients, fn)."""
+    from .workloads import ambient_rows, stratified_state
+    h = amr.build(wl.n, ratio, wl.g, wl.L, tags, tile0=tile0)
+    states, ambs = [], []
+    for p in h.patches:
+        dx = h.dx(p.level)
+        x = (np.arange(p.box[0] - p.g, p.box[1] + p.g + 1) + 0.5) * dx
+        y = (np.arange(p.box[2] - p.g, p.box[3] + p.g + 1) + 0.5) * dx
+        states.append(stratified_state(wl, gas, x, y))
+        ambs.append(ambient_rows(wl, gas, y))
+
+    def ambient(lev):
+        n, g = h.ncells(lev), h.g
+        y = (np.arange(-g, n + g) + 0.5) * h.dx(lev)
+        return (-g,) + ambient_rows(wl, gas, y)
+
+    return h, states, ambs, ambient
+
+
+PERTURBATION_FIELDS = ("rho", "rho_u", "rho_v", "rho_s", "rho_Tn", "p", "T")
+
+
+def perturbation_errors(h, ref, test, ambients, gas):
+    """Per field of two per-patch state lists [(w, p, T), ...] over every
+    patch interior: rel_l2 and rel_linf of `test` against `ref`, and
+    rel_l2_of_perturbation, the L2 error over the L2 norm of ref's
+    perturbation (rho, rho s, rho T_n less their ambient rows, p less p0,
+    the momenta themselves; none for T)."""
+    err = {}
+    for f_i, name in enumerate(PERTURBATION_FIELDS):
+        num = den = dev = linf = amax = 0.0
+        for p, a, b, am in zip(h.patches, ref, test, ambients):
+            g = p.g
+            pick = (lambda st: st[0][f_i]) if f_i < 5 else (lambda st: st[f_i - 4])
+            x, y = pick(a)[g:-g, g:-g], pick(b)[g:-g, g:-g]
+            if f_i in (0, 3, 4):
+                pert = x - am[(0, None, None, 1, 2)[f_i]][g:-g, None]
+            elif f_i in (1, 2):
+                pert = x
+            elif f_i == 5:
+                pert = x - gas.p0
+            else:
+                pert = None
+            num += float(np.sum((x - y) ** 2))
+            den += float(np.sum(x ** 2))
+            if pert is not None:
+                dev += float(np.sum(pert ** 2))
+            linf = max(linf, float(np.abs(x - y).max()))
+            amax = max(amax, float(np.abs(x).max()))
+        err[name] = {"rel_l2": float(np.sqrt(num / den)), "rel_linf": linf / amax,
+                     "rel_l2_of_perturbation": float(np.sqrt(num / dev)) if dev > 0 else None}
+    return err
+
+
 class DeviceHierarchy:
     def __init__(self, h, gas, reflect=(1, 1, 1, 1)):
         self.h, self.gas, self.reflect = h, gas, tuple(reflect)
@@ -354,6 +438,27 @@ class DeviceHierarchy:
         vid = _lib.VARIANT_ID[variant] if isinstance(variant, str) else int(variant)
         gs = self.gas.to_struct()
         _lib.check(self.lib.enl_batch_fp32_prepare(self.handle, vid, ctypes.byref(gs), self._reflect()))
+
+    def fp32_set_switch(self, sw, lo=0.2, hi=0.8, every_stage=True, totals=None):
+        """The learned switch (a switch.Switch, or None for the sensor) on
+        level 0 of the FP32 form: its decisions are made over all level-0
+        patches before level 0's rhs, every stage or once per step
+        (every_stage=False).  totals: a device int64 tensor of 3 that gains
+        (candidates, WENO, undecided) per decision, or None."""
+        if totals is not None:
+            assert totals.is_cuda and totals.dtype.itemsize == 8 and totals.numel() >= 3
+        self._switch, self._switch_totals = sw, totals  # both must outlive their use
+        _lib.check(self.lib.enl_batch_fp32_set_switch(
+            self.handle, sw.handle if sw is not None else None, float(lo), float(hi), int(bool(every_stage)),
+            ctypes.c_void_p(totals.data_ptr()) if totals is not None else None))
+
+    def fp32_decisions(self, pid):
+        """The switch's last decision map of level-0 patch pid (ny x nx uint8:
+        0 DRP, 1 WENO, 2 the sensor decides)."""
+        p = self.h.patches[pid]
+        out = np.empty((p.ny, p.nx), np.uint8)
+        _lib.check(self.lib.enl_batch_fp32_decisions(self.handle, int(pid), _p(out)))
+        return out
 
     def fp32_load(self):
         _lib.check(self.lib.enl_batch_fp32_load(self.handle))

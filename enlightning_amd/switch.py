@@ -99,10 +99,10 @@ class Switch:
                                               _ptr(d), counts))
         return d, tuple(counts)
 
-    def advance(self, dp, nsteps, dt, lo, hi, variant=None, rk=RK3):
-        """enl_advance_patch_fused with this switch deciding level 0's cells;
-        returns the summed counts (candidates, WENO, undecided) over the
-        stages."""
+    def advance(self, dp, nsteps, dt, lo, hi, variant=None, rk=RK3, every_stage=True):
+        """enl_advance_patch_fused with this switch deciding level 0's cells
+        before every stage, or once per step (every_stage=False); returns the
+        summed counts (candidates, WENO, undecided) over the decisions."""
         lib = self.lib
         P.use_torch_stream()
         variant = dp.wl.variant if variant is None else variant
@@ -117,7 +117,7 @@ class Switch:
         D = ctypes.c_double * len(a1)
         gs = dp.gas.to_struct()
         _lib.check(lib.enl_advance_patch_learned(
-            self.handle, float(lo), float(hi), _ptr(dp._decide), _ptr(totals), vid, dp.box[0], dp.box[1],
+            self.handle, float(lo), float(hi), int(bool(every_stage)), _ptr(dp._decide), _ptr(totals), vid, dp.box[0], dp.box[1],
             dp.box[2], dp.box[3], dp.g, D(*dp.xlo), D(*dp.dx), ctypes.byref(gs), (ctypes.c_int * 4)(*dp.reflect),
             len(a1), D(*a1), D(*a2), D(*b), float(dt), int(nsteps), _ptr(dp.w), _ptr(dp.p), _ptr(dp.T),
             _ptr(dp.src), _ptr(dp._scratch), nbytes, float(getattr(dp, "absorb_width", 0.0))))
@@ -216,7 +216,8 @@ def sensor_map(dp):
 
 def candidates(dp):
     """The prefilter's candidates (bool ny x nx): some sensor pair with
-    t1^2 + t2^2 >= 5e-5 eps."""
+    both t1^2 + t2^2 and (t1 - t2)^2 at or above 5e-5 eps (either below it
+    proves the sensor's DRP)."""
     from .gas import sensor_eps
     g, n = dp.g, dp.n
     r = dp.w[0]
@@ -228,10 +229,10 @@ def candidates(dp):
     for q in (-1, 0, 1):
         t1 = dxr[J, g - 1 + q - 1 + 1:g - 1 + q + n]          # r[i+q] - r[i-1+q]
         t2 = dxr[J, g + q:g + q + n]                          # r[i+1+q] - r[i+q]
-        cand |= t1 * t1 + t2 * t2 >= thr
+        cand |= (t1 * t1 + t2 * t2 >= thr) & ((t1 - t2) * (t1 - t2) >= thr)
         t1 = dyr[g - 1 + q:g - 1 + q + n, g:g + n]
         t2 = dyr[g + q:g + q + n, g:g + n]
-        cand |= t1 * t1 + t2 * t2 >= thr
+        cand |= (t1 * t1 + t2 * t2 >= thr) & ((t1 - t2) * (t1 - t2) >= thr)
     return cand
 
 
@@ -313,6 +314,68 @@ def evaluate(sw, X, y, lo=0.2, hi=0.8):
         "missed_shocks": int((pos & drp).sum().item()),
         "false_weno": int((~pos & weno).sum().item()),
     }
+
+
+def channel_probabilities(sw, wl, gas, seeds, snapshots=(0, 20, 60, 100), dt=None):
+    """p(WENO) of the network on every prefilter candidate of the given
+    channels at the given steps of the plain FP64 fused advance, split by the
+    sensor's own choice there: (p of sensor-WENO cells, p of sensor-DRP
+    candidates), float32 host arrays, and the level-0 cell count visited."""
+    dt = wl.dt(gas) if dt is None else dt
+    pw, pd = [], []
+    visited = 0
+    for seed in seeds:
+        dp = channel_patch(wl, gas, seed)
+        done = 0
+        for st in snapshots:
+            if st > done:
+                dp.advance_fused(st - done, dt)
+                done = st
+            lab = sensor_map(dp).flatten()
+            cand = candidates(dp).flatten()
+            assert not bool((lab.bool() & ~cand).any().item()), "a sensor-WENO cell outside the prefilter's candidates"
+            for sel, out in ((cand & (lab == 1), pw), (cand & (lab == 0), pd)):
+                cells = torch.nonzero(sel).flatten()
+                for c0 in range(0, cells.numel(), 1 << 20):
+                    X = sw.features(dp, cells[c0:c0 + (1 << 20)])
+                    out.append(sw.forward(X).cpu().numpy())
+            visited += wl.n * wl.n
+        del dp
+        torch.cuda.empty_cache()
+    cat = lambda a: np.concatenate(a) if a else np.zeros(0, np.float32)
+    return cat(pw), cat(pd), visited
+
+
+def calibrate_lo(p_weno, margin=0.5):
+    """The DRP threshold that calls none of the given sensor-WENO cells DRP:
+    margin times the smallest p among them (the matrix-core inference may
+    round p differently from the forward kernel by far less than that)."""
+    return float(margin * p_weno.min()) if p_weno.size else 0.0
+
+
+def decision_shares(p_weno, p_drp, lo, hi):
+    """Counts of the network's calls on candidates at thresholds (lo, hi):
+    missed shocks (sensor WENO, p < lo), network DRP, network WENO, left to
+    the sensor."""
+    allp = np.concatenate([p_weno, p_drp])
+    return {"candidates": int(allp.size), "sensor_weno": int(p_weno.size),
+            "missed_shocks": int((p_weno < lo).sum()), "network_drp": int((allp < lo).sum()),
+            "network_weno": int((allp > hi).sum()), "to_sensor": int(((allp >= lo) & (allp <= hi)).sum()),
+            "false_weno": int((p_drp > hi).sum())}
+
+
+THRESHOLDS_FILE = os.path.join(os.path.dirname(__file__), "data", "switch_thresholds.json")
+
+
+def load_thresholds(default=(0.2, 0.8)):
+    """(lo, hi) calibrated for the shipped parameters (train_switch
+    --calibrate), or `default` if no calibration is shipped."""
+    import json
+    if os.path.exists(THRESHOLDS_FILE):
+        with open(THRESHOLDS_FILE) as f:
+            t = json.load(f)
+        return float(t["lo"]), float(t["hi"])
+    return default
 
 
 def load_params():

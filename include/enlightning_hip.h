@@ -355,6 +355,26 @@ int enl_batch_fp32_prepare(void *handle, int variant, const enl_gas_t *gas, cons
 int enl_batch_fp32_load(void *handle);
 /* FP32 form -> the current FP64 state (ambient + perturbation). */
 int enl_batch_fp32_store(void *handle);
+/* The learned switch (enl_switch_create) in place of the density sensor on
+ * level 0 of the FP32 form (rhs_hybrid_high_llf.m4:239-256, the level-0
+ * patches of the loop at enlightning.C:459-583): before level 0's rhs the
+ * prefilter (on the full density) and the network (on the differences of
+ * the perturbations rho' and p', the deviation from the ambient) run over
+ * every level-0 patch at once and write one decision map per patch (0 DRP,
+ * 1 WENO, 2 the sensor decides), which the stage kernel reads.  A cell that
+ * is a candidate only through the ambient's own profile (its perturbation
+ * pairs are provably smooth) is left to the sensor without a network call.  every_stage != 0 decides before each RK stage;
+ * every_stage == 0 decides at the first stage of a step and keeps the map
+ * for the step's other stages.  totals (device, 3 values, or NULL) gains
+ * per decision the candidates (cells not provably DRP), the network's WENO
+ * cells and the cells left to the sensor.  sw == NULL restores the sensor.  The switch must
+ * outlive its use by this batch.  With lo = 0, hi = 1 and every_stage set,
+ * every decision is the sensor's. */
+int enl_batch_fp32_set_switch(void *handle, void *sw, float lo, float hi, int every_stage,
+                              unsigned long long *totals);
+/* The decision map of level-0 patch pid as the switch last wrote it
+ * (nx*ny bytes, row by row, host or device memory). */
+int enl_batch_fp32_decisions(void *handle, int pid, unsigned char *out);
 
 /* ---- learned WENO/DRP switch (no reference counterpart; stands in for the
  * hybrid variants' density sensor, rhs_hybrid_high_llf.m4:239-256) --------
@@ -387,9 +407,12 @@ int enl_switch_decide(void *handle, int ifirst0, int ilast0, int ifirst1, int il
                       const double *pres, const enl_gas_t *gas, float lo, float hi, unsigned char *decide,
                       unsigned *counts);
 /* enl_advance_patch_fused on level 0 of a hybrid variant with the learned
- * switch deciding every stage; totals (device, 3 x u64, optional) accumulate
+ * switch deciding level 0's cells: before every RK stage (every_stage != 0)
+ * or at the first stage of each step, the map kept for the step's other
+ * stages (every_stage == 0); totals (device, 3 x u64, optional) accumulate
  * the decision counts. */
-int enl_advance_patch_learned(void *sw, float lo, float hi, unsigned char *decide, unsigned long long *totals,
+int enl_advance_patch_learned(void *sw, float lo, float hi, int every_stage, unsigned char *decide,
+                              unsigned long long *totals,
                               int variant, int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw,
                               const double *xlo, const double *dx, const enl_gas_t *gas, const int *reflect, int order,
                               const double *alpha_1, const double *alpha_2, const double *beta, double dt, int nsteps,
