@@ -248,11 +248,15 @@ def test_pulse_source_zeroed_after_its_step():
             cpu.zero_source()
             dev.zero_source()
             assert all(not np.any(s) for s in dev.sources())
+        # fields still at rounding level where the wave has not arrived (the
+        # momenta of a patch away from the channel: ~1e-7 of rho0*c, built
+        # from differences of pressure-scale fluxes) are compared against
+        # 1e-14 of their natural scale instead of their own size
         for (wc, _, _), (wd, _, _), p in zip(cpu.states(), dev.states(), h.patches):
             g = p.g
             for k in range(5):
                 a, b = wd[k, g:-g, g:-g], wc[k, g:-g, g:-g]
-                assert np.abs(a - b).max() <= 1e-10 * max(np.abs(b).max(), 1e-6 * natural[k]), (it, p.pid, k)
+                assert np.abs(a - b).max() <= 1e-10 * max(np.abs(b).max(), 1e-4 * natural[k]), (it, p.pid, k)
     assert added == 2
     assert not same(dev.states(), keep.states(), h)
     dev.close()
