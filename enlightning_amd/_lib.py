@@ -95,6 +95,11 @@ PROTOTYPES = {
     "enl_batch_fp32_load": ([_P], _I),
     "enl_batch_fp32_set_switch": ([_P, _P, ctypes.c_float, ctypes.c_float, _I, _P], _I),
     "enl_batch_fp32_decisions": ([_P, _I, _P], _I),
+    "enl_plan_build": ([_I, _I, _I, _I, _P, _I, _I, _I, _P, _P], _I),
+    "enl_plan_from_boxes": ([_I, _I, _I, _I, _P, _P], _I),
+    "enl_plan_transfer": ([_P, _P, _I, _P], _I),
+    "enl_plan_get": ([_P, ctypes.c_char_p, _I, _P, _P], _I),
+    "enl_plan_free": ([_P], _I),
     "enl_batch_fp32_store": ([_P], _I),
     "enl_batch_patch_state": ([_P, _I] + [_P] * 4, _I),
     "enl_batch_copy_state": ([_P, _I, _I] + [_P] * 4, _I),
@@ -126,6 +131,12 @@ def load():
         raise RuntimeError(
             "libenlightning_hip.so is not built (expected at %s); run "
             "`python -c 'import __graft_entry__ as g; g.build()'`" % LIB_PATH)
+    # torch first: the library then binds to the HIP runtime torch has
+    # loaded, so that device pointers of torch tensors are its own
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, (argtypes, restype) in PROTOTYPES.items():
         fn = getattr(lib, name)

@@ -38,6 +38,7 @@ per-patch kernels (not vendored, so none of it is parity-pinned):
 All cell addresses are (patch id, local index) pairs with local index
 (i - ifirst0 + g) + Nx * (j - ifirst1 + g) inside the patch's ghosted box.
 """
+import ctypes
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -82,6 +83,8 @@ class Hierarchy:
     coarsen: dict = None
     # owner map per level (level index space -> pid or -1), built on demand
     owners: dict = field(default_factory=dict, repr=False, compare=False)
+    # the native plan (enl_plan_*) whose lists fill and coarsen view, if any
+    plan: object = field(default=None, repr=False, compare=False)
 
     @property
     def nlevels(self):
@@ -209,11 +212,85 @@ def _owner_at(h, level, i, j):
     return own[j, i] if s == 1 else own[j // s, i // s]
 
 
-def build(n0, ratio, g, L, coarse_tags, tile0=64, tag_buffer=2, fine_max=64, fine_min=8, previous=None):
+class NativePlan:
+    """A plan of the library's regrid planner (enl_plan_*, enl_regrid.hip):
+    the same boxes and lists as the numpy builders here, built in C++.
+    Arrays taken from it view its memory and keep it alive."""
+
+    def __init__(self, handle):
+        from . import _lib
+        self.lib = _lib.load()
+        self.handle = handle
+
+    def get(self, name, level=0, copy=False):
+        from . import _lib
+        data, count = ctypes.c_void_p(), ctypes.c_long()
+        kind = self.lib.enl_plan_get(self.handle, name.encode(), int(level), ctypes.byref(data), ctypes.byref(count))
+        _lib.check(0 if kind in (0, 1) else kind)
+        ct = ctypes.c_double if kind == 1 else ctypes.c_int32
+        if count.value == 0:
+            return np.zeros(0, np.float64 if kind == 1 else np.int32)
+        buf = (ct * count.value).from_address(data.value)
+        buf._owner = self  # the plan outlives the view
+        a = np.ctypeslib.as_array(buf)
+        return a.copy() if copy else a
+
+    def fill(self, level):
+        f = lambda n: self.get("fill." + n, level)  # noqa: E731
+        return {"copy": tuple(f("copy." + k) for k in ("p", "dst", "sp", "sl")),
+                "interp": tuple(f("interp." + k) for k in ("p", "dst", "cp", "cl", "fx", "fy")),
+                "edge": tuple(f("edge." + k) for k in ("p", "dst", "src", "neg")),
+                "corner": tuple(f("corner." + k) for k in ("p", "dst", "src", "neg"))}
+
+    def coarsen(self):
+        return tuple(self.get("coarsen." + k) for k in ("cp", "cl", "fp", "fl"))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self.lib.enl_plan_free(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def _boxes_array(h):
+    return np.ascontiguousarray([[p.level] + list(p.box) for p in h.patches], dtype=np.int32).reshape(-1, 5)
+
+
+def _from_plan(plan, n0, ratio, g, L):
+    h = Hierarchy(n0=n0, ratio=ratio, g=g, L=L)
+    for row in plan.get("boxes").reshape(-1, 5):
+        h.patches.append(Patch(len(h.patches), int(row[0]), tuple(int(v) for v in row[1:]), g))
+    h.fill = [plan.fill(lev) for lev in range(h.nlevels)]
+    h.coarsen = plan.coarsen() if h.nlevels > 1 else None
+    h.plan = plan
+    return h
+
+
+def build(n0, ratio, g, L, coarse_tags, tile0=64, tag_buffer=2, fine_max=64, fine_min=8, previous=None, native=True):
     """Hierarchy with level 0 = base patches and, if any cell is tagged, a
     level-1 of clustered boxes; with its ghost-fill and coarsen schedules.
     At a regrid, `previous` (the hierarchy being replaced) lends its level-0
-    fill schedule when its base patches are the same."""
+    fill schedule when its base patches are the same.  native: the library's
+    planner (enl_plan_build) builds it; native=False: the numpy builders
+    below, which state the same lists (the tests hold the two equal)."""
+    if native:
+        from . import _lib
+        lib = _lib.load()
+        tags = None if coarse_tags is None else np.ascontiguousarray(coarse_tags, dtype=np.int32)
+        prev = previous.plan.handle if previous is not None and previous.plan is not None else None
+        handle = ctypes.c_void_p()
+        _lib.check(lib.enl_plan_build(n0, ratio, g, tile0, None if tags is None else ctypes.c_void_p(tags.ctypes.data),
+                                      tag_buffer, fine_max, fine_min, prev, ctypes.byref(handle)))
+        h = _from_plan(NativePlan(handle), n0, ratio, g, L)
+        if (previous is not None and (previous.n0, previous.ratio, previous.g) == (n0, ratio, g)
+                and [p.box for p in previous.level_patches(0)] == [p.box for p in h.level_patches(0)]
+                and [p.pid for p in previous.level_patches(0)] == [p.pid for p in h.level_patches(0)]):
+            h.fill[0] = previous.fill[0]  # the same lists: lent, as the numpy builder lends them
+            if 0 in previous.owners:
+                h.owners[0] = previous.owners[0]
+        return h
     h = Hierarchy(n0=n0, ratio=ratio, g=g, L=L)
     for b in base_boxes(n0, tile0, g):
         h.patches.append(Patch(len(h.patches), 0, b, g))
@@ -224,9 +301,17 @@ def build(n0, ratio, g, L, coarse_tags, tile0=64, tag_buffer=2, fine_max=64, fin
     return _schedules(h, previous)
 
 
-def from_boxes(n0, ratio, g, L, boxes):
+def from_boxes(n0, ratio, g, L, boxes, native=True):
     """Hierarchy from (level, ifirst0, ilast0, ifirst1, ilast1) rows in pid
     order (a restart's boxes), with its schedules."""
+    if native:
+        from . import _lib
+        lib = _lib.load()
+        b = np.ascontiguousarray(boxes, dtype=np.int32).reshape(-1, 5)
+        handle = ctypes.c_void_p()
+        _lib.check(lib.enl_plan_from_boxes(n0, ratio, g, len(b), ctypes.c_void_p(b.ctypes.data),
+                                           ctypes.byref(handle)))
+        return _from_plan(NativePlan(handle), n0, ratio, g, L)
     h = Hierarchy(n0=n0, ratio=ratio, g=g, L=L)
     for row in boxes:
         lev, b = int(row[0]), tuple(int(v) for v in row[1:5])
@@ -405,7 +490,7 @@ def make_coarsen(h):
             _locals(B0[fp], B2[fp], NX[fp], g, IC * r, JC * r))
 
 
-def make_transfer(new, old):
+def make_transfer(new, old, native=True):
     """Regrid transfer schedule from hierarchy `old` to `new` (same base
     grid, ratio and ghost width).  A new patch with the level and box of an
     old one is a "same" pair (new pid, old pid), copied whole; for every
@@ -417,6 +502,17 @@ def make_transfer(new, old):
     fill takes them afterwards."""
     if (new.n0, new.ratio, new.g) != (old.n0, old.ratio, old.g):
         raise ValueError("make_transfer: the hierarchies differ in base grid, ratio or ghost width")
+    if native and new.plan is not None:
+        from . import _lib
+        ob = _boxes_array(old)
+        rc = new.plan.lib.enl_plan_transfer(new.plan.handle, old.plan.handle if old.plan is not None else None,
+                                            len(ob), ctypes.c_void_p(ob.ctypes.data))
+        if rc:
+            raise ValueError("make_transfer: %s" % _lib.last_error()[1])
+        t = lambda n: new.plan.get("transfer." + n, copy=True)  # noqa: E731
+        return {"same": (t("same.new"), t("same.old")),
+                "copy": tuple(t("copy." + k) for k in ("dp", "dl", "sp", "sl")),
+                "interp": tuple(t("interp." + k) for k in ("dp", "dl", "sp", "sl", "fx", "fy"))}
     r, g = new.ratio, new.g
     old_box = {(p.level, p.box): p.pid for p in old.patches}
     s_new, s_old, rest = [], [], []

@@ -34,31 +34,23 @@ __global__ void __launch_bounds__(256) rk_kernel(RkArgs a) {
 // consecutive cells of a row (one double2 per array and component).  Valid
 // when nx and g0 are even and every base pointer is 16-byte aligned, so the
 // interior of each ghosted row starts on a 16-byte boundary.
-// blockIdx.y = row j; the 15 loads of a row's five components are issued
-// before its first store, so they are in flight together.
 __global__ void __launch_bounds__(256) rk_kernel_v(RkArgs a) {
   const Geom& G = a.geom;
   const int ip = blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= G.nx / 2) return;
   const double bdt = a.beta * a.dt;  // beta*dt*KK groups as (beta*dt)*KK
   const long ik = (long)G.nx * G.ny;
-  for (int j = blockIdx.y; j < G.ny; j += gridDim.y) {
-    const long g = (long)(2 * ip + G.g0) + (long)(j + G.g1) * G.sy;
-    const long c = (long)(2 * ip) + (long)j * G.nx;
-    double2 w[5], v[5], K[5];
-#pragma unroll
-    for (int k = 0; k < 5; k++) {
-      w[k] = *(const double2*)(a.w + g + k * G.sk);
-      v[k] = *(const double2*)(a.ww + c + k * ik);
-      K[k] = *(const double2*)(a.KK + c + k * ik);
-    }
-#pragma unroll
-    for (int k = 0; k < 5; k++) {
-      double2 o;
-      o.x = a.alpha_1 * v[k].x + a.alpha_2 * w[k].x + bdt * K[k].x;
-      o.y = a.alpha_1 * v[k].y + a.alpha_2 * w[k].y + bdt * K[k].y;
-      *(double2*)(a.w + g + k * G.sk) = o;
-    }
+  for (int row = blockIdx.y; row < G.ny * 5; row += gridDim.y) {
+    const int j = row % G.ny, k = row / G.ny;
+    const long g = (long)(2 * ip + G.g0) + (long)(j + G.g1) * G.sy + k * G.sk;
+    const long c = (long)(2 * ip) + (long)j * G.nx + k * ik;
+    double2 w = *(const double2*)(a.w + g);
+    double2 v = *(const double2*)(a.ww + c);
+    double2 K = *(const double2*)(a.KK + c);
+    double2 o;
+    o.x = a.alpha_1 * v.x + a.alpha_2 * w.x + bdt * K.x;
+    o.y = a.alpha_1 * v.y + a.alpha_2 * w.y + bdt * K.y;
+    *(double2*)(a.w + g) = o;
   }
 }
 
@@ -209,7 +201,7 @@ inline unsigned rows_grid(int rows) { return (unsigned)(rows < 16384 ? rows : 16
 hipError_t launch_runge_kutta(const RkArgs& a, hipStream_t s) {
   if (a.geom.nx <= 0 || a.geom.ny <= 0) return hipSuccess;
   if (vec_ok(a.geom) && aligned16(a.w) && aligned16(a.ww) && aligned16(a.KK)) {
-    dim3 grid((a.geom.nx / 2 + 255) / 256, rows_grid(a.geom.ny));
+    dim3 grid((a.geom.nx / 2 + 255) / 256, rows_grid(a.geom.ny * 5));
     hipLaunchKernelGGL(rk_kernel_v, grid, dim3(256), 0, s, a);
     return hipGetLastError();
   }

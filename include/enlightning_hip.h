@@ -335,6 +335,39 @@ int enl_batch_tag_cells(void *handle, int level, const int *reflect, int refine_
                         const double *tag_tolerance, int *tags);
 int enl_batch_destroy(void *handle);
 
+/* ---- regrid planning on the host (gridding_algorithm->regridAllFinerLevels,
+ * main.C:489-502; the refine and coarsen schedules enlightning.C:259-288
+ * registers) -------------------------------------------------------------
+ * A plan holds a two-level hierarchy's boxes and its schedules as flat
+ * integer (and double) lists, in the layout enl_batch_set_fill,
+ * enl_batch_set_coarsen and enl_batch_transfer take.  Boxes are 5 ints per
+ * patch in pid order: level, ifirst0, ilast0, ifirst1, ilast1. */
+/* Level 0: the n0 x n0 base grid in tiles of tile0 (a last tile narrower
+ * than g joins the one before it); level 1, when any of the n0*n0 coarse
+ * tags (row by row, may be NULL) is set: the tags dilated by tag_buffer,
+ * clustered per block of fine_max/ratio coarse cells into bounding boxes
+ * (at least fine_min/ratio coarse cells a side inside the block, extended
+ * to the domain edge when within one coarse cell of it), refined by ratio.
+ * previous (a plan or NULL) lends its level-0 fill schedule when its base
+ * boxes are the same. */
+int enl_plan_build(int n0, int ratio, int g, int tile0, const int *tags, int tag_buffer, int fine_max, int fine_min,
+                   const void *previous, void **plan);
+/* A plan of given boxes (a restart's), with its schedules. */
+int enl_plan_from_boxes(int n0, int ratio, int g, int npatch, const int *boxes, void **plan);
+/* The regrid transfer from the old hierarchy (old_plan, or when that is
+ * NULL the n_old boxes old_boxes) into the plan's: unchanged patches whole
+ * ("transfer.same.new/old"), every other interior cell copied from the old
+ * patch of its level ("transfer.copy.dp/dl/sp/sl") or refined from the old
+ * coarser level ("transfer.interp.dp/dl/sp/sl/fx/fy"). */
+int enl_plan_transfer(void *plan, void *old_plan, int n_old, const int *old_boxes);
+/* A list of the plan: "boxes", "fill.copy.p/dst/sp/sl", "fill.interp.p/dst/
+ * cp/cl/fx/fy", "fill.edge.p/dst/src/neg", "fill.corner.p/dst/src/neg" (of
+ * one level), "coarsen.cp/cl/fp/fl", the transfer lists.  *data points into
+ * the plan (valid until enl_plan_free), *count is its length; returns 1 for
+ * a list of doubles, 0 for ints (or an absent, empty list). */
+int enl_plan_get(const void *plan, const char *name, int level, const void **data, long *count);
+int enl_plan_free(void *plan);
+
 /* ---- FP32-perturbation form of the batched hierarchy (no reference
  * counterpart; the precision form of the hot path) -------------------------
  * The state is stored as FP32 deviations from an FP64 ambient that depends
