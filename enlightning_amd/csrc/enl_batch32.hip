@@ -62,11 +62,13 @@ struct Patch32 {
 // and FP64 differences rounded once.
 struct RowF {
   double aW[3];  // rho, rho*s, rho*T_n
+  double aS64, aTn64;  // s and T_n in FP64: their row differences are formed
+                       // in FP64 and rounded once
   float aRho, aS, aTn, aT, aD;  // rho, s, T_n, T, rho - rho0
   float cy2, TmTn, pA;          // (c + c_lr*y)^2, T - T_n, p - p0
   float d1Rho, d1T, d2T, pad;   // rho(r+1)-rho(r), T(r+1)-T(r-1), T(r-1)-2T(r)+T(r+1)
 };
-static_assert(sizeof(RowF) % 4 == 0, "RowF is copied word by word");
+static_assert(sizeof(RowF) % 8 == 0, "RowF is copied word by word and holds doubles");
 
 // Ambient value of stored component k (w_n 0..4, p, T) from a row record.
 __host__ __device__ inline double amb_of(const double* rec, int k) {
@@ -230,15 +232,19 @@ struct Lds32 {
   RowF row[TH];  // per staged row
 };
 
+// The y fluxes of the components whose ambient depends on height (rho,
+// rho*s, rho*T_n), in FP64 for the FP64 reconstruction: the stored
+// momentum times the row's FP64 ambient s or T_n plus the perturbation
+// (rho v s = (rho v) s), so that the ambient's row-to-row change is not
+// rounded to FP32 (a rounding that stays with the row and biases every
+// step alike).
 template <int BX, int BY>
-__device__ inline float flux_y(const Lds32<BX, BY>& L, int e, int r, int k) {
-  const float rv = L.rho[e] * L.v[e];
+__device__ inline double flux_y64(const Lds32<BX, BY>& L, int e, int r, int k) {
+  const double m = (double)L.w[2][e];
   switch (k) {
-    case 0: return rv;
-    case 1: return rv * L.u[e];
-    case 2: return rv * L.v[e] + (L.p[e] + L.row[r].pA);
-    case 3: return rv * (L.row[r].aS + L.s[e]);
-    default: return rv * (L.row[r].aTn + L.tn[e]);
+    case 0: return m;
+    case 3: return m * (L.row[r].aS64 + (double)L.s[e]);
+    default: return m * (L.row[r].aTn64 + (double)L.tn[e]);
   }
 }
 
@@ -250,7 +256,10 @@ __device__ inline float flux_y(const Lds32<BX, BY>& L, int e, int r, int k) {
 // smooth field's nearly cancelling fluxes lose no more than the rounding of
 // their difference.  The fluxes are those of rhs_hybrid_high_llf.m4:125-156
 // on full values (ambient + perturbation); the mass flux is the stored
-// momentum rather than rho times its quotient u.
+// momentum rather than rho times its quotient u.  Along y the ambient s and
+// T_n change from row to row: that change is formed in FP64 and rounded
+// once (each row's FP32 value carries its own fixed rounding, which the
+// difference of two rows would keep at full size).
 template <int BX, int BY>
 __device__ inline float dflux_x(const Lds32<BX, BY>& L, int e1, int e2, int r, int k) {
   const float m1 = L.w[1][e1], m2 = L.w[1][e2];
@@ -270,9 +279,11 @@ __device__ inline float dflux_y(const Lds32<BX, BY>& L, int e1, int e2, int r1, 
     case 1: return m1 * L.u[e1] - m2 * L.u[e2];
     case 2: return ((L.p[e1] - L.p[e2]) + (L.row[r1].pA - L.row[r2].pA)) + (m1 * L.v[e1] - m2 * L.v[e2]);
     case 3:
-      return (L.row[r1].aS * (m1 - m2) + (L.row[r1].aS - L.row[r2].aS) * m2) + (m1 * L.s[e1] - m2 * L.s[e2]);
+      return (L.row[r1].aS * (m1 - m2) + (float)(L.row[r1].aS64 - L.row[r2].aS64) * m2) +
+             (m1 * L.s[e1] - m2 * L.s[e2]);
     default:
-      return (L.row[r1].aTn * (m1 - m2) + (L.row[r1].aTn - L.row[r2].aTn) * m2) + (m1 * L.tn[e1] - m2 * L.tn[e2]);
+      return (L.row[r1].aTn * (m1 - m2) + (float)(L.row[r1].aTn64 - L.row[r2].aTn64) * m2) +
+             (m1 * L.tn[e1] - m2 * L.tn[e2]);
   }
 }
 
@@ -455,7 +466,7 @@ __global__ void __launch_bounds__(BX* BY) stage32_kernel(const int4* __restrict_
         double Fd[7], Wd[7];
 #pragma unroll
         for (int m = 0; m < 7; m++) {
-          Fd[m] = (double)flux_y(L, e + (m - 3) * S, r + m - 3, k);
+          Fd[m] = flux_y64(L, e + (m - 3) * S, r + m - 3, k);
           Wd[m] = L.row[r + m - 3].aW[c] + (double)L.w[k][e + (m - 3) * S];
         }
         y = (float)weno_dir64(Fd, Wd, (double)ag);
@@ -897,6 +908,8 @@ int setup32(Batch* b) {
         R.aW[0] = rec[0];
         R.aW[1] = rec[1];
         R.aW[2] = rec[2];
+        R.aS64 = rec[5];
+        R.aTn64 = rec[6];
         R.aRho = (float)rec[0];
         R.aS = (float)rec[5];
         R.aTn = (float)rec[6];

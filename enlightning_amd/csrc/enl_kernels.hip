@@ -45,8 +45,13 @@ __global__ void __launch_bounds__(256) rk_kernel_v(RkArgs a) {
     const long g = (long)(2 * ip + G.g0) + (long)(j + G.g1) * G.sy + k * G.sk;
     const long c = (long)(2 * ip) + (long)j * G.nx + k * ik;
     double2 w = *(const double2*)(a.w + g);
-    double2 v = *(const double2*)(a.ww + c);
-    double2 K = *(const double2*)(a.KK + c);
+    // ww and KK are read once per step: non-temporal loads keep them from
+    // displacing w, which the next kernels read again
+    double2 v, K;
+    v.x = __builtin_nontemporal_load(a.ww + c);
+    v.y = __builtin_nontemporal_load(a.ww + c + 1);
+    K.x = __builtin_nontemporal_load(a.KK + c);
+    K.y = __builtin_nontemporal_load(a.KK + c + 1);
     double2 o;
     o.x = a.alpha_1 * v.x + a.alpha_2 * w.x + bdt * K.x;
     o.y = a.alpha_1 * v.y + a.alpha_2 * w.y + bdt * K.y;
@@ -75,15 +80,18 @@ __global__ void __launch_bounds__(256) state_kernel_v(StateArgs a) {
   }
 }
 
+// blockIdx.y = row j + ny * component k (one row of one component per
+// block and iteration: more, shorter blocks stream faster than five
+// components per block).
 __global__ void __launch_bounds__(256) copy_interior_kernel_v(double* ww, const double* w, Geom G) {
   const int ip = blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= G.nx / 2) return;
   const long ik = (long)G.nx * G.ny;
-  for (int j = blockIdx.y; j < G.ny; j += gridDim.y) {
-    const long g = (long)(2 * ip + G.g0) + (long)(j + G.g1) * G.sy;
-    const long c = (long)(2 * ip) + (long)j * G.nx;
-#pragma unroll
-    for (int k = 0; k < 5; k++) *(double2*)(ww + c + k * ik) = *(const double2*)(w + g + k * G.sk);
+  for (int row = blockIdx.y; row < G.ny * 5; row += gridDim.y) {
+    const int j = row % G.ny, k = row / G.ny;
+    const long g = (long)(2 * ip + G.g0) + (long)(j + G.g1) * G.sy + k * G.sk;
+    const long c = (long)(2 * ip) + (long)j * G.nx + k * ik;
+    *(double2*)(ww + c) = *(const double2*)(w + g);
   }
 }
 
@@ -247,7 +255,7 @@ hipError_t launch_physical_bc(const BcArgs& a, hipStream_t s) {
 hipError_t launch_copy_interior(double* ww, const double* w, const Geom& g, hipStream_t s) {
   if (g.nx <= 0 || g.ny <= 0) return hipSuccess;
   if (vec_ok(g) && aligned16(ww) && aligned16(w)) {
-    dim3 grid((g.nx / 2 + 255) / 256, rows_grid(g.ny));
+    dim3 grid((g.nx / 2 + 255) / 256, rows_grid(g.ny * 5));
     hipLaunchKernelGGL(copy_interior_kernel_v, grid, dim3(256), 0, s, ww, w, g);
     return hipGetLastError();
   }
