@@ -74,10 +74,9 @@ batch_split_kernel(const int4* __restrict__ tiles, int ntiles, const RhsArgs* __
   split_body<V, P, true, BX, BY>(args[tl.x], tl.y, tl.z, (unsigned)tl.x, L);
 }
 
-// Every cell of the tile WENO (levels above 0, rhs_weno_* variants).  Three
-// waves per SIMD, as many as the tile's LDS allows (at most 168 VGPRs).
+// Every cell of the tile WENO (levels above 0, rhs_weno_* variants).
 template <int V, bool P, int BX, int BY>
-__global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(3))) batch_dense_kernel(const int4* __restrict__ tiles, int ntiles,
+__global__ void __launch_bounds__(BX* BY) batch_dense_kernel(const int4* __restrict__ tiles, int ntiles,
                                                               const RhsArgs* __restrict__ args) {
   ENL_BATCH_LDS(BX, BY, L, true)
   const int4 tl = tiles[xcd_tile((int)blockIdx.x, ntiles)];

@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(256) rhs_weno_list_kernel(RhsArgs a) {
 }
 
 template <int V, bool P, bool FUSED>
-__global__ void __launch_bounds__(Sh::threads) __attribute__((amdgpu_waves_per_eu(3))) rhs_dense_kernel(RhsArgs a) {
+__global__ void __launch_bounds__(Sh::threads) rhs_dense_kernel(RhsArgs a) {
   ENL_TILE_LDS(L, true)
   ENL_TILE_ORIGIN(a, i0, j0)
   dense_body<V, P, FUSED, kBX, kBY>(a, i0, j0, L);

@@ -366,11 +366,10 @@ __device__ inline void llf_alpha(const double* u, const double* v, int j, const 
   }
 }
 
-// WENO branch of one cell out of an LDS tile that holds the raw w_k.  hk
-// holds the physical source H on entry and the cell's rhs on return (the
-// source is added last, as in the reference's sum).
+// WENO branch of one cell out of an LDS tile that holds the raw w_k.
 template <int V, int S, bool P>
-__device__ inline void cell_weno_tile(const Tile& t, int e, int j, const RhsArgs& a, double hk[5]) {
+__device__ inline void cell_weno_tile(const Tile& t, int e, int j, const RhsArgs& a, const double H[5],
+                                      double out[5]) {
 #pragma clang fp contract(fast)
   using Sc = Scheme<V>;
   constexpr int R = Sc::order == 5 ? 3 : 2;
@@ -387,40 +386,19 @@ __device__ inline void cell_weno_tile(const Tile& t, int e, int j, const RhsArgs
     af = a.alpha;
     ag = a.alpha;
   }
-  // one component at a time (a rolled loop: k is the same for every lane,
-  // so the flux switch and the selects below are scalar), the x direction
-  // finished before the y direction's loads: only one direction's stencil
-  // and split fluxes are live at once
-#pragma unroll 1
+#pragma unroll
   for (int k = 0; k < 5; k++) {
-    const double* wk = k == 0 ? t.w[0] : k == 1 ? t.w[1] : k == 2 ? t.w[2] : k == 3 ? t.w[3] : t.w[4];
-    double x, y;
-    {
-      double F[7], W[7];
+    double F[7], Wx[7], G[7], Wy[7];
 #pragma unroll
-      for (int m = -R; m <= R; m++) {
-        F[m + 3] = tflux_f(t, e + m, k);
-        W[m + 3] = wk[e + m];
-      }
-      x = weno_dir<Sc::order>(F, W, af);
+    for (int m = -R; m <= R; m++) {
+      F[m + 3] = tflux_f(t, e + m, k);
+      Wx[m + 3] = t.w[k][e + m];
+      G[m + 3] = tflux_g(t, e + m * S, k);
+      Wy[m + 3] = t.w[k][e + m * S];
     }
-    __builtin_amdgcn_sched_barrier(0);
-    {
-      double G[7], W[7];
-#pragma unroll
-      for (int m = -R; m <= R; m++) {
-        G[m + 3] = tflux_g(t, e + m * S, k);
-        W[m + 3] = wk[e + m * S];
-      }
-      y = weno_dir<Sc::order>(G, W, ag);
-    }
-    const double h = k == 0 ? hk[0] : k == 1 ? hk[1] : k == 2 ? hk[2] : k == 3 ? hk[3] : hk[4];
-    const double o = divc<P>(x, a.dx, a.inv_dx) + divc<P>(y, a.dy, a.inv_dy) + h;
-    hk[0] = k == 0 ? o : hk[0];
-    hk[1] = k == 1 ? o : hk[1];
-    hk[2] = k == 2 ? o : hk[2];
-    hk[3] = k == 3 ? o : hk[3];
-    hk[4] = k == 4 ? o : hk[4];
+    double x = weno_dir<Sc::order>(F, Wx, af);
+    double y = weno_dir<Sc::order>(G, Wy, ag);
+    out[k] = divc<P>(x, a.dx, a.inv_dx) + divc<P>(y, a.dy, a.inv_dy) + H[k];
   }
 }
 
@@ -800,10 +778,10 @@ __device__ inline void dense_body(const RhsArgs& a, int i0, int j0, TileLds<BX, 
   t.s = L.s;
   t.tn = L.tn;
   const int e = (threadIdx.y + Sh::halo) * Sh::TW + threadIdx.x + Sh::halo;
-  double K[5];
-  cell_source<Sh::TW, P>(t, e, a.H0[(long)i + (long)j * a.h0_sy], a, K);
-  cell_weno_tile<V, Sh::TW, P>(t, e, j, a, K);
-  cell_output<FUSED>(a, i, j, K);
+  double H[5], out[5];
+  cell_source<Sh::TW, P>(t, e, a.H0[(long)i + (long)j * a.h0_sy], a, H);
+  cell_weno_tile<V, Sh::TW, P>(t, e, j, a, H, out);
+  cell_output<FUSED>(a, i, j, out);
   if (a.switch_map) a.switch_map[(long)i + (long)j * G.nx] = 1;
 }
 
