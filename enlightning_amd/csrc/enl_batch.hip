@@ -74,13 +74,15 @@ batch_split_kernel(const int4* __restrict__ tiles, int ntiles, const RhsArgs* __
   split_body<V, P, true, BX, BY>(args[tl.x], tl.y, tl.z, (unsigned)tl.x, L);
 }
 
-// Every cell of the tile WENO (levels above 0, rhs_weno_* variants).
+// Every cell of the tile WENO (levels above 0, rhs_weno_* variants): the
+// two directions on two thread planes (dense2_body), block (BX, BY, 2).
 template <int V, bool P, int BX, int BY>
-__global__ void __launch_bounds__(BX* BY) batch_dense_kernel(const int4* __restrict__ tiles, int ntiles,
-                                                              const RhsArgs* __restrict__ args) {
+__global__ void __launch_bounds__(2 * BX * BY) __attribute__((amdgpu_waves_per_eu(4))) batch_dense_kernel(const int4* __restrict__ tiles, int ntiles,
+                                                                  const RhsArgs* __restrict__ args) {
   ENL_BATCH_LDS(BX, BY, L, true)
+  __shared__ double lds_x[10 * BX * BY];
   const int4 tl = tiles[xcd_tile((int)blockIdx.x, ntiles)];
-  dense_body<V, P, true, BX, BY>(args[tl.x], tl.y, tl.z, L);
+  dense2_body<V, P, true, BX, BY>(args[tl.x], tl.y, tl.z, L, lds_x);
 }
 
 // The listed WENO cells of the level, ten lanes each (list_block).
@@ -410,7 +412,7 @@ void stage_launch(Batch* b, const RhsArgs* args, int st, int lev, hipStream_t s)
   const int4* tiles = b->tiles + t0;
   if (!Sc::drp || (Sc::hybrid && lev != 0)) {
     // rhs_weno_*, and hybrid variants above level 0 (the sensor is off)
-    hipLaunchKernelGGL((batch_dense_kernel<V, P, BX, BY>), dim3(nt), dim3(BX, BY), 0, s, tiles, nt, args);
+    hipLaunchKernelGGL((batch_dense_kernel<V, P, BX, BY>), dim3(nt), dim3(BX, BY, 2), 0, s, tiles, nt, args);
     return;
   }
   hipLaunchKernelGGL((batch_split_kernel<V, P, BX, BY>), dim3(nt), dim3(BX, BY), 0, s, tiles, nt, args);

@@ -89,11 +89,14 @@ __global__ void __launch_bounds__(256) rhs_weno_list_kernel(RhsArgs a) {
   list_block<V, P, FUSED>([&](int) -> const RhsArgs& { return a; }, a.list, *a.count);
 }
 
+// Every cell WENO: the two directions on two thread planes (dense2_body),
+// block (kBX, kBY, 2).
 template <int V, bool P, bool FUSED>
-__global__ void __launch_bounds__(Sh::threads) rhs_dense_kernel(RhsArgs a) {
+__global__ void __launch_bounds__(2 * Sh::threads) __attribute__((amdgpu_waves_per_eu(4))) rhs_dense_kernel(RhsArgs a) {
   ENL_TILE_LDS(L, true)
+  __shared__ double lds_x[10 * Sh::threads];
   ENL_TILE_ORIGIN(a, i0, j0)
-  dense_body<V, P, FUSED, kBX, kBY>(a, i0, j0, L);
+  dense2_body<V, P, FUSED, kBX, kBY>(a, i0, j0, L, lds_x);
 }
 
 __global__ void reset_count_kernel(unsigned* count) { *count = 0; }
@@ -112,7 +115,7 @@ hipError_t launch_t(const RhsArgs& a, hipStream_t s) {
   dim3 grid((unsigned)(((G.nx + kBX - 1) / kBX) * ((G.ny + kBY - 1) / kBY)));
   if constexpr (!Sc::drp) {
     // rhs_weno_*: every cell on the WENO path
-    hipLaunchKernelGGL((rhs_dense_kernel<V, P, F>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((rhs_dense_kernel<V, P, F>), grid, dim3(kBX, kBY, 2), 0, s, a);
   } else if constexpr (!Sc::hybrid) {
     // rhs_drp: every cell on the DRP path
     if constexpr (F)
@@ -121,7 +124,7 @@ hipError_t launch_t(const RhsArgs& a, hipStream_t s) {
       hipLaunchKernelGGL((rhs_split_kernel<V, P, F>), grid, block, 0, s, a);
   } else if (a.level != 0) {
     // hybrid above level 0: the sensor is off, every cell WENO
-    hipLaunchKernelGGL((rhs_dense_kernel<V, P, F>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((rhs_dense_kernel<V, P, F>), grid, dim3(kBX, kBY, 2), 0, s, a);
   } else {
     if (!a.list || !a.count) return hipErrorInvalidValue;
     hipLaunchKernelGGL(reset_count_kernel, dim3(1), dim3(1), 0, s, a.count);

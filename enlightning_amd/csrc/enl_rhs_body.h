@@ -785,4 +785,135 @@ __device__ inline void dense_body(const RhsArgs& a, int i0, int j0, TileLds<BX, 
   if (a.switch_map) a.switch_map[(long)i + (long)j * G.nx] = 1;
 }
 
+// Dense tile on two thread planes: threadIdx.z = 0 forms the source H and
+// the x direction of its cell's five components, threadIdx.z = 1 the y
+// direction.  The x plane's results meet the y plane's in LDS (xs: X then H,
+// 10 x BX*BY doubles) and the y plane forms each component as (X + Y) + H,
+// the reference's order, and writes the cell.  One direction per thread
+// needs about half the registers of both, so more waves fit a SIMD.
+template <int V, bool P, bool FUSED, int BX, int BY>
+__device__ inline void dense2_body(const RhsArgs& a, int i0, int j0, TileLds<BX, BY>& L, double* xs) {
+#pragma clang fp contract(fast)
+  using Sh = TileShape<BX, BY>;
+  using Sc = Scheme<V>;
+  const Geom& G = a.geom;
+  constexpr int NT = 2 * Sh::threads;
+  constexpr int S = Sh::TW;
+  constexpr int R = Sc::order == 5 ? 3 : 2;
+  const int tid = threadIdx.x + BX * (threadIdx.y + BY * threadIdx.z);
+  {
+    // staging by both planes; every load of a thread before its first store
+    constexpr int kIt = (Sh::TN + NT - 1) / NT;
+    const bool inside = i0 - Sh::halo >= -G.g0 && i0 + Sh::TW - Sh::halo <= G.nx + G.g0 &&
+                        j0 - Sh::halo >= -G.g1 && j0 + Sh::TH - Sh::halo <= G.ny + G.g1;
+    double v[kIt][7];
+    bool in[kIt];
+#pragma unroll
+    for (int it = 0; it < kIt; it++) {
+      const int e = tid + it * NT;
+      const int gi = i0 + (e % Sh::TW) - Sh::halo, gj = j0 + (e / Sh::TW) - Sh::halo;
+      in[it] = e < Sh::TN && (inside || (gi >= -G.g0 && gi < G.nx + G.g0 && gj >= -G.g1 && gj < G.ny + G.g1));
+      const long gidx = in[it] ? (long)(gi + G.g0) + (long)(gj + G.g1) * G.sy : 0;
+#pragma unroll
+      for (int k = 0; k < 5; k++) v[it][k] = a.w[gidx + k * G.sk];
+      v[it][5] = a.p[gidx];
+      v[it][6] = a.T[gidx];
+    }
+#pragma unroll
+    for (int it = 0; it < kIt; it++) {
+      if (!in[it]) continue;
+      const int e = tid + it * NT;
+      const double w0 = v[it][0];
+      L.w[0][e] = w0;
+      L.p[e] = v[it][5];
+      L.T[e] = v[it][6];
+      const double iw0 = den_ok(w0) ? rcp_nr(w0) : 1.0 / w0;
+      L.u[e] = v[it][1] * iw0;
+      L.v[e] = v[it][2] * iw0;
+      L.s[e] = v[it][3] * iw0;
+      L.tn[e] = v[it][4] * iw0;
+#pragma unroll
+      for (int k = 1; k < 5; k++) L.w[k][e] = v[it][k];
+    }
+  }
+  __syncthreads();
+  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+  const bool live = i < G.nx && j < G.ny;
+  const int c = threadIdx.x + BX * threadIdx.y;
+  const int e = (threadIdx.y + Sh::halo) * S + threadIdx.x + Sh::halo;
+  Tile t;
+  t.w0 = L.w[0];
+  for (int k = 0; k < 5; k++) t.w[k] = L.w[k];
+  t.p = L.p;
+  t.T = L.T;
+  t.u = L.u;
+  t.v = L.v;
+  t.s = L.s;
+  t.tn = L.tn;
+  const bool yplane = threadIdx.z != 0;
+  double Y[5];
+  if (live && !yplane) {
+    {
+      double H[5];
+      cell_source<S, P>(t, e, a.H0[(long)i + (long)j * a.h0_sy], a, H);
+#pragma unroll
+      for (int k = 0; k < 5; k++) xs[(5 + k) * Sh::threads + c] = H[k];
+    }
+    // x: LLF speed over the x window at this row's height
+    double af;
+    if constexpr (Sc::llf) {
+      af = 0;
+      const double height = a.ylo + a.dy * ((double)(j) + 0.5);
+#pragma unroll
+      for (int q = -R; q <= R; q++) {
+        const double vf = fabs(t.u[e + q]) + a.c + a.c_lr * height;
+        if (vf > af) af = vf;
+      }
+    } else {
+      af = a.alpha;
+    }
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      double F[7], W[7];
+#pragma unroll
+      for (int m = -R; m <= R; m++) {
+        F[m + 3] = tflux_f(t, e + m, k);
+        W[m + 3] = t.w[k][e + m];
+      }
+      xs[k * Sh::threads + c] = divc<P>(weno_dir<Sc::order>(F, W, af), a.dx, a.inv_dx);
+    }
+  } else if (live) {
+    // y: LLF speed over the y window at each point's height
+    double ag;
+    if constexpr (Sc::llf) {
+      ag = 0;
+#pragma unroll
+      for (int q = -R; q <= R; q++) {
+        const double height = a.ylo + a.dy * ((double)(j + q) + 0.5);
+        const double vg = fabs(t.v[e + q * S]) + a.c + a.c_lr * height;
+        if (vg > ag) ag = vg;
+      }
+    } else {
+      ag = a.alpha;
+    }
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      double Gy[7], W[7];
+#pragma unroll
+      for (int m = -R; m <= R; m++) {
+        Gy[m + 3] = tflux_g(t, e + m * S, k);
+        W[m + 3] = t.w[k][e + m * S];
+      }
+      Y[k] = divc<P>(weno_dir<Sc::order>(Gy, W, ag), a.dy, a.inv_dy);
+    }
+  }
+  __syncthreads();
+  if (!live || !yplane) return;
+  double K[5];
+#pragma unroll
+  for (int k = 0; k < 5; k++) K[k] = (xs[k * Sh::threads + c] + Y[k]) + xs[(5 + k) * Sh::threads + c];
+  cell_output<FUSED>(a, i, j, K);
+  if (a.switch_map) a.switch_map[(long)i + (long)j * G.nx] = 1;
+}
+
 }  // namespace enl
