@@ -671,9 +671,16 @@ __device__ inline void list_cell(const RhsArgs& a, int cell) {
 }
 
 // One (direction, component) of a listed WENO cell: the quantity list_cell
-// forms as X[k] (dir 0) or Y[k] (dir 1), by the same operations.
+// forms as X[k] (dir 0) or Y[k] (dir 1), by the same operations.  The five
+// lanes of one (cell, direction) share the window: lane k loads points k and
+// k + 5 (rho and the normal velocity, one quotient each) and the others
+// reach them by wavefront shuffles; the LLF speed is each lane's maximum
+// over its own points, then the maximum over the five lanes by shuffles
+// (rhs_hybrid_high_llf.m4:270-284: the max over the window, order-free).
+// gbase is the wave lane of the group's component 0; the group's lanes are
+// active together.
 template <int V, bool P>
-__device__ inline double list_item(const RhsArgs& a, int cell, int dir, int k) {
+__device__ inline double list_item(const RhsArgs& a, int cell, int dir, int k, int gbase) {
 #pragma clang fp contract(fast)
   using Sc = Scheme<V>;
   constexpr int R = Sc::order == 5 ? 3 : 2;
@@ -682,80 +689,111 @@ __device__ inline double list_item(const RhsArgs& a, int cell, int dir, int k) {
   const long g = (long)(i + G.g0) + (long)(j + G.g1) * G.sy;
   const long step = dir == 0 ? 1 : G.sy;
   const long kn = dir == 0 ? G.sk : 2 * G.sk;
+  // this lane's points of the window: pi = k and pi = k + 5 (point m = pi - R)
+  const bool two = k + 5 <= 2 * R;
+  double wa, ua, wb = 0.0, ub = 0.0;
+  {
+    const long ga = g + (long)(k - R) * step;
+    wa = a.w[ga];
+    ua = a.w[ga + kn] / wa;
+    if (two) {
+      const long gb = g + (long)(k + 5 - R) * step;
+      wb = a.w[gb];
+      ub = a.w[gb + kn] / wb;
+    }
+  }
   double w0[7], un[7];
 #pragma unroll
-  for (int m = -R; m <= R; m++) {
-    long gm = g + m * step;
-    w0[m + 3] = a.w[gm];
-    un[m + 3] = a.w[gm + kn] / w0[m + 3];
+  for (int pi = 0; pi <= 2 * R; pi++) {
+    const int src = gbase + pi % 5;
+    w0[pi - R + 3] = __shfl(pi < 5 ? wa : wb, src);
+    un[pi - R + 3] = __shfl(pi < 5 ? ua : ub, src);
   }
   double alpha = a.alpha;
   if constexpr (Sc::llf) {
+    // the x window at row j's height, the y window at each point's height
+    auto speed = [&](int pi, double u) {
+      const double height = a.ylo + a.dy * ((double)(dir == 0 ? j : j + pi - R) + 0.5);
+      return fabs(u) + a.c + a.c_lr * height;
+    };
+    double mine = 0;
+    const double sa = speed(k, ua);
+    if (sa > mine) mine = sa;
+    if (two) {
+      const double sb = speed(k + 5, ub);
+      if (sb > mine) mine = sb;
+    }
     alpha = 0;
 #pragma unroll
-    for (int m = -R; m <= R; m++) {
-      double height = a.ylo + a.dy * ((double)(dir == 0 ? j : j + m) + 0.5);
-      double vv = fabs(un[m + 3]) + a.c + a.c_lr * height;
-      if (vv > alpha) alpha = vv;
+    for (int d = 0; d < 5; d++) {
+      const double o = __shfl(mine, gbase + d);
+      if (o > alpha) alpha = o;
     }
   }
   return list_dir<V, P, false>(a, g, j, dir, k, w0, un, alpha);
 }
 
 // The WENO list, ten lanes per cell (two directions x five components) so a
-// short list still spreads over the machine: lanes store their term in LDS,
-// then one lane per cell forms rhs = X + Y + H in list_cell's order and
-// writes the cell.  Entries are (patch << 32) | (i + j*nx); args_of(patch)
-// gives the patch's arguments.  A wave serves its lanes patch by patch
-// (consecutive entries come from one tile: usually a single pass) so the
-// arguments stay uniform.  Must be called by all 256 lanes of the block.
-constexpr int kListCellsPerBlock = 25;
+// short list still spreads over the machine.  A wave holds six cells (lanes
+// 0..59; a cell's ten lanes never straddle two waves, so they can shuffle):
+// lane (dir, k) forms its term, lane (0, k) takes the y term of lane (1, k)
+// by a shuffle, forms rhs_k = X[k] + Y[k] + H[k] (list_cell's order) and
+// writes component k the way cell_output does; in the fused form lane (0,
+// 0) takes the new rho*s from lane (0, 3) for p and T.  The waves of a
+// block work apart (no LDS, no barrier).  Entries are (patch << 32) | (i +
+// j*nx); args_of(patch) gives the patch's arguments.  A wave serves its
+// lanes patch by patch (consecutive entries come from one tile: usually a
+// single pass) so the arguments stay uniform.
+constexpr int kListCellsPerWave = 6;
+constexpr int kListCellsPerBlock = 4 * kListCellsPerWave;
 
 template <int V, bool P, bool FUSED, class ArgsOf>
 __device__ inline void list_block(ArgsOf args_of, const unsigned long long* list, unsigned n) {
-  __shared__ double term[kListCellsPerBlock][10];
   const int t = threadIdx.x;
+  const int lane = t & 63, slot = lane / 10, item = lane % 10, k = item % 5;
+  const int lc = (t >> 6) * kListCellsPerWave + slot;
+  const int gbase = lane - k;  // lane (dir, 0) of this lane's group
   for (unsigned c0 = blockIdx.x * kListCellsPerBlock; c0 < n; c0 += gridDim.x * kListCellsPerBlock) {
-    {
-      const int lc = t / 10, item = t % 10;
-      const unsigned q = c0 + lc;
-      bool todo = lc < kListCellsPerBlock && q < n;
-      const unsigned long long e = todo ? list[q] : 0ull;
-      const int pid = (int)(e >> 32);
-      for (;;) {
-        const unsigned long long m = __ballot(todo);
-        if (!m) break;
-        const int cur = __builtin_amdgcn_readfirstlane(__shfl(pid, __ffsll((long long)m) - 1));
-        if (todo && pid == cur) {
-          term[lc][item] = list_item<V, P>(args_of(cur), (int)(e & 0xffffffffu), item / 5, item % 5);
-          todo = false;
-        }
-      }
-    }
-    __syncthreads();
-    {
-      const unsigned q = c0 + t;
-      bool todo = t < kListCellsPerBlock && q < n;
-      const unsigned long long e = todo ? list[q] : 0ull;
-      const int pid = (int)(e >> 32);
-      for (;;) {
-        const unsigned long long m = __ballot(todo);
-        if (!m) break;
-        const int cur = __builtin_amdgcn_readfirstlane(__shfl(pid, __ffsll((long long)m) - 1));
-        if (todo && pid == cur) {
-          const RhsArgs& a = args_of(cur);
-          const int cell = (int)(e & 0xffffffffu);
+    const unsigned q = c0 + lc;
+    bool todo = slot < kListCellsPerWave && q < n;
+    const unsigned long long e = todo ? list[q] : 0ull;
+    const int pid = (int)(e >> 32);
+    for (;;) {
+      const unsigned long long m = __ballot(todo);
+      if (!m) break;
+      const int cur = __builtin_amdgcn_readfirstlane(__shfl(pid, __ffsll((long long)m) - 1));
+      if (todo && pid == cur) {
+        const RhsArgs& a = args_of(cur);
+        const int cell = (int)(e & 0xffffffffu);
+        const double term = list_item<V, P>(a, cell, item / 5, k, gbase);
+        const double y = __shfl(term, item < 5 ? lane + 5 : lane);
+        if (item < 5) {
           const Geom& G = a.geom;
           const long ik = (long)G.nx * G.ny;
-          double K[5];
-#pragma unroll
-          for (int k = 0; k < 5; k++) K[k] = term[t][k] + term[t][5 + k] + a.KK[(long)cell + k * ik];
-          cell_output<FUSED>(a, cell % G.nx, cell / G.nx, K);
-          todo = false;
+          const double K = term + y + a.KK[(long)cell + k * ik];
+          const int i = cell % G.nx, j = cell / G.nx;
+          if constexpr (!FUSED) {
+            a.KK[(long)cell + k * ik] = K;
+          } else {
+            const long g = (long)(i + G.g0) + (long)(j + G.g1) * G.sy;
+            const double wa = a.ww[a.ww_off + i + j * a.ww_sy + k * a.ww_sk];
+            const double wb = a.w[g + k * G.sk];
+            const double wn = a.alpha_1 * wa + a.alpha_2 * wb + a.bdt * K;
+            a.w_out[g + k * G.sk] = wn;
+            const double wn3 = __shfl(wn, gbase + 3);
+            if (k == 0) {
+              const double xc1 = a.ylo + a.dy * ((double)(j) + 0.5);
+              double p, T;
+              eos_p_T_nr(a.c, a.c_lr, a.rho0, a.gamma1, a.c_p, a.p0, a.T0, a.c_v, a.R, a.eos_k1, a.eos_ok, xc1, wn,
+                         wn3, p, T);
+              a.p_out[g] = p;
+              a.T_out[g] = T;
+            }
+          }
         }
+        todo = false;
       }
     }
-    __syncthreads();
   }
 }
 
