@@ -6,6 +6,8 @@ with an ImportError-like RuntimeError when the shared object is missing.
 import ctypes
 import os
 
+import numpy as _np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libenlightning_hip.so")
 
@@ -44,6 +46,14 @@ class EnlPatchDesc(ctypes.Structure):
     _fields_ = [("level", ctypes.c_int), ("box", ctypes.c_int * 4), ("xlo", ctypes.c_double * 2),
                 ("dx", ctypes.c_double * 2)]
 
+
+# enl_patch_desc_t as a numpy record (the ctypes layout above: level and box,
+# then the doubles at offset 24)
+PATCH_DESC_DTYPE = _np.dtype({"names": ["level", "box", "xlo", "dx"],
+                              "formats": [_np.int32, (_np.int32, 4), (_np.float64, 2), (_np.float64, 2)],
+                              "offsets": [EnlPatchDesc.level.offset, EnlPatchDesc.box.offset,
+                                          EnlPatchDesc.xlo.offset, EnlPatchDesc.dx.offset],
+                              "itemsize": ctypes.sizeof(EnlPatchDesc)})
 
 PROTOTYPES = {
     "rhs_": (_RHS_ARGS, None),
@@ -98,6 +108,11 @@ PROTOTYPES = {
     "enl_plan_build": ([_I, _I, _I, _I, _P, _I, _I, _I, _P, _P], _I),
     "enl_plan_from_boxes": ([_I, _I, _I, _I, _P, _P], _I),
     "enl_plan_transfer": ([_P, _P, _I, _P], _I),
+    "enl_plan_transfer_rects": ([_P, _P, _I, _P], _I),
+    "enl_batch_set_fill_rects": ([_P, _I, _I, _I, _P], _I),
+    "enl_batch_set_coarsen_rects": ([_P, _I, _P], _I),
+    "enl_batch_transfer_rects": ([_P, _P, _P, _I, _P, _P, _I, _P], _I),
+    "enl_batch_tag_map": ([_P, _I, _P, _I, _P, _P, _I], _I),
     "enl_plan_get": ([_P, ctypes.c_char_p, _I, _P, _P], _I),
     "enl_plan_free": ([_P], _I),
     "enl_batch_fp32_store": ([_P], _I),
@@ -115,6 +130,7 @@ PROTOTYPES = {
     "enl_read_lightning": ([ctypes.c_char_p, _P, _I], _I),
     "enl_sample_mics": ([_I] * 4 + [_P, _P, _P, _I, _P, _P, _P, ctypes.c_long, _P, _P], _I),
     "enl_batch_destroy": ([_P], _I),
+    "enl_batch_trim": ([], _I),
     "enl_advance_patch": ([_I] * 6 + [_P, _P, ctypes.POINTER(EnlGas), _P, _I, _P, _P, _P, _D, _I, _I]
                           + [_P] * 6 + [_D], _I),
 }

@@ -245,6 +245,15 @@ class NativePlan:
     def coarsen(self):
         return tuple(self.get("coarsen." + k) for k in ("cp", "cl", "fp", "fl"))
 
+    def fill_rects(self, level):
+        """(rects (n, 8) int32 enl_rect_t rows, n1, n2): the level's ghost-fill
+        schedule as rectangles, n1 first-pass rows then n2 corner rows."""
+        n1, n2 = (int(v) for v in self.get("fill.rects_n", level))
+        return self.get("fill.rects", level).reshape(-1, 8), n1, n2
+
+    def coarsen_rects(self):
+        return self.get("coarsen.rects").reshape(-1, 8)
+
     def __del__(self):
         try:
             if self.handle:
@@ -258,12 +267,62 @@ def _boxes_array(h):
     return np.ascontiguousarray([[p.level] + list(p.box) for p in h.patches], dtype=np.int32).reshape(-1, 5)
 
 
+class _LazyFill:
+    """The per-level fill lists of a native plan, built (by the plan) on
+    first access; a level may be lent by another hierarchy's lists."""
+
+    def __init__(self, plan, nlevels):
+        self._plan, self._n, self._have, self._lent = plan, nlevels, {}, {}
+
+    def lend(self, level, other):
+        self._lent[level] = other
+
+    def __len__(self):
+        return self._n
+
+    def __getitem__(self, level):
+        if level < 0:
+            level += self._n
+        if not 0 <= level < self._n:
+            raise IndexError(level)
+        if level in self._lent:
+            return self._lent[level][level]
+        if level not in self._have:
+            self._have[level] = self._plan.fill(level)
+        return self._have[level]
+
+    def __iter__(self):
+        return (self[lev] for lev in range(self._n))
+
+
+class _LazyCoarsen:
+    """The per-cell coarsen lists (cp, cl, fp, fl) of a native plan, built on
+    first access."""
+
+    def __init__(self, plan):
+        self._plan, self._lists = plan, None
+
+    def _get(self):
+        if self._lists is None:
+            self._lists = self._plan.coarsen()
+        return self._lists
+
+    def __len__(self):
+        return 4
+
+    def __getitem__(self, k):
+        return self._get()[k]
+
+    def __iter__(self):
+        return iter(self._get())
+
+
 def _from_plan(plan, n0, ratio, g, L):
     h = Hierarchy(n0=n0, ratio=ratio, g=g, L=L)
-    for row in plan.get("boxes").reshape(-1, 5):
-        h.patches.append(Patch(len(h.patches), int(row[0]), tuple(int(v) for v in row[1:]), g))
-    h.fill = [plan.fill(lev) for lev in range(h.nlevels)]
-    h.coarsen = plan.coarsen() if h.nlevels > 1 else None
+    rows = plan.get("boxes").reshape(-1, 5).tolist()
+    h.patches = [Patch(k, row[0], tuple(row[1:]), g) for k, row in enumerate(rows)]
+    h.fill = _LazyFill(plan, h.nlevels)
+    h.coarsen = _LazyCoarsen(plan) if h.nlevels > 1 else None
     h.plan = plan
     return h
 
@@ -287,7 +346,10 @@ def build(n0, ratio, g, L, coarse_tags, tile0=64, tag_buffer=2, fine_max=64, fin
         if (previous is not None and (previous.n0, previous.ratio, previous.g) == (n0, ratio, g)
                 and [p.box for p in previous.level_patches(0)] == [p.box for p in h.level_patches(0)]
                 and [p.pid for p in previous.level_patches(0)] == [p.pid for p in h.level_patches(0)]):
-            h.fill[0] = previous.fill[0]  # the same lists: lent, as the numpy builder lends them
+            if isinstance(h.fill, _LazyFill):
+                h.fill.lend(0, previous.fill)  # the same lists: lent, as the numpy builder lends them
+            else:
+                h.fill[0] = previous.fill[0]
             if 0 in previous.owners:
                 h.owners[0] = previous.owners[0]
         return h
@@ -488,6 +550,26 @@ def make_coarsen(h):
     g = h.g
     return (cp.astype(np.int32), _locals(B0[cp], B2[cp], NX[cp], g, IC, JC), fp.astype(np.int32),
             _locals(B0[fp], B2[fp], NX[fp], g, IC * r, JC * r))
+
+
+def transfer_rects(new, old):
+    """The regrid transfer from hierarchy `old` to `new` as the library's
+    planner states it in rectangles (enl_plan_transfer_rects): the "same"
+    pairs (new pid, old pid) and an (n, 8) int32 array of enl_rect_t rows
+    (copies from old patches of the same level, refinements from the old
+    coarser level) for enl_batch_transfer_rects.  Both need native plans."""
+    if (new.n0, new.ratio, new.g) != (old.n0, old.ratio, old.g):
+        raise ValueError("transfer_rects: the hierarchies differ in base grid, ratio or ghost width")
+    if new.plan is None:
+        raise ValueError("transfer_rects: the new hierarchy has no native plan")
+    from . import _lib
+    ob = _boxes_array(old)
+    rc = new.plan.lib.enl_plan_transfer_rects(new.plan.handle, old.plan.handle if old.plan is not None else None,
+                                              len(ob), ctypes.c_void_p(ob.ctypes.data))
+    if rc:
+        raise ValueError("transfer_rects: %s" % _lib.last_error()[1])
+    t = lambda n: new.plan.get("transfer." + n, copy=True)  # noqa: E731
+    return {"same": (t("same.new"), t("same.old")), "rects": t("rects").reshape(-1, 8)}
 
 
 def make_transfer(new, old, native=True):

@@ -16,6 +16,7 @@ constexpr int kMaxLevels = 8;
 // One patch in device memory.  Offsets are in doubles from the pool base.
 struct PatchDev {
   int nx, ny, g, level;
+  int bi0, bj0;     // the box's lower corner in the level's index space
   long sy, sk;      // ghosted row / component strides
   long state[3];    // three ghosted state blocks [w_n(5), pres, temp], 7*sk each
   long src;         // ghosted source
@@ -56,6 +57,10 @@ struct Batch {
   int level_tile_begin[9] = {0};  // tiles sorted by level
   bool all_pow2 = true;
   LevelFill fill[8];
+  void* fill_buf[8] = {nullptr};  // one allocation per level holding its lists (rectangle path)
+  size_t fill_cap[8] = {0};
+  void* co_buf = nullptr;         // the coarsen lists (rectangle path)
+  size_t co_cap = 0;
   int ncoarsen = 0;
   int *co_cp = nullptr, *co_cl = nullptr, *co_fp = nullptr, *co_fl = nullptr;
   unsigned long long* list = nullptr;
@@ -78,6 +83,13 @@ struct Batch {
   int ab_begin[4][9] = {{0}};
 };
 
+// Device memory of the batched hierarchies: blocks a destroyed batch frees
+// stay in a per-device cache and serve the next batch of a similar size (a
+// regrid replaces a ~600 MB pool every few steps).  batch_free waits for the
+// device first, as hipFree does.
+hipError_t batch_malloc(void** p, size_t bytes);
+void batch_free(void* p);
+
 inline unsigned grid_for(long n, int block = 256, long cap = 4096) {
   long b = (n + block - 1) / block;
   if (b < 1) b = 1;
@@ -88,7 +100,7 @@ template <typename T>
 inline int upload(Batch* b, T** dst, const T* src, long n) {
   *dst = nullptr;
   if (n <= 0) return 0;
-  hipError_t e = hipMalloc((void**)dst, sizeof(T) * (size_t)n);
+  hipError_t e = batch_malloc((void**)dst, sizeof(T) * (size_t)n);
   if (e != hipSuccess) return check_hip(e, "hipMalloc(batch table)");
   b->allocations.push_back(*dst);
   return check_hip(hipMemcpy(*dst, src, sizeof(T) * (size_t)n, hipMemcpyHostToDevice), "upload batch table");
@@ -103,6 +115,19 @@ inline int stage_in(int st, int order) { return st == 0 ? 0 : stage_out(st - 1, 
 // Absorbing envelopes of one level's patches in state block `block`, on the
 // sides whose reflect flag is 0, in the reference's side order.
 int absorb_level(Batch* b, int lev, int block, const int* reflect, const enl_gas_t* gas, hipStream_t s);
+
+// Linear refinement of the coarse cell at c (row stride sy) to the fine
+// centre (ax, ay) coarse widths from its centre, unlimited centred slopes.
+__device__ inline double lin_refine(const double* c, long sy, double ax, double ay) {
+  return c[0] + ax * (0.5 * (c[1] - c[-1])) + ay * (0.5 * (c[sy] - c[-sy]));
+}
+
+// Fills the ghosts of every level of state block `buf`, level by level.
+void fill_all(Batch* b, int buf, const int* reflect, hipStream_t s);
+
+// Frees device memory the batch owns (batch_free) and drops it from its
+// allocation list.
+void release(Batch* b, void* p);
 
 // Ghost fill of one level of FP64 state block `buf` (-1: the source) from the
 // level's schedule; the coarse-to-fine refine reads block `cbuf` of the

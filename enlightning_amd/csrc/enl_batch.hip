@@ -117,12 +117,6 @@ __device__ inline const double* coarse_base(const FillArgs& f, const PatchDev& p
   return f.pool + (f.cbuf >= 0 ? p.state[f.cbuf] : p.src);
 }
 
-// linear refinement of the coarse cell at c (row stride sy) to the fine
-// centre (ax, ay) coarse widths from its centre, unlimited centred slopes
-__device__ inline double lin_refine(const double* c, long sy, double ax, double ay) {
-  return c[0] + ax * (0.5 * (c[1] - c[-1])) + ay * (0.5 * (c[sy] - c[-sy]));
-}
-
 // same-level copy: every component of the cell
 __device__ inline void fill_copy(const FillArgs& f, int q, const int* dp, const int* dl, const int* sp, const int* sl) {
   const PatchDev& D = f.patches[dp[q]];
@@ -293,6 +287,17 @@ void fill_level(Batch* b, int level, int buf, int cbuf, int ncomp, const int* re
                        L.k_s, L.k_n);
 }
 
+void fill_all(Batch* b, int buf, const int* reflect, hipStream_t s) {
+  for (int lev = 0; lev < b->nlevels; lev++) fill_level(b, lev, buf, buf, 7, reflect, s);
+}
+
+void release(Batch* b, void* p) {
+  if (!p) return;
+  batch_free(p);
+  for (auto& a : b->allocations)
+    if (a == p) a = nullptr;
+}
+
 
 namespace {
 
@@ -309,7 +314,7 @@ int make_tiles(Batch* b) {
   }
   b->level_tile_begin[kMaxLevels] = (int)t.size();
   if (b->tiles) {
-    (void)hipFree(b->tiles);
+    batch_free(b->tiles);
     for (auto& a : b->allocations)
       if (a == b->tiles) a = nullptr;
   }
@@ -317,10 +322,6 @@ int make_tiles(Batch* b) {
   return upload(b, &b->tiles, t.data(), (long)t.size());
 }
 
-
-void fill_all(Batch* b, int buf, const int* reflect, hipStream_t s) {
-  for (int lev = 0; lev < b->nlevels; lev++) fill_level(b, lev, buf, buf, 7, reflect, s);
-}
 
 // Arguments common to every patch: gas constants and the WENO list.
 RhsArgs base_args(Batch* b, const enl_gas_t* gas) {
@@ -553,7 +554,7 @@ int batch_eval_rhs64(Batch* b, int variant, const enl_gas_t* gas, const int* ref
     patch_args(b, p, in, out, 0, kMaxStages - 1, t[p]);
   }
   RhsArgs* d = nullptr;
-  hipError_t e = hipMalloc((void**)&d, sizeof(RhsArgs) * t.size());
+  hipError_t e = batch_malloc((void**)&d, sizeof(RhsArgs) * t.size());
   if (e != hipSuccess) return check_hip(e, "hipMalloc(eval arguments)");
   int rc = check_hip(hipMemcpy(d, t.data(), sizeof(RhsArgs) * t.size(), hipMemcpyHostToDevice), "eval arguments");
   if (!rc) {
@@ -565,7 +566,7 @@ int batch_eval_rhs64(Batch* b, int variant, const enl_gas_t* gas, const int* ref
     }
   }
   if (!rc) rc = check_hip(hipStreamSynchronize(s), "batch_eval_rhs64");
-  (void)hipFree(d);
+  batch_free(d);
   return rc;
 }
 
@@ -656,6 +657,8 @@ int enl_batch_create(int npatch, const enl_patch_desc_t* patches, int gcw, int r
     }
     pd.g = gcw;
     pd.level = d.level;
+    pd.bi0 = d.box[0];
+    pd.bj0 = d.box[2];
     pd.sy = pd.nx + 2L * gcw;
     pd.sk = pd.sy * (pd.ny + 2L * gcw);
     for (int s = 0; s < 3; s++) {
@@ -682,7 +685,7 @@ int enl_batch_create(int npatch, const enl_patch_desc_t* patches, int gcw, int r
     if (d.level + 1 > b->nlevels) b->nlevels = d.level + 1;
   }
   b->pool_doubles = (size_t)off;
-  hipError_t e = hipMalloc((void**)&b->pool, sizeof(double) * b->pool_doubles);
+  hipError_t e = batch_malloc((void**)&b->pool, sizeof(double) * b->pool_doubles);
   if (e != hipSuccess) {
     delete b;
     return check_hip(e, "hipMalloc(batch pool)");
@@ -691,12 +694,12 @@ int enl_batch_create(int npatch, const enl_patch_desc_t* patches, int gcw, int r
   (void)hipMemset(b->pool, 0, sizeof(double) * b->pool_doubles);
   int rc = upload(b, &b->dp, b->hp.data(), npatch);
   if (!rc) {
-    e = hipMalloc((void**)&b->list, sizeof(unsigned long long) * (size_t)b->cells);
+    e = batch_malloc((void**)&b->list, sizeof(unsigned long long) * (size_t)b->cells);
     if (e == hipSuccess) b->allocations.push_back(b->list);
-    if (e == hipSuccess) e = hipMalloc((void**)&b->count, sizeof(unsigned) * kMaxStages * kMaxLevels);
+    if (e == hipSuccess) e = batch_malloc((void**)&b->count, sizeof(unsigned) * kMaxStages * kMaxLevels);
     if (e == hipSuccess) b->allocations.push_back(b->count);
     if (e == hipSuccess) e = hipMemset(b->count, 0, sizeof(unsigned) * kMaxStages * kMaxLevels);
-    if (e == hipSuccess) e = hipMalloc((void**)&b->d_args, sizeof(RhsArgs) * (size_t)kMaxStages * npatch);
+    if (e == hipSuccess) e = batch_malloc((void**)&b->d_args, sizeof(RhsArgs) * (size_t)kMaxStages * npatch);
     if (e == hipSuccess) b->allocations.push_back(b->d_args);
     if (e != hipSuccess) rc = check_hip(e, "hipMalloc(batch list)");
   }
@@ -707,7 +710,7 @@ int enl_batch_create(int npatch, const enl_patch_desc_t* patches, int gcw, int r
   if (!rc) rc = make_tiles(b);
   if (rc) {
     for (void* a : b->allocations)
-      if (a) (void)hipFree(a);
+      if (a) batch_free(a);
     delete b;
     return rc;
   }
@@ -880,20 +883,30 @@ int enl_batch_copy_state(void* handle, int pid, int to_device, double* w_n, doub
   return check_hip(hipStreamSynchronize(s), "enl_batch_copy_state sync");
 }
 
+namespace {
+// The interior of patch blockIdx.y's ghosted source into its interior copy.
+__global__ void __launch_bounds__(256) source_interior_kernel(double* pool, const PatchDev* __restrict__ dp) {
+  const PatchDev pd = dp[blockIdx.y];
+  const long n = (long)pd.nx * pd.ny;
+  for (long q = blockIdx.x * 256L + threadIdx.x; q < n; q += (long)gridDim.x * 256L) {
+    const long i = q % pd.nx, j = q / pd.nx;
+    pool[pd.src_int + q] = pool[pd.src + (i + pd.g) + (j + pd.g) * pd.sy];
+  }
+}
+}  // namespace
+
 int enl_batch_prepare(void* handle, const int* reflect) {
   // ghosts of the source (constant over the run) and its interior copy for
   // the plain mode's drop-in calls
   Batch* b = (Batch*)handle;
   if (!b) return set_error(-2, "null batch");
+  if (b->npatch > 65535) return set_error(-2, "enl_batch_prepare: more than 65535 patches");
   hipStream_t s = lib_stream();
   for (int lev = 0; lev < b->nlevels; lev++) fill_level(b, lev, -1, -1, 1, reflect, s);
-  for (int p = 0; p < b->npatch; p++) {
-    const PatchDev& pd = b->hp[p];
-    hipError_t e = hipMemcpy2DAsync(b->pool + pd.src_int, sizeof(double) * pd.nx,
-                                    b->pool + pd.src + pd.g + pd.g * pd.sy, sizeof(double) * pd.sy,
-                                    sizeof(double) * pd.nx, pd.ny, hipMemcpyDeviceToDevice, s);
-    if (e != hipSuccess) return check_hip(e, "source interior copy");
-  }
+  long most = 0;
+  for (const PatchDev& pd : b->hp) most = std::max(most, (long)pd.nx * pd.ny);
+  const unsigned bx = (unsigned)std::min<long>((most + 255) / 256, 64);
+  source_interior_kernel<<<dim3(bx, b->npatch), 256, 0, s>>>(b->pool, b->dp);
   return check_hip(hipGetLastError(), "enl_batch_prepare");
 }
 
@@ -1047,11 +1060,11 @@ int enl_batch_set_absorb(void* handle, const double* grid_xlo, const double* gri
   // it is re-captured on the next graph advance
   drop_graph(b);
   if (b->ab_rects) {
-    (void)hipFree(b->ab_rects);
+    batch_free(b->ab_rects);
     b->ab_rects = nullptr;
   }
   if (b->ab_tab) {
-    (void)hipFree(b->ab_tab);
+    batch_free(b->ab_tab);
     b->ab_tab = nullptr;
   }
   for (int s = 0; s < 4; s++)
@@ -1101,8 +1114,8 @@ int enl_batch_set_absorb(void* handle, const double* grid_xlo, const double* gri
   // levels are contiguous inside a side's block, so ab_begin[s][l + 1] ends
   // level l
   hipError_t e = hipSuccess;
-  if (!flat.empty()) e = hipMalloc((void**)&b->ab_rects, sizeof(AbsRect) * flat.size());
-  if (e == hipSuccess && !tab.empty()) e = hipMalloc((void**)&b->ab_tab, sizeof(double) * tab.size());
+  if (!flat.empty()) e = batch_malloc((void**)&b->ab_rects, sizeof(AbsRect) * flat.size());
+  if (e == hipSuccess && !tab.empty()) e = batch_malloc((void**)&b->ab_tab, sizeof(double) * tab.size());
   if (e == hipSuccess && !flat.empty())
     e = hipMemcpy(b->ab_rects, flat.data(), sizeof(AbsRect) * flat.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess && !tab.empty())
@@ -1155,7 +1168,7 @@ int enl_batch_transfer(void* new_handle, void* old_handle, const int* reflect, i
     const size_t ibytes = sizeof(int) * (size_t)(2 * nsame + 4 * ncopy + 4 * ninterp);
     const size_t bytes = ibytes + sizeof(double) * (size_t)(2 * ninterp);
     char* buf = nullptr;
-    hipError_t e = hipMalloc((void**)&buf, bytes);
+    hipError_t e = batch_malloc((void**)&buf, bytes);
     if (e != hipSuccess) return check_hip(e, "hipMalloc(transfer items)");
     std::vector<char> h(bytes);
     int* hi = (int*)h.data();
@@ -1202,7 +1215,7 @@ int enl_batch_transfer(void* new_handle, void* old_handle, const int* reflect, i
     }
     // the staging copy reads h, and the kernel reads buf: wait for both
     const hipError_t e2 = hipStreamSynchronize(s);
-    (void)hipFree(buf);
+    batch_free(buf);
     if (e != hipSuccess) return check_hip(e, "transfer launch");
     if (e2 != hipSuccess) return check_hip(e2, "transfer sync");
   }
@@ -1226,7 +1239,7 @@ int enl_batch_tag_cells(void* handle, int level, const int* reflect, int refine_
   const bool dev = is_device_ptr(tags);
   int* out = tags;
   if (!dev) {
-    hipError_t e = hipMalloc((void**)&out, sizeof(int) * (size_t)total);
+    hipError_t e = batch_malloc((void**)&out, sizeof(int) * (size_t)total);
     if (e != hipSuccess) return check_hip(e, "hipMalloc(tags)");
   }
   // the stencil reads the state's ghosts; the source's are filled by
@@ -1253,7 +1266,7 @@ int enl_batch_tag_cells(void* handle, int level, const int* reflect, int refine_
   }
   if (e == hipSuccess && !dev) e = hipMemcpyAsync(tags, out, sizeof(int) * (size_t)total, hipMemcpyDeviceToHost, s);
   const hipError_t e2 = dev ? hipSuccess : hipStreamSynchronize(s);
-  if (!dev) (void)hipFree(out);
+  if (!dev) batch_free(out);
   if (e != hipSuccess) return check_hip(e, "enl_batch_tag_cells");
   return check_hip(e2, "enl_batch_tag_cells sync");
 }
@@ -1265,10 +1278,10 @@ int enl_batch_destroy(void* handle) {
   if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   fp32_release(b);
-  if (b->ab_rects) (void)hipFree(b->ab_rects);
-  if (b->ab_tab) (void)hipFree(b->ab_tab);
+  if (b->ab_rects) batch_free(b->ab_rects);
+  if (b->ab_tab) batch_free(b->ab_tab);
   for (void* a : b->allocations)
-    if (a) (void)hipFree(a);
+    if (a) batch_free(a);
   delete b;
   return 0;
 }

@@ -823,7 +823,7 @@ Args32 base32(Batch* b, int p, const enl_gas_t* gas) {
 
 template <typename T>
 int alloc(Fp32Form* F, T** p, size_t n, const char* what) {
-  hipError_t e = hipMalloc((void**)p, sizeof(T) * (n ? n : 1));
+  hipError_t e = batch_malloc((void**)p, sizeof(T) * (n ? n : 1));
   if (e != hipSuccess) return check_hip(e, what);
   F->allocs.push_back(*p);
   return 0;
@@ -839,7 +839,7 @@ void drop_graph32(Fp32Form* F) {
 // Buffers, ambient records and patch table of the FP32 form.
 int setup32(Batch* b) {
   Fp32Form* F = b->f32;
-  for (void* a : F->allocs) (void)hipFree(a);
+  for (void* a : F->allocs) batch_free(a);
   F->allocs.clear();
   drop_graph32(F);
   F->ready = false;
@@ -1063,7 +1063,7 @@ int enqueue32(Batch* b, const int* reflect, int order, int nsteps, hipStream_t s
 void fp32_release(Batch* b) {
   if (!b->f32) return;
   drop_graph32(b->f32);
-  for (void* a : b->f32->allocs) (void)hipFree(a);
+  for (void* a : b->f32->allocs) batch_free(a);
   delete b->f32;
   b->f32 = nullptr;
 }

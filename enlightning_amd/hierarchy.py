@@ -151,6 +151,18 @@ def perturbation_errors(h, ref, test, ambients, gas):
     return err
 
 
+_MAPS = {}
+
+
+def _pinned_map(n):
+    """An (n, n) int32 array in pinned host memory (kept for reuse): the
+    device writes a tag map into it directly."""
+    import torch
+    if n not in _MAPS:
+        _MAPS[n] = torch.empty((n, n), dtype=torch.int32, pin_memory=True)
+    return _MAPS[n].numpy()
+
+
 class DeviceHierarchy:
     def __init__(self, h, gas, reflect=(1, 1, 1, 1)):
         self.h, self.gas, self.reflect = h, gas, tuple(reflect)
@@ -158,21 +170,29 @@ class DeviceHierarchy:
         self.tiles = {}  # level -> (bx, by) set by set_tiles / autotune; None: every level
         self.lib = _lib.load()
         n = len(h.patches)
-        desc = (_lib.EnlPatchDesc * n)()
-        for k, p in enumerate(h.patches):
-            d = desc[k]
-            d.level = p.level
-            for q in range(4):
-                d.box[q] = p.box[q]
-            dx = h.dx(p.level)
-            d.xlo[0] = p.box[0] * dx
-            d.xlo[1] = p.box[2] * dx
-            d.dx[0] = dx
-            d.dx[1] = dx
+        # the patch table (enl_patch_desc_t rows) built with numpy
+        desc = np.zeros(n, dtype=_lib.PATCH_DESC_DTYPE)
+        if n:
+            lev = np.array([p.level for p in h.patches], np.int32)
+            box = np.array([p.box for p in h.patches], np.int32).reshape(n, 4)
+            dx = h.L / (h.n0 * np.power(float(h.ratio), lev))
+            desc["level"], desc["box"] = lev, box
+            desc["xlo"] = np.stack([box[:, 0] * dx, box[:, 2] * dx], axis=1)
+            desc["dx"] = np.stack([dx, dx], axis=1)
         handle = ctypes.c_void_p()
-        _lib.check(self.lib.enl_batch_create(n, desc, h.g, h.ratio, ctypes.byref(handle)))
+        _lib.check(self.lib.enl_batch_create(n, desc.ctypes.data_as(ctypes.POINTER(_lib.EnlPatchDesc)), h.g, h.ratio,
+                                             ctypes.byref(handle)))
         self.handle = handle
         self._keep = []
+        if h.plan is not None:
+            # the planner's rectangles, expanded into per-cell lists on the device
+            for lev in range(h.nlevels):
+                rects, n1, n2 = h.plan.fill_rects(lev)
+                _lib.check(self.lib.enl_batch_set_fill_rects(self.handle, lev, n1, n2, _p(rects)))
+            if h.nlevels > 1:
+                rects = h.plan.coarsen_rects()
+                _lib.check(self.lib.enl_batch_set_coarsen_rects(self.handle, len(rects), _p(rects)))
+            return
         for lev, f in enumerate(h.fill):
             c, it, e, k = f["copy"], f["interp"], f["edge"], f["corner"]
             arrs = [np.ascontiguousarray(a) for a in c + it + e + k]
@@ -381,17 +401,11 @@ class DeviceHierarchy:
         patch's device data in one call (enl_batch_tag_cells;
         enlightning::tagGradientDetectorCells, enlightning.C:586-628), the
         input of the next regrid's clustering."""
-        pats = self.h.level_patches(0)
-        flat = np.zeros(sum(p.nx * p.ny for p in pats), np.int32)
-        tol = (ctypes.c_double * len(self.gas.tag_tolerance))(*self.gas.tag_tolerance)
-        _lib.check(self.lib.enl_batch_tag_cells(self.handle, 0, self._reflect(), int(refine_tag_val), tol, _p(flat)))
         n0 = self.h.ncells(0)
-        tags = np.zeros((n0, n0), np.int32)
-        off = 0
-        for p in pats:
-            tags[p.box[2]:p.box[3] + 1, p.box[0]:p.box[1] + 1] = flat[off:off + p.nx * p.ny].reshape(p.ny, p.nx)
-            off += p.nx * p.ny
-        return tags
+        tags = _pinned_map(n0)
+        tol = (ctypes.c_double * len(self.gas.tag_tolerance))(*self.gas.tag_tolerance)
+        _lib.check(self.lib.enl_batch_tag_map(self.handle, 0, self._reflect(), int(refine_tag_val), tol, _p(tags), n0))
+        return tags.copy()
 
     def regrid(self, new_h, transfer=None):
         """A DeviceHierarchy on new_h's boxes holding this hierarchy's state,
@@ -402,14 +416,23 @@ class DeviceHierarchy:
         keeps its interiors and stays usable; absorbing envelopes and tile
         shapes carry over.  The FP64 state moves: after FP32-form steps call
         fp32_store() first."""
-        tr = amr.make_transfer(new_h, self.h) if transfer is None else transfer
+        if transfer is None:
+            tr = amr.transfer_rects(new_h, self.h) if new_h.plan is not None else amr.make_transfer(new_h, self.h)
+        else:
+            tr = transfer
         new = DeviceHierarchy(new_h, self.gas, self.reflect)
         try:
-            sm, c, it = tr["same"], tr["copy"], tr["interp"]
-            arrs = [np.ascontiguousarray(a) for a in sm + c + it]
-            P = [_p(a) for a in arrs]
-            _lib.check(self.lib.enl_batch_transfer(new.handle, self.handle, self._reflect(), len(sm[0]), *P[0:2],
-                                                   len(c[0]), *P[2:6], len(it[0]), *P[6:12]))
+            if "rects" in tr:
+                sm, rects = tr["same"], np.ascontiguousarray(tr["rects"], dtype=np.int32)
+                a, b = np.ascontiguousarray(sm[0], np.int32), np.ascontiguousarray(sm[1], np.int32)
+                _lib.check(self.lib.enl_batch_transfer_rects(new.handle, self.handle, self._reflect(), len(a), _p(a),
+                                                             _p(b), len(rects), _p(rects)))
+            else:
+                sm, c, it = tr["same"], tr["copy"], tr["interp"]
+                arrs = [np.ascontiguousarray(a) for a in sm + c + it]
+                P = [_p(a) for a in arrs]
+                _lib.check(self.lib.enl_batch_transfer(new.handle, self.handle, self._reflect(), len(sm[0]),
+                                                       *P[0:2], len(c[0]), *P[2:6], len(it[0]), *P[6:12]))
             if self.absorb_width > 0:
                 new.set_absorb(self.absorb_width)
             # the tile shapes chosen for this hierarchy (results do not

@@ -333,7 +333,44 @@ int enl_batch_transfer(void *new_handle, void *old_handle, const int *reflect, i
  * memory; host memory is filled before the call returns). */
 int enl_batch_tag_cells(void *handle, int level, const int *reflect, int refine_tag_val,
                         const double *tag_tolerance, int *tags);
+/* The same tags written into a map of the level's index space: map[j*nmap +
+ * i] for every cell (i, j) of a patch of the level, 0 elsewhere (one launch
+ * over all patches; host or device memory).  The tag input of the next
+ * regrid's clustering (enl_plan_build). */
+int enl_batch_tag_map(void *handle, int level, const int *reflect, int refine_tag_val, const double *tag_tolerance,
+                      int *map, int nmap);
+
+/* The schedules above as rectangles of cells: ni x nj cells; cell (a, b)
+ * has destination local dl + a + b*Nx(dp) and source
+ *   kind 0 (copy):   sl + a + b*Nx(sp), patch sp;
+ *   kind 1 (refine): coarse cell sl + pa/r + (pb/r)*Nx(sp) of patch sp,
+ *                    pa = (phase & 255) + a, pb = (phase >> 8) + b, fine
+ *                    centre offsets fx = ((pa % r) + 0.5)/r - 0.5, fy alike;
+ *   kind 1+s (physical side s = 1..4, sp = dp): sl + b*Nx (left, right) or
+ *                    sl + a (bottom, top), the side's negation.
+ * A coarsen rectangle lists coarse cells: dl + a + b*Nx(dp) averages the
+ * r x r fine block whose lower-left cell is sl + a*r + b*r*Nx(sp). */
+typedef struct enl_rect {
+  int dp, sp, dl, sl, ni, nj, kind, phase;
+} enl_rect_t;
+/* A level's ghost-fill schedule from rectangles: n1 copy / refine / edge
+ * rectangles, then n2 corner rectangles (a second pass).  The device
+ * expands them into the lists enl_batch_set_fill would hold (same items,
+ * rectangle order).  Rectangles outside their patches' ghosted boxes,
+ * refinements whose +-1 stencil leaves the coarse box and level mismatches
+ * are rejected before anything runs. */
+int enl_batch_set_fill_rects(void *handle, int level, int n1, int n2, const enl_rect_t *rects);
+/* The fine-to-coarse schedule from rectangles of coarse cells. */
+int enl_batch_set_coarsen_rects(void *handle, int n, const enl_rect_t *rects);
+/* enl_batch_transfer with the copy and refine items as rectangles (dp, dl
+ * in the new batch; sp, sl in the old one). */
+int enl_batch_transfer_rects(void *new_handle, void *old_handle, const int *reflect, int nsame, const int *s_new,
+                             const int *s_old, int nrect, const enl_rect_t *rects);
 int enl_batch_destroy(void *handle);
+/* Device memory that destroyed batches gave back stays cached for the next
+ * batch of a similar size (a regrid replaces the hierarchy's pool every few
+ * steps); this returns the current device's cached blocks to the runtime. */
+int enl_batch_trim(void);
 
 /* ---- regrid planning on the host (gridding_algorithm->regridAllFinerLevels,
  * main.C:489-502; the refine and coarsen schedules enlightning.C:259-288
@@ -360,11 +397,16 @@ int enl_plan_from_boxes(int n0, int ratio, int g, int npatch, const int *boxes, 
  * patch of its level ("transfer.copy.dp/dl/sp/sl") or refined from the old
  * coarser level ("transfer.interp.dp/dl/sp/sl/fx/fy"). */
 int enl_plan_transfer(void *plan, void *old_plan, int n_old, const int *old_boxes);
-/* A list of the plan: "boxes", "fill.copy.p/dst/sp/sl", "fill.interp.p/dst/
+/* The same transfer as rectangles ("transfer.rects", enl_rect_t rows of at
+ * most 1024 cells) beside the "same" pairs, for enl_batch_transfer_rects. */
+int enl_plan_transfer_rects(void *plan, void *old_plan, int n_old, const int *old_boxes);
+/* A list of the plan: "boxes", "fill.rects" and "fill.rects_n" (n1, n2) of
+ * one level, "coarsen.rects", "fill.copy.p/dst/sp/sl", "fill.interp.p/dst/
  * cp/cl/fx/fy", "fill.edge.p/dst/src/neg", "fill.corner.p/dst/src/neg" (of
- * one level), "coarsen.cp/cl/fp/fl", the transfer lists.  *data points into
- * the plan (valid until enl_plan_free), *count is its length; returns 1 for
- * a list of doubles, 0 for ints (or an absent, empty list). */
+ * one level), "coarsen.cp/cl/fp/fl", the transfer lists.  The per-cell fill
+ * and coarsen lists are built on first request.  *data points into the plan
+ * (valid until enl_plan_free), *count is its length; returns 1 for a list
+ * of doubles, 0 for ints (or an absent, empty list). */
 int enl_plan_get(const void *plan, const char *name, int level, const void **data, long *count);
 int enl_plan_free(void *plan);
 

@@ -80,6 +80,73 @@ def test_transfer_matches_oracle_bit_for_bit():
     dev.close()
 
 
+def test_transfer_rects_match_oracle_bit_for_bit():
+    """The regrid's default path: the planner's rectangles
+    (amr.transfer_rects, enl_batch_transfer_rects) move the same values as
+    the oracle's per-cell transfer, bit for bit, into every ghosted array."""
+    gas, old, new = hierarchies()
+    states = [patch_initial_state(old, p, WL64, gas) for p in old.patches]
+    dev = D.Hierarchy(old, gas)
+    dev.load_initial(WL64, states=states, sources=random_sources(old, 6))
+    cpu = twin_of(dev, old, gas)
+    if not D.DRY:
+        rt = amr.transfer_rects(new, old)
+        print("transfer: %d rectangles, %d whole patches" % (len(rt["rects"]), len(rt["same"][0])), flush=True)
+    dnew = dev.regrid(new)
+    cnew = HostHierarchy(new, gas, dev.reflect)
+    cnew.transfer_from(cpu, amr.make_transfer(new, old, native=False))
+    for k, ((wd, pd, Td), (wc, pc, Tc), sd, sc) in enumerate(zip(dnew.states(), cnew.states(), dnew.sources(),
+                                                              cnew.src)):
+        assert np.array_equal(wd, wc), k
+        assert np.array_equal(pd, pc) and np.array_equal(Td, Tc), k
+        assert np.array_equal(sd, sc), k
+    dnew.close()
+    dev.close()
+
+
+def test_rectangle_schedules_equal_list_schedules():
+    """A hierarchy whose fill and coarsen lists the device expanded from the
+    planner's rectangles (a native plan) steps bit-identically to the same
+    hierarchy given the numpy builders' per-cell lists."""
+    gas = Gas()
+    tags = level_tags(gas, WL64, oracle.tag_cells)
+    hr = amr.build(WL64.n, 4, 4, 1.0, tags, tile0=16)
+    hl = amr.build(WL64.n, 4, 4, 1.0, tags, tile0=16, native=False)
+    assert hr.plan is not None and hl.plan is None
+    states = [patch_initial_state(hr, p, WL64, gas) for p in hr.patches]
+    srcs = random_sources(hr, 8)
+    out = []
+    for h in (hr, hl):
+        dev = D.Hierarchy(h, gas)
+        dev.load_initial(WL64, states=states, sources=srcs)
+        dev.advance(3, 0.5 * h.dx(1) / gas.c, mode="graph")
+        out.append(dev.states())
+        dev.close()
+    for k, (a, b) in enumerate(zip(*out)):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y), k
+
+
+def test_transfer_rects_reject_out_of_range():
+    gas, old, new = hierarchies()
+    if D.DRY:
+        pytest.skip("the range check is the C ABI's")
+    dev = D.Hierarchy(old, gas)
+    dev.load_initial(WL64)
+    rt = amr.transfer_rects(new, old)
+    refine = np.nonzero(rt["rects"][:, 6] == 1)[0]
+    assert len(refine)
+    bad = {"same": rt["same"], "rects": rt["rects"].copy()}
+    bad["rects"][refine[0], 3] = 0  # the refinement stencil would leave the coarse patch's ghosted box
+    with pytest.raises(RuntimeError, match="out of range"):
+        dev.regrid(new, bad)
+    bad = {"same": rt["same"], "rects": rt["rects"].copy()}
+    bad["rects"][0, 4] = 10 ** 6  # a rectangle wider than its patch
+    with pytest.raises(RuntimeError, match="out of range"):
+        dev.regrid(new, bad)
+    dev.close()
+
+
 def test_transfer_rejects_out_of_range_items():
     gas, old, new = hierarchies()
     dev = D.Hierarchy(old, gas)
