@@ -175,17 +175,20 @@ def amr_bench(args):
     rtags = d.level0_tags()
     t1 = time.perf_counter()
     h2 = amr.build(n, 4, 4, 1.0, rtags, tile0=64, previous=h)
-    tr = amr.make_transfer(h2, h)
+    tr = amr.transfer_rects(h2, h)
     t2 = time.perf_counter()
     d2 = d.regrid(h2, tr)
     torch.cuda.synchronize()
     t3 = time.perf_counter()
     d2.close()
+    rc = tr["rects"]
     regrid = {"tags_device_ms": (t1 - t0) * 1e3, "cluster_and_schedules_host_ms": (t2 - t1) * 1e3,
               "new_batch_and_transfer_device_ms": (t3 - t2) * 1e3, "total_ms": (t3 - t0) * 1e3,
-              "fine_patches_after": len(h2.level_patches(1)), "copied_cells": int(len(tr["copy"][0])),
-              "refined_cells": int(len(tr["interp"][0])),
-              "planner": "native (enl_plan_build / enl_plan_transfer)"}
+              "fine_patches_after": len(h2.level_patches(1)), "unchanged_patches": int(len(tr["same"][0])),
+              "transfer_rectangles": int(len(rc)),
+              "copied_cells": int((rc[:, 4] * rc[:, 5])[rc[:, 6] == 0].sum()),
+              "refined_cells": int((rc[:, 4] * rc[:, 5])[rc[:, 6] == 1].sum()),
+              "planner": "native (enl_plan_build / enl_plan_transfer_rects), schedules as rectangles"}
     # the reference's cadence (input.txt:7, main.C:489-502): two graph steps,
     # then a regrid (device tags, native plan, new batch, transfer; the new
     # hierarchy's graph is captured on its first step), repeated
