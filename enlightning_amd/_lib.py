@@ -16,6 +16,7 @@ VARIANTS = [
     "weno_high_llf", "weno_high", "weno_low_llf", "weno_low", "drp",
 ]
 VARIANT_ID = {n: i for i, n in enumerate(VARIANTS)}
+PACK_SINGLE, PACK_STRIDED, PACK_CHUNKED = 0, 1, 2  # ENL_PACK_*
 KERNEL_KINDS = ["rhs", "runge_kutta", "update_state", "physical_bc", "copy_interior", "stage_fused"]
 
 # Every exported function and its prototype (argtypes, restype).
@@ -87,6 +88,7 @@ PROTOTYPES = {
     "enl_batch_set_coarsen": ([_P, _I] + [_P] * 4, _I),
     "enl_batch_set_tiles": ([_P, _I, _I], _I),
     "enl_batch_set_level_tiles": ([_P, _I, _I, _I], _I),
+    "enl_batch_set_level_packing": ([_P, _I, _I, _I], _I),
     "enl_switch_create": ([_P], _I),
     "enl_switch_destroy": ([_P], _I),
     "enl_switch_set_params": ([_P, _P], _I),

@@ -6,6 +6,13 @@ Every HIP source is compiled with:
                       with it to the last bit outside exp/log;
   -Xclang -target-feature -Xclang -packed-fp32-ops
                       no packed FP32 instructions on this hardware pool.
+The stage kernels' sources also take -mllvm -disable-machine-licm: their
+kernels loop over tiles (persistent packs) or WENO cells, and machine
+loop-invariant code motion moved the floating-point constants and LDS
+addresses of a whole stencil out of those loops into registers held across
+them (60-87 VGPRs spilled by the split kernel's pack loop; the WENO list
+kernel at 197 VGPRs against 162 without it).  It moves code only, so the
+results are the same bits.
 """
 import os
 import shutil
@@ -20,6 +27,8 @@ LIB = os.path.join(HERE, "libenlightning_hip.so")
 SOURCES = ["enl_rhs.hip", "enl_kernels.hip", "enl_abi.hip", "enl_driver.hip", "enl_batch.hip", "enl_batch32.hip", "enl_switch.hip",
            "enl_absorb.hip", "enl_source.hip", "enl_regrid.hip", "enl_batch_rects.hip"]
 ARCH = os.environ.get("ENL_OFFLOAD_ARCH", "gfx950")
+# sources whose kernels loop over tiles or cells (see the module docstring)
+NO_MACHINE_LICM = {"enl_rhs.hip", "enl_batch.hip", "enl_batch32.hip"}
 
 
 def hipcc():
@@ -49,7 +58,8 @@ def build(verbose=False, jobs=4):
     for src in SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
         objs.append(obj)
-        cmd = [cc] + flags() + ["-c", os.path.join(CSRC, src), "-o", obj]
+        extra = ["-mllvm", "-disable-machine-licm"] if src in NO_MACHINE_LICM else []
+        cmd = [cc] + flags() + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

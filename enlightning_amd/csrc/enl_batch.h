@@ -55,6 +55,23 @@ struct Batch {
   int ntiles = 0;
   int tile_bx[8], tile_by[8];     // tile shape per level
   int level_tile_begin[9] = {0};  // tiles sorted by level
+  // How a level's tiles are packed into the workgroups of its stage launch
+  // (ENL_PACK_*): one tile per workgroup, or a persistent grid of
+  // pack_wpc[level] workgroups per CU each working through a pack of tiles.
+  // packs + level_pack_begin[level] holds the level's npack + 1 pack
+  // offsets into its (reordered) tiles.
+  int pack_mode[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int pack_wpc[8] = {4, 4, 4, 4, 4, 4, 4, 4};
+  int ncu = 0;
+  int* packs = nullptr;
+  int level_pack_begin[9] = {0};
+  int level_npack[8] = {0};
+  bool tiles_dirty = true;  // tiles and packs are (re)built by ensure_tiles before the next launch
+  // reusable device buffers for uploads of rectangles and for tag maps
+  void* stage = nullptr;
+  size_t stage_cap = 0;
+  void* tagmap = nullptr;
+  size_t tagmap_cap = 0;
   bool all_pow2 = true;
   LevelFill fill[8];
   void* fill_buf[8] = {nullptr};  // one allocation per level holding its lists (rectangle path)
@@ -121,6 +138,15 @@ int absorb_level(Batch* b, int lev, int block, const int* reflect, const enl_gas
 __device__ inline double lin_refine(const double* c, long sy, double ax, double ay) {
   return c[0] + ax * (0.5 * (c[1] - c[-1])) + ay * (0.5 * (c[sy] - c[-sy]));
 }
+
+// Builds the tiles and packs of every level when a tile shape or packing
+// changed since the last build (or none was built): called by the entry
+// points that launch stage kernels, outside any graph capture.
+int ensure_tiles(Batch* b);
+
+// Device memory for `bytes` in *buf, reusing it when it holds enough (the
+// allocation is the batch's, freed with it).
+int reserve(Batch* b, void** buf, size_t* cap, size_t bytes);
 
 // Fills the ghosts of every level of state block `buf`, level by level.
 void fill_all(Batch* b, int buf, const int* reflect, hipStream_t s);

@@ -66,23 +66,71 @@ namespace {
 
 // One tile per workgroup: sensor + DRP cells finish here, WENO cells go to
 // the list (split_body).  The fused epilogue is capped at 128 VGPRs.
-template <int V, bool P, int BX, int BY>
+// The tile's LDS pointers re-based by an offset the compiler cannot see
+// through (a scalar zero written by inline asm in each iteration of a tile
+// loop): LDS addresses then depend on the iteration, so the loop-invariant
+// code motion does not hoist the stencil's LDS addresses of a whole tile
+// out of the loop and hold them in registers across it.
+template <int BX, int BY>
+__device__ inline TileLds<BX, BY> rebased(const TileLds<BX, BY>& L) {
+  int z;
+  asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+  TileLds<BX, BY> R;
+  for (int k = 0; k < 5; k++) R.w[k] = L.w[k] ? L.w[k] + z : nullptr;
+  R.p = L.p + z;
+  R.T = L.T + z;
+  R.u = L.u + z;
+  R.v = L.v + z;
+  R.s = L.s + z;
+  R.tn = L.tn + z;
+  return R;
+}
+
+// PACKED = false: one tile per workgroup, tiles[blockIdx.x] (make_tiles
+// stores them in dispatch order).  PACKED = true: workgroup blockIdx.x works
+// through its pack, tiles packs[w] .. packs[w + 1] - 1, with a barrier
+// between tiles for the LDS.
+template <int V, bool P, int BX, int BY, bool PACKED>
 __global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(5)))
-batch_split_kernel(const int4* __restrict__ tiles, int ntiles, const RhsArgs* __restrict__ args) {
+batch_split_kernel(const int4* __restrict__ tiles, const int* __restrict__ packs, const RhsArgs* __restrict__ args) {
   ENL_BATCH_LDS(BX, BY, L, false)
-  const int4 tl = tiles[xcd_tile((int)blockIdx.x, ntiles)];
-  split_body<V, P, true, BX, BY>(args[tl.x], tl.y, tl.z, (unsigned)tl.x, L);
+  if constexpr (!PACKED) {
+    const int4 tl = tiles[blockIdx.x];
+    split_body<V, P, true, BX, BY>(args[tl.x], tl.y, tl.z, (unsigned)tl.x, L);
+  } else {
+    const int q1 = packs[blockIdx.x + 1];
+#pragma clang loop unroll(disable)
+    for (int q = packs[blockIdx.x]; q < q1; q++) {
+      const int4 tl = tiles[q];
+      TileLds<BX, BY> Lq = rebased(L);
+      split_body<V, P, true, BX, BY>(args[tl.x], tl.y, tl.z, (unsigned)tl.x, Lq);
+      __syncthreads();
+    }
+  }
 }
 
 // Every cell of the tile WENO (levels above 0, rhs_weno_* variants): the
 // two directions on two thread planes (dense2_body), block (BX, BY, 2).
-template <int V, bool P, int BX, int BY>
-__global__ void __launch_bounds__(2 * BX * BY) __attribute__((amdgpu_waves_per_eu(4))) batch_dense_kernel(const int4* __restrict__ tiles, int ntiles,
-                                                                  const RhsArgs* __restrict__ args) {
+template <int V, bool P, int BX, int BY, bool PACKED>
+__global__ void __launch_bounds__(2 * BX * BY) __attribute__((amdgpu_waves_per_eu(4)))
+batch_dense_kernel(const int4* __restrict__ tiles, const int* __restrict__ packs, const RhsArgs* __restrict__ args) {
   ENL_BATCH_LDS(BX, BY, L, true)
   __shared__ double lds_x[10 * BX * BY];
-  const int4 tl = tiles[xcd_tile((int)blockIdx.x, ntiles)];
-  dense2_body<V, P, true, BX, BY>(args[tl.x], tl.y, tl.z, L, lds_x);
+  if constexpr (!PACKED) {
+    const int4 tl = tiles[blockIdx.x];
+    dense2_body<V, P, true, BX, BY>(args[tl.x], tl.y, tl.z, L, lds_x);
+  } else {
+    const int q1 = packs[blockIdx.x + 1];
+#pragma clang loop unroll(disable)
+    for (int q = packs[blockIdx.x]; q < q1; q++) {
+      const int4 tl = tiles[q];
+      TileLds<BX, BY> Lq = rebased(L);
+      int z;
+      asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+      dense2_body<V, P, true, BX, BY>(args[tl.x], tl.y, tl.z, Lq, lds_x + z);
+      __syncthreads();
+    }
+  }
 }
 
 // The listed WENO cells of the level, ten lanes each (list_block).
@@ -298,28 +346,108 @@ void release(Batch* b, void* p) {
     if (a == p) a = nullptr;
 }
 
+namespace {
+int make_tiles(Batch* b);
+}  // namespace
+
+int ensure_tiles(Batch* b) { return b->tiles_dirty ? make_tiles(b) : 0; }
+
 
 namespace {
 
 
+// Tiles of every level (patch by patch, row-major inside a patch) and their
+// packing into workgroups.  Workgroup w of a launch runs on XCD w % 8 (the
+// dispatcher deals workgroups round-robin over the 8 XCDs), so each XCD is
+// given one contiguous range of the level's tiles, and its workgroups split
+// that range:
+//   ENL_PACK_SINGLE   one tile per workgroup, in the range's order (the
+//                     dispatch order of xcd_tile);
+//   ENL_PACK_STRIDED  a persistent grid of wpc workgroups per CU, the j-th
+//                     workgroup of an XCD taking tiles j, j + P, j + 2P, ...
+//                     of its range (P workgroups on the XCD): neighbouring
+//                     tiles run side by side, as dispatched singly;
+//   ENL_PACK_CHUNKED  the same grid, each workgroup taking one contiguous
+//                     run of the range, the runs of equal estimated cost (a
+//                     tile's valid cells plus half its area for the staging
+//                     and the halo): a workgroup walks along its patches.
+// The reordered tiles go to b->tiles, the pack offsets to b->packs.
 int make_tiles(Batch* b) {
+  if (!b->ncu) {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                 hipSuccess || n <= 0)
+      n = 256;
+    b->ncu = n;
+  }
   std::vector<int4> t;
+  std::vector<int> packs;
   for (int lev = 0; lev < kMaxLevels; lev++) {
     b->level_tile_begin[lev] = (int)t.size();
+    b->level_pack_begin[lev] = (int)packs.size();
     const int bx = b->tile_bx[lev], by = b->tile_by[lev];
+    std::vector<int4> T;
+    std::vector<double> cost;
     for (int p = 0; p < b->npatch; p++)
       if (b->hp[p].level == lev)
         for (int j0 = 0; j0 < b->hp[p].ny; j0 += by)
-          for (int i0 = 0; i0 < b->hp[p].nx; i0 += bx) t.push_back(make_int4(p, i0, j0, 0));
+          for (int i0 = 0; i0 < b->hp[p].nx; i0 += bx) {
+            T.push_back(make_int4(p, i0, j0, 0));
+            cost.push_back(std::min(bx, b->hp[p].nx - i0) * std::min(by, b->hp[p].ny - j0) + 0.5 * bx * by);
+          }
+    const int nt = (int)T.size();
+    if (nt == 0) {
+      b->level_npack[lev] = 0;
+      continue;
+    }
+    const int q8 = nt / 8, r8 = nt % 8;
+    auto range0 = [&](int x) { return x * q8 + std::min(x, r8); };  // XCD x: tiles [range0(x), range0(x + 1))
+    const int mode = b->pack_mode[lev];
+    const int G = mode == ENL_PACK_SINGLE ? nt : std::min(nt, std::max(8, b->ncu * b->pack_wpc[lev]));
+    std::vector<std::vector<int>> own((size_t)G);  // tile indices of each pack
+    for (int x = 0; x < 8; x++) {
+      const int a0 = range0(x), a1 = range0(x + 1);
+      std::vector<int> ws;  // the packs on XCD x
+      for (int w = x; w < G; w += 8) ws.push_back(w);
+      const int P = (int)ws.size();
+      if (a1 <= a0 || P == 0) continue;
+      if (mode == ENL_PACK_CHUNKED) {
+        double total = 0;
+        for (int k = a0; k < a1; k++) total += cost[(size_t)k];
+        double acc = 0;
+        int j = 0;
+        for (int k = a0; k < a1; k++) {
+          // the next pack starts when this one has its share of the cost
+          while (j + 1 < P && acc >= total * (j + 1) / P) j++;
+          own[(size_t)ws[(size_t)j]].push_back(k);
+          acc += cost[(size_t)k];
+        }
+      } else {
+        for (int k = a0; k < a1; k++) own[(size_t)ws[(size_t)((k - a0) % P)]].push_back(k);
+      }
+    }
+    const int base = (int)t.size();
+    int npack = 0;
+    for (int w = 0; w < G; w++) {
+      if (own[(size_t)w].empty()) continue;
+      packs.push_back((int)t.size() - base);
+      for (int k : own[(size_t)w]) t.push_back(T[(size_t)k]);
+      npack++;
+    }
+    packs.push_back((int)t.size() - base);
+    b->level_npack[lev] = npack;
   }
   b->level_tile_begin[kMaxLevels] = (int)t.size();
-  if (b->tiles) {
-    batch_free(b->tiles);
-    for (auto& a : b->allocations)
-      if (a == b->tiles) a = nullptr;
-  }
+  b->level_pack_begin[kMaxLevels] = (int)packs.size();
+  release(b, b->tiles);
+  release(b, b->packs);
+  b->tiles = nullptr;
+  b->packs = nullptr;
   b->ntiles = (int)t.size();
-  return upload(b, &b->tiles, t.data(), (long)t.size());
+  int rc = upload(b, &b->tiles, t.data(), (long)t.size());
+  rc = rc ? rc : upload(b, &b->packs, packs.data(), (long)packs.size());
+  if (!rc) b->tiles_dirty = false;
+  return rc;
 }
 
 
@@ -407,16 +535,24 @@ int prepare_args(Batch* b, const enl_gas_t* gas, int order, const double* a1, co
 template <int V, bool P, int BX, int BY>
 void stage_launch(Batch* b, const RhsArgs* args, int st, int lev, hipStream_t s) {
   using Sc = Scheme<V>;
-  const int t0 = b->level_tile_begin[lev];
-  const int nt = b->level_tile_begin[lev + 1] - t0;
-  if (nt <= 0) return;
-  const int4* tiles = b->tiles + t0;
+  const int np = b->level_npack[lev];
+  if (np <= 0) return;
+  const int4* tiles = b->tiles + b->level_tile_begin[lev];
+  const int* packs = b->packs + b->level_pack_begin[lev];
+  const bool packed = b->pack_mode[lev] != ENL_PACK_SINGLE;
   if (!Sc::drp || (Sc::hybrid && lev != 0)) {
     // rhs_weno_*, and hybrid variants above level 0 (the sensor is off)
-    hipLaunchKernelGGL((batch_dense_kernel<V, P, BX, BY>), dim3(nt), dim3(BX, BY, 2), 0, s, tiles, nt, args);
+    if (packed)
+      hipLaunchKernelGGL((batch_dense_kernel<V, P, BX, BY, true>), dim3(np), dim3(BX, BY, 2), 0, s, tiles, packs, args);
+    else
+      hipLaunchKernelGGL((batch_dense_kernel<V, P, BX, BY, false>), dim3(np), dim3(BX, BY, 2), 0, s, tiles, packs,
+                         args);
     return;
   }
-  hipLaunchKernelGGL((batch_split_kernel<V, P, BX, BY>), dim3(nt), dim3(BX, BY), 0, s, tiles, nt, args);
+  if (packed)
+    hipLaunchKernelGGL((batch_split_kernel<V, P, BX, BY, true>), dim3(np), dim3(BX, BY), 0, s, tiles, packs, args);
+  else
+    hipLaunchKernelGGL((batch_split_kernel<V, P, BX, BY, false>), dim3(np), dim3(BX, BY), 0, s, tiles, packs, args);
   if (Sc::hybrid)
     hipLaunchKernelGGL((batch_list_kernel<V, P>), dim3(grid_for(b->level_cells[lev], kListCellsPerBlock, 4096)), dim3(256), 0, s,
                        args, (const unsigned long long*)b->list,
@@ -526,8 +662,9 @@ int enqueue_batched(Batch* b, int variant, const enl_gas_t* gas, const int* refl
     if (order == 1) {
       for (int p = 0; p < b->npatch; p++) {
         const PatchDev& pd = b->hp[p];
-        hipMemcpyAsync(b->pool + pd.state[0], b->pool + pd.state[1], sizeof(double) * 7 * pd.sk,
-                       hipMemcpyDeviceToDevice, s);
+        hipError_t e = hipMemcpyAsync(b->pool + pd.state[0], b->pool + pd.state[1], sizeof(double) * 7 * pd.sk,
+                                      hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return check_hip(e, "RK1 state copy");
       }
     }
     if (b->ncoarsen)
@@ -546,6 +683,7 @@ int enqueue_batched(Batch* b, int variant, const enl_gas_t* gas, const int* refl
 // and a temporary argument table.
 int batch_eval_rhs64(Batch* b, int variant, const enl_gas_t* gas, const int* reflect, int in, int out,
                      hipStream_t s) {
+  if (int rc = ensure_tiles(b)) return rc;
   std::vector<RhsArgs> t((size_t)b->npatch, base_args(b, gas));
   for (int p = 0; p < b->npatch; p++) {
     t[p].alpha_1 = 0;
@@ -707,7 +845,7 @@ int enl_batch_create(int npatch, const enl_patch_desc_t* patches, int gcw, int r
     b->tile_bx[lev] = 32;
     b->tile_by[lev] = 8;
   }
-  if (!rc) rc = make_tiles(b);
+  b->tiles_dirty = true;  // built by the first launch (ensure_tiles)
   if (rc) {
     for (void* a : b->allocations)
       if (a) batch_free(a);
@@ -838,7 +976,8 @@ int enl_batch_set_tiles(void* handle, int bx, int by) {
     b->tile_bx[lev] = bx;
     b->tile_by[lev] = by;
   }
-  return make_tiles(b);
+  b->tiles_dirty = true;
+  return 0;
 }
 
 int enl_batch_set_level_tiles(void* handle, int level, int bx, int by) {
@@ -849,7 +988,21 @@ int enl_batch_set_level_tiles(void* handle, int level, int bx, int by) {
   drop_graph(b);
   b->tile_bx[level] = bx;
   b->tile_by[level] = by;
-  return make_tiles(b);
+  b->tiles_dirty = true;
+  return 0;
+}
+
+int enl_batch_set_level_packing(void* handle, int level, int mode, int wpc) {
+  Batch* b = (Batch*)handle;
+  if (!b) return set_error(-2, "null batch");
+  if (level < 0 || level >= kMaxLevels) return set_error(-2, "bad level %d", level);
+  if (mode < ENL_PACK_SINGLE || mode > ENL_PACK_CHUNKED || wpc < 1 || wpc > 32)
+    return set_error(-2, "enl_batch_set_level_packing: bad mode %d or workgroups per CU %d", mode, wpc);
+  drop_graph(b);
+  b->pack_mode[level] = mode;
+  b->pack_wpc[level] = wpc;
+  b->tiles_dirty = true;
+  return 0;
 }
 
 int enl_batch_patch_state(void* handle, int pid, double** w_n, double** pres, double** temp, double** source) {
@@ -979,6 +1132,7 @@ int enl_batch_advance(void* handle, int mode, int variant, const enl_gas_t* gas,
                       const double* alpha_1, const double* alpha_2, const double* beta, double dt, int nsteps) {
   Batch* b = (Batch*)handle;
   if (!b) return set_error(-2, "null batch");
+  if (int rc = ensure_tiles(b)) return rc;
   if (variant < 0 || variant >= kNumVariants) return set_error(-1, "unknown rhs variant");
   if (order < 1 || order > kMaxStages) return set_error(-2, "RK order out of range");
   hipStream_t s = lib_stream();
@@ -1009,6 +1163,8 @@ int enl_batch_advance(void* handle, int mode, int variant, const enl_gas_t* gas,
   put(beta, order * sizeof(double));
   put(b->tile_bx, sizeof b->tile_bx);
   put(b->tile_by, sizeof b->tile_by);
+  put(b->pack_mode, sizeof b->pack_mode);
+  put(b->pack_wpc, sizeof b->pack_wpc);
   put(&b->absorb, sizeof b->absorb);
   put(&b->ab_rects, sizeof b->ab_rects);
   bool same = b->graph_exec && key == b->graph_key;

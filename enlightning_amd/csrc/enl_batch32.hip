@@ -292,14 +292,9 @@ __device__ inline float dflux_y(const Lds32<BX, BY>& L, int e1, int e2, int r1, 
 // or as the learned switch's decision map says where one is given).
 // EVAL: store K32 into kout instead of stepping.
 template <int BX, int BY, bool SENSOR, bool EVAL>
-__global__ void __launch_bounds__(BX* BY) stage32_kernel(const int4* __restrict__ tiles, int ntiles,
-                                                         const Args32* __restrict__ args) {
+__device__ __attribute__((always_inline)) inline void stage32_body(const Args32& a, int i0, int j0, Lds32<BX, BY>& L) {
   using L_t = Lds32<BX, BY>;
   constexpr int TW = L_t::TW, TH = L_t::TH, TN = L_t::TN, S = TW;
-  __shared__ L_t L;
-  const int4 tl = tiles[xcd_tile((int)blockIdx.x, ntiles)];
-  const Args32& a = args[tl.x];
-  const int i0 = tl.y, j0 = tl.z;
   const int tid = threadIdx.x + threadIdx.y * BX;
   const int jr = j0 - 3;  // local row of staged row 0 (>= -g since g >= 3)
   // per-row ambient of the staged rows inside the ghosted array
@@ -517,6 +512,33 @@ __global__ void __launch_bounds__(BX* BY) stage32_kernel(const int4* __restrict_
     const float Tpn = L.row[r].aT * expm1f((sp + a.R * log1pf(rp * rcpf(L.row[r].aRho))) * a.inv_cv);
     a.out[gx + 5 * a.sk] = pp;
     a.out[gx + 6 * a.sk] = Tpn;
+  }
+}
+
+// PACKED = false: one tile per workgroup, tiles[blockIdx.x] (make_tiles
+// stores them in dispatch order).  PACKED = true: workgroup blockIdx.x works
+// through its pack, tiles packs[w] .. packs[w + 1] - 1, with a barrier
+// between tiles for the LDS; the LDS is addressed from a base the compiler
+// cannot see through in each iteration, so that it does not hoist a whole
+// tile's stencil addresses out of the loop into registers.
+template <int BX, int BY, bool SENSOR, bool EVAL, bool PACKED>
+__global__ void __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(4))) stage32_kernel(const int4* __restrict__ tiles, const int* __restrict__ packs,
+                                                         const Args32* __restrict__ args) {
+  __shared__ Lds32<BX, BY> L;
+  if constexpr (!PACKED) {
+    const int4 tl = tiles[blockIdx.x];
+    stage32_body<BX, BY, SENSOR, EVAL>(args[tl.x], tl.y, tl.z, L);
+  } else {
+    const int q1 = packs[blockIdx.x + 1];
+#pragma clang loop unroll(disable)
+    for (int q = packs[blockIdx.x]; q < q1; q++) {
+      const int4 tl = tiles[q];
+      int z;
+      asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+      Lds32<BX, BY>& Lq = *(Lds32<BX, BY>*)((char*)&L + z);
+      stage32_body<BX, BY, SENSOR, EVAL>(args[tl.x], tl.y, tl.z, Lq);
+      __syncthreads();
+    }
   }
 }
 
@@ -757,13 +779,19 @@ void fill32_level(Batch* b, int level, int buf, int cbuf, const int* reflect, hi
 
 template <int BX, int BY, bool EVAL>
 void stage32_launch(Batch* b, const Args32* args, int lev, hipStream_t s) {
-  const int t0 = b->level_tile_begin[lev];
-  const int nt = b->level_tile_begin[lev + 1] - t0;
-  if (nt <= 0) return;
-  if (lev == 0)
-    hipLaunchKernelGGL((stage32_kernel<BX, BY, true, EVAL>), dim3(nt), dim3(BX, BY), 0, s, b->tiles + t0, nt, args);
+  const int np = b->level_npack[lev];
+  if (np <= 0) return;
+  const int4* tiles = b->tiles + b->level_tile_begin[lev];
+  const int* packs = b->packs + b->level_pack_begin[lev];
+  const bool packed = b->pack_mode[lev] != ENL_PACK_SINGLE;
+  if (lev == 0 && packed)
+    hipLaunchKernelGGL((stage32_kernel<BX, BY, true, EVAL, true>), dim3(np), dim3(BX, BY), 0, s, tiles, packs, args);
+  else if (lev == 0)
+    hipLaunchKernelGGL((stage32_kernel<BX, BY, true, EVAL, false>), dim3(np), dim3(BX, BY), 0, s, tiles, packs, args);
+  else if (packed)
+    hipLaunchKernelGGL((stage32_kernel<BX, BY, false, EVAL, true>), dim3(np), dim3(BX, BY), 0, s, tiles, packs, args);
   else
-    hipLaunchKernelGGL((stage32_kernel<BX, BY, false, EVAL>), dim3(nt), dim3(BX, BY), 0, s, b->tiles + t0, nt,
+    hipLaunchKernelGGL((stage32_kernel<BX, BY, false, EVAL, false>), dim3(np), dim3(BX, BY), 0, s, tiles, packs,
                        args);
 }
 
@@ -1085,6 +1113,10 @@ int fp32_advance(Batch* b, int mode, const enl_gas_t* gas, const int* reflect, i
   put(reflect, 4 * sizeof(int));
   put(b->tile_bx, sizeof b->tile_bx);
   put(b->tile_by, sizeof b->tile_by);
+  put(b->pack_mode, sizeof b->pack_mode);
+  put(b->pack_wpc, sizeof b->pack_wpc);
+  put(&b->tiles, sizeof b->tiles);
+  put(&b->packs, sizeof b->packs);
   put(&F->sw, sizeof F->sw);
   put(&F->lo, sizeof F->lo);
   put(&F->hi, sizeof F->hi);
@@ -1172,6 +1204,7 @@ int enl_batch_set_ambient(void* handle, int level, int row_lo, int nrows, const 
 int enl_batch_fp32_prepare(void* handle, int variant, const enl_gas_t* gas, const int* reflect) {
   Batch* b = (Batch*)handle;
   if (!b || !gas || !reflect) return set_error(-2, "enl_batch_fp32_prepare: bad arguments");
+  if (int rc = ensure_tiles(b)) return rc;
   if (variant != kHybridHighLlf) return set_error(-1, "the FP32 form implements rhs_hybrid_high_llf only");
   if (!b->f32) return set_error(-2, "enl_batch_fp32_prepare: no ambient set");
   for (int l = 0; l < b->nlevels; l++)

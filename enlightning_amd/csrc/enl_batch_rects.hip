@@ -85,6 +85,21 @@ hipError_t batch_malloc(void** p, size_t bytes) {
   return hipSuccess;
 }
 
+int reserve(Batch* b, void** buf, size_t* cap, size_t bytes) {
+  if (*buf && *cap >= bytes) return 0;
+  release(b, *buf);
+  *buf = nullptr;
+  *cap = 0;
+  hipError_t e = batch_malloc(buf, bytes);
+  if (e != hipSuccess) {
+    *buf = nullptr;
+    return check_hip(e, "hipMalloc(schedule lists)");
+  }
+  b->allocations.push_back(*buf);
+  *cap = bytes;
+  return 0;
+}
+
 void batch_free(void* p) {
   if (!p) return;
   // as hipFree: queued work that may still read the block ends first
@@ -256,37 +271,19 @@ __global__ void __launch_bounds__(256) tag_map_kernel(const double* pool, const 
   }
 }
 
-// Device memory for `bytes`, reusing *buf when it holds enough.
-int reserve(Batch* b, void** buf, size_t* cap, size_t bytes) {
-  if (*buf && *cap >= bytes) return 0;
-  release(b, *buf);
-  *buf = nullptr;
-  *cap = 0;
-  hipError_t e = batch_malloc(buf, bytes);
-  if (e != hipSuccess) {
-    *buf = nullptr;
-    return check_hip(e, "hipMalloc(schedule lists)");
-  }
-  b->allocations.push_back(*buf);
-  *cap = bytes;
-  return 0;
-}
 
-// Uploads n rects and their offsets (off: n entries, or none) into a
-// temporary device buffer (freed by the caller after the stream has
-// consumed it).
-int upload_rects(const enl_rect_t* rects, const std::vector<int>& off, int n, char** dev, hipStream_t s) {
+// Uploads n rects and their offsets (off: n entries, or none) into the
+// batch's staging buffer; the caller synchronizes the stream before the
+// buffer is used again.
+int upload_rects(Batch* b, const enl_rect_t* rects, const std::vector<int>& off, int n, char** dev, hipStream_t s) {
   const size_t rb = sizeof(enl_rect_t) * (size_t)n, ob = sizeof(int) * off.size();
   *dev = nullptr;
-  hipError_t e = batch_malloc((void**)dev, rb + ob);
-  if (e != hipSuccess) return check_hip(e, "hipMalloc(rectangles)");
-  e = hipMemcpyAsync(*dev, rects, rb, hipMemcpyHostToDevice, s);
+  int rc = reserve(b, &b->stage, &b->stage_cap, rb + ob);
+  if (rc) return rc;
+  *dev = (char*)b->stage;
+  hipError_t e = hipMemcpyAsync(*dev, rects, rb, hipMemcpyHostToDevice, s);
   if (e == hipSuccess && ob) e = hipMemcpyAsync(*dev + rb, off.data(), ob, hipMemcpyHostToDevice, s);
-  if (e != hipSuccess) {
-    batch_free(*dev);
-    *dev = nullptr;
-    return check_hip(e, "upload rectangles");
-  }
+  if (e != hipSuccess) return check_hip(e, "upload rectangles");
   return 0;
 }
 
@@ -371,13 +368,12 @@ int enl_batch_set_fill_rects(void* handle, int level, int n1, int n2, const enl_
   if (n == 0) return 0;
   hipStream_t s = lib_stream();
   char* dev = nullptr;
-  rc = upload_rects(rects, off, n, &dev, s);
+  rc = upload_rects(b, rects, off, n, &dev, s);
   if (rc) return rc;
   hipLaunchKernelGGL(expand_fill_kernel, dim3(std::min(n, 4096)), dim3(256), 0, s, (const enl_rect_t*)dev,
                      (const int*)(dev + sizeof(enl_rect_t) * (size_t)n), n, n1, b->dp, b->ratio, L);
   hipError_t e = hipGetLastError();
   const hipError_t e2 = hipStreamSynchronize(s);
-  batch_free(dev);
   if (e != hipSuccess) return check_hip(e, "expand fill rectangles");
   return check_hip(e2, "expand fill rectangles sync");
 }
@@ -412,14 +408,13 @@ int enl_batch_set_coarsen_rects(void* handle, int n, const enl_rect_t* rects) {
   if (n == 0) return 0;
   hipStream_t s = lib_stream();
   char* dev = nullptr;
-  rc = upload_rects(rects, off, n, &dev, s);
+  rc = upload_rects(b, rects, off, n, &dev, s);
   if (rc) return rc;
   hipLaunchKernelGGL(expand_coarsen_kernel, dim3(std::min(n, 4096)), dim3(256), 0, s, (const enl_rect_t*)dev,
                      (const int*)(dev + sizeof(enl_rect_t) * (size_t)n), n, b->dp, r, b->co_cp, b->co_cl, b->co_fp,
                      b->co_fl);
   hipError_t e = hipGetLastError();
   const hipError_t e2 = hipStreamSynchronize(s);
-  batch_free(dev);
   if (e != hipSuccess) return check_hip(e, "expand coarsen rectangles");
   return check_hip(e2, "expand coarsen rectangles sync");
 }
@@ -469,13 +464,12 @@ int enl_batch_transfer_rects(void* new_handle, void* old_handle, const int* refl
   const int n = (int)all.size();
   if (n > 0) {
     char* dev = nullptr;
-    int rc = upload_rects(all.data(), std::vector<int>(), n, &dev, s);
+    int rc = upload_rects(nb, all.data(), std::vector<int>(), n, &dev, s);
     if (rc) return rc;
     hipLaunchKernelGGL(transfer_rect_kernel, dim3(std::min(n, 8192)), dim3(256), 0, s, nb->pool, nb->dp, ob->pool,
                        ob->dp, (const enl_rect_t*)dev, n, r);
     hipError_t e = hipGetLastError();
     const hipError_t e2 = hipStreamSynchronize(s);
-    batch_free(dev);
     if (e != hipSuccess) return check_hip(e, "transfer rectangles launch");
     if (e2 != hipSuccess) return check_hip(e2, "transfer rectangles sync");
   }
@@ -504,8 +498,9 @@ int enl_batch_tag_map(void* handle, int level, const int* reflect, int refine_ta
   const bool dev = is_device_ptr(map);
   int* out = map;
   if (!dev) {
-    hipError_t e = batch_malloc((void**)&out, sizeof(int) * (size_t)cells);
-    if (e != hipSuccess) return check_hip(e, "hipMalloc(tag map)");
+    int rc = reserve(b, &b->tagmap, &b->tagmap_cap, sizeof(int) * (size_t)cells);
+    if (rc) return rc;
+    out = (int*)b->tagmap;
   }
   // the stencil reads the state's ghosts; the source's are filled by
   // enl_batch_prepare
@@ -519,7 +514,6 @@ int enl_batch_tag_map(void* handle, int level, const int* reflect, int refine_ta
   hipError_t e = hipGetLastError();
   if (e == hipSuccess && !dev) e = hipMemcpyAsync(map, out, sizeof(int) * (size_t)cells, hipMemcpyDeviceToHost, s);
   const hipError_t e2 = dev ? hipSuccess : hipStreamSynchronize(s);
-  if (!dev) batch_free(out);
   if (e != hipSuccess) return check_hip(e, "enl_batch_tag_map");
   return check_hip(e2, "enl_batch_tag_map sync");
 }

@@ -168,6 +168,7 @@ class DeviceHierarchy:
         self.h, self.gas, self.reflect = h, gas, tuple(reflect)
         self.absorb_width = 0.0
         self.tiles = {}  # level -> (bx, by) set by set_tiles / autotune; None: every level
+        self.packing = {}  # level -> (mode, workgroups per CU) set by set_packing / autotune
         self.lib = _lib.load()
         n = len(h.patches)
         # the patch table (enl_patch_desc_t rows) built with numpy
@@ -219,6 +220,20 @@ class DeviceHierarchy:
         _lib.check(self.lib.enl_batch_set_absorb(self.handle, D(0.0, 0.0), D(n0 * dx0, n0 * dx0), float(width)))
         self.absorb_width = float(width)
 
+    # how a level's tiles are packed into the stage launch's workgroups
+    # (enl_batch_set_level_packing): (mode, workgroups per CU)
+    PACKINGS = ((_lib.PACK_SINGLE, 0), (_lib.PACK_STRIDED, 2), (_lib.PACK_STRIDED, 4), (_lib.PACK_CHUNKED, 2),
+                (_lib.PACK_CHUNKED, 4))
+    PACK_NAMES = {_lib.PACK_SINGLE: "single", _lib.PACK_STRIDED: "strided", _lib.PACK_CHUNKED: "chunked"}
+
+    def set_packing(self, level, mode, wpc=4):
+        _lib.check(self.lib.enl_batch_set_level_packing(self.handle, level, int(mode), max(1, int(wpc))))
+        self.packing[level] = (int(mode), int(wpc))
+
+    def packing_name(self, level):
+        mode, wpc = self.packing.get(level, (_lib.PACK_SINGLE, 0))
+        return self.PACK_NAMES[mode] + ("" if mode == _lib.PACK_SINGLE else "/%d per CU" % wpc)
+
     def set_tiles(self, bx, by, level=None):
         if level is None:
             _lib.check(self.lib.enl_batch_set_tiles(self.handle, bx, by))
@@ -228,11 +243,14 @@ class DeviceHierarchy:
             self.tiles[level] = (bx, by)
 
     def autotune(self, dt, states, wl, steps=2, variant="hybrid_high_llf", rk=RK3, mode="graph"):
-        """Tile shape per level by coordinate descent over measured steps of
-        `mode` ("graph" or "fp32_graph"): each level in turn tries every shape
-        with the others fixed and keeps the fastest.  Reloads `states` before
-        each trial; returns {level: (bx, by)} and the per-trial times (ms per
-        step)."""
+        """Tile shape and packing per level by coordinate descent over
+        measured steps of `mode` ("graph" or "fp32_graph"): each level in
+        turn tries every tile shape with the others fixed and keeps the
+        fastest, then every packing of its tiles into workgroups (one tile
+        per workgroup, or persistent packs: strided or cost-balanced chunks
+        at 2 or 4 workgroups per CU) at that shape.  Reloads `states` before
+        each trial; returns {level: (bx, by)} (the packings are in
+        self.packing) and the per-trial times (ms per step)."""
         import time
 
         def reload():
@@ -256,9 +274,17 @@ class DeviceHierarchy:
             for shape in self.TILE_SHAPES:
                 self.set_tiles(*shape, level=lev)
                 times[shape] = measure()
-                trials["L%d:%dx%d" % ((lev,) + shape)] = times[shape]
+                trials["L%d:%dx%d:%s" % ((lev,) + shape + (self.packing_name(lev),))] = times[shape]
             best[lev] = min(times, key=times.get)
             self.set_tiles(*best[lev], level=lev)
+            ptimes = {self.packing.get(lev, self.PACKINGS[0]): times[best[lev]]}
+            for pk in self.PACKINGS:
+                if pk in ptimes:
+                    continue
+                self.set_packing(lev, *pk)
+                ptimes[pk] = measure()
+                trials["L%d:%dx%d:%s" % ((lev,) + best[lev] + (self.packing_name(lev),))] = ptimes[pk]
+            self.set_packing(lev, *min(ptimes, key=ptimes.get))
         return best, trials
 
     def load_initial(self, wl, states=None, sources=None):
@@ -413,8 +439,8 @@ class DeviceHierarchy:
         amr.make_transfer: unchanged patches whole, copies from the old
         patches of each level, linear refinement from the old coarser level
         elsewhere).  This hierarchy
-        keeps its interiors and stays usable; absorbing envelopes and tile
-        shapes carry over.  The FP64 state moves: after FP32-form steps call
+        keeps its interiors and stays usable; absorbing envelopes, tile
+        shapes and packings carry over.  The FP64 state moves: after FP32-form steps call
         fp32_store() first."""
         if transfer is None:
             tr = amr.transfer_rects(new_h, self.h) if new_h.plan is not None else amr.make_transfer(new_h, self.h)
@@ -440,6 +466,9 @@ class DeviceHierarchy:
             for lev, (bx, by) in sorted(self.tiles.items(), key=lambda kv: -1 if kv[0] is None else kv[0]):
                 if lev is None or lev < new_h.nlevels:
                     new.set_tiles(bx, by, level=lev)
+            for lev, pk in sorted(self.packing.items()):
+                if lev < new_h.nlevels:
+                    new.set_packing(lev, *pk)
         except Exception:
             new.close()
             raise

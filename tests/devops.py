@@ -116,6 +116,9 @@ else:
         def set_tiles(self, bx, by, level=None):
             pass
 
+        def set_packing(self, level, mode, wpc=4):
+            pass
+
         def set_absorb(self, width):
             self.absorb_width = width
 

@@ -65,25 +65,57 @@ def same(sa, sb, h):
     return ok
 
 
-def test_modes_and_tiles_bit_identical():
+# packings of a level's tiles into workgroups (ENL_PACK_*: single, strided,
+# chunked) with workgroups per CU
+PACKS = ((0, 0), (1, 1), (1, 4), (2, 1), (2, 4))
+
+
+def test_modes_tiles_and_packings_bit_identical():
+    """Plain drop-in, fused and graph modes, every tile shape (also one per
+    level) and every packing of the tiles into workgroups (one tile each,
+    persistent strided or chunked packs, on either level) give the same
+    bits."""
     gas, h = make(WL128, 32)
     dt = dt_of(h, gas)
     results = []
-    for mode, tiles in (("plain", (32, 8)), ("fused", (32, 8)), ("graph", (32, 8)), ("fused", (16, 16)),
-                        ("fused", (16, 8)), ("graph", (8, 8)), ("graph", ((16, 16), (8, 8)))):
+    for mode, tiles, packs in (("plain", (32, 8), None), ("fused", (32, 8), None), ("graph", (32, 8), None),
+                               ("fused", (16, 16), None), ("fused", (16, 8), None), ("graph", (8, 8), None),
+                               ("graph", ((16, 16), (8, 8)), None), ("graph", (16, 16), ((1, 4), (2, 1))),
+                               ("fused", (8, 8), ((2, 4), (1, 1))), ("graph", ((32, 8), (16, 8)), ((1, 1), (2, 4)))):
         d = D.Hierarchy(h, gas)
         if isinstance(tiles[0], tuple):
             for lev, shape in enumerate(tiles):  # a shape per level
                 d.set_tiles(*shape, level=lev)
         else:
             d.set_tiles(*tiles)
+        for lev, pk in enumerate(packs or ()):
+            d.set_packing(lev, *pk)
         d.load_initial(WL128)
         d.advance(3, dt, mode=mode)
-        results.append(((mode, tiles), d.states()))
+        results.append(((mode, tiles, packs), d.states()))
         d.close()
     ref = results[0][1]
-    for (mode, tiles), st in results[1:]:
-        assert same(ref, st, h), (mode, tiles)
+    for key, st in results[1:]:
+        assert same(ref, st, h), key
+
+
+def test_packing_change_recaptures_the_graph():
+    """A packing set after a graph step takes effect in the next replay (the
+    graph is captured again), with the same bits."""
+    gas, h = make(WL128, 32)
+    dt = dt_of(h, gas)
+    d = D.Hierarchy(h, gas)
+    d.set_tiles(16, 8)
+    out = []
+    for pk in PACKS:
+        d.load_initial(WL128)
+        for lev in range(h.nlevels):
+            d.set_packing(lev, *pk)
+        d.advance(2, dt, mode="graph")
+        out.append(d.states())
+    d.close()
+    for st in out[1:]:
+        assert same(out[0], st, h)
 
 
 @pytest.mark.parametrize("variant", ["hybrid_low_llf", "hybrid_high", "weno_high_llf", "weno_low", "drp"])

@@ -64,13 +64,15 @@ def ambient_fn(h, gas, wl):
     return amb
 
 
-def run_both(steps, pulses=True, mode32="fp32_graph"):
+def run_both(steps, pulses=True, mode32="fp32_graph", packing=None):
     gas, h = hierarchy()
     states = patch_states(h, gas, WEAK, pulses)
     dt = 0.5 * h.dx(1) / gas.c
     d = D.Hierarchy(h, gas)
     if not hasattr(d, "fp32_prepare"):
         pytest.skip("the FP32 form has no CPU stand-in")
+    for lev, pk in enumerate(packing or ()):
+        d.set_packing(lev, *pk)
     d.load_initial(WEAK, states)
     d.advance(steps, dt, mode="graph")
     s64 = d.states()
@@ -149,6 +151,18 @@ def test_fp32_eager_equals_graph():
     for x, y in zip(a, b):
         for k in range(3):
             assert np.array_equal(x[k], y[k])
+
+
+def test_fp32_packings_bit_identical():
+    """The FP32 stage kernel over persistent packs of tiles (strided on level
+    0, cost-balanced chunks on level 1, and the reverse) gives the bits of
+    one tile per workgroup."""
+    _, _, _, a, _ = run_both(3)
+    for packing in (((1, 2), (2, 4)), ((2, 1), (1, 4))):
+        _, _, _, b, _ = run_both(3, packing=packing)
+        for x, y in zip(a, b):
+            for k in range(3):
+                assert np.array_equal(x[k], y[k]), packing
 
 
 def test_fp32_form_across_a_regrid():
