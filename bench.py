@@ -222,6 +222,7 @@ def amr_bench(args):
         "fine_patch_side_min_max": [int(min(sizes)), int(max(sizes))] if sizes else None,
         "autotune_ms_per_step": tune,
         "tiles_per_level": ["%dx%d" % best[l] for l in sorted(best)],
+        "packing_per_level": [d.packing_name(l) for l in sorted(best)],
         "graph": {"value": cells * args.amr_steps / t_graph, "ms_per_step": t_graph * 1e3 / args.amr_steps,
                   "steps": args.amr_steps},
         "plain_dropin_per_patch": {"value": cells * args.amr_plain_steps / t_plain,

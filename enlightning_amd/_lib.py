@@ -133,6 +133,8 @@ PROTOTYPES = {
     "enl_sample_mics": ([_I] * 4 + [_P, _P, _P, _I, _P, _P, _P, ctypes.c_long, _P, _P], _I),
     "enl_batch_destroy": ([_P], _I),
     "enl_batch_trim": ([], _I),
+    "enl_batch_gather": ([_P, _I, _I, _P, ctypes.c_long], _I),
+    "enl_batch_scatter": ([_P, _I, _P, ctypes.c_long], _I),
     "enl_advance_patch": ([_I] * 6 + [_P, _P, ctypes.POINTER(EnlGas), _P, _I, _P, _P, _P, _D, _I, _I]
                           + [_P] * 6 + [_D], _I),
 }

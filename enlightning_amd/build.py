@@ -25,7 +25,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libenlightning_hip.so")
 SOURCES = ["enl_rhs.hip", "enl_kernels.hip", "enl_abi.hip", "enl_driver.hip", "enl_batch.hip", "enl_batch32.hip", "enl_switch.hip",
-           "enl_absorb.hip", "enl_source.hip", "enl_regrid.hip", "enl_batch_rects.hip"]
+           "enl_absorb.hip", "enl_source.hip", "enl_regrid.hip", "enl_batch_rects.hip", "enl_batch_io.hip"]
 ARCH = os.environ.get("ENL_OFFLOAD_ARCH", "gfx950")
 # sources whose kernels loop over tiles or cells (see the module docstring)
 NO_MACHINE_LICM = {"enl_rhs.hip", "enl_batch.hip", "enl_batch32.hip"}

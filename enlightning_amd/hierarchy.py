@@ -312,38 +312,58 @@ class DeviceHierarchy:
                                               len(a1), D(*a1), D(*a2), D(*b), float(dt), int(nsteps)))
 
     def states(self):
-        """Per patch (w, p, T) host copies of the current state."""
-        out = []
-        for k, p in enumerate(self.h.patches):
-            w = np.empty((5, p.Ny, p.Nx))
-            pr = np.empty((p.Ny, p.Nx))
-            T = np.empty((p.Ny, p.Nx))
-            _lib.check(self.lib.enl_batch_copy_state(self.handle, k, 0, _p(w), _p(pr), _p(T), None))
-            out.append((w, pr, T))
-        return out
+        """Per patch (w, p, T) host copies of the current state (ghosted), from
+        one enl_batch_gather."""
+        return [(v[:5], v[5], v[6]) for v in self.gather(0x7F, ghosted=True)]
 
     def sources(self):
         """Per patch host copies of the ghosted source."""
-        out = []
-        for k, p in enumerate(self.h.patches):
-            src = np.empty((p.Ny, p.Nx))
-            _lib.check(self.lib.enl_batch_copy_state(self.handle, k, 0, None, None, None, _p(src)))
-            out.append(src)
+        return [v[0] for v in self.gather(0x80, ghosted=True)]
+
+    PLANES = ("w_n #0", "w_n #1", "w_n #2", "w_n #3", "w_n #4", "pressure", "temperature", "source")
+
+    def _plane_layout(self, planes, ghosted):
+        n = bin(planes).count("1")
+        shapes = [(n, p.Ny, p.Nx) if ghosted else (n, p.ny, p.nx) for p in self.h.patches]
+        return shapes, int(sum(np.prod(sh) for sh in shapes))
+
+    def gather(self, planes=0xFF, ghosted=True):
+        """Per patch an array (planes, rows, columns) of the selected SOLN
+        planes (bit k: w_n #k for k < 5, 5 pressure, 6 temperature, 7 source)
+        over the ghosted box or the interior, all from one device gather
+        (enl_batch_gather) into one host buffer."""
+        shapes, total = self._plane_layout(planes, ghosted)
+        buf = np.empty(total)
+        _lib.check(self.lib.enl_batch_gather(self.handle, int(planes), int(bool(ghosted)), _p(buf), total))
+        out, off = [], 0
+        for sh in shapes:
+            n = int(np.prod(sh))
+            out.append(buf[off:off + n].reshape(sh))
+            off += n
         return out
+
+    def scatter(self, planes, arrays):
+        """The inverse of gather over the ghosted boxes: per patch an array
+        (planes, Ny, Nx) into the device (enl_batch_scatter), no ghost fill."""
+        shapes, total = self._plane_layout(planes, True)
+        buf = np.empty(total)
+        off = 0
+        for sh, a in zip(shapes, arrays):
+            n = int(np.prod(sh))
+            buf[off:off + n] = np.asarray(a, dtype=np.float64).reshape(-1)
+            off += n
+        _lib.check(self.lib.enl_batch_scatter(self.handle, int(planes), _p(buf), total))
 
     def set_states(self, states):
         """Upload per patch (w, p, T) ghosted host arrays as the current state
         (no ghost fill, no update_state)."""
-        for k, (w, pr, T) in enumerate(states):
-            arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (w, pr, T)]
-            _lib.check(self.lib.enl_batch_copy_state(self.handle, k, 1, *[_p(a) for a in arrs], None))
+        self.scatter(0x7F, [np.concatenate([w.reshape(5, *pr.shape), pr[None], T[None]]) for w, pr, T in states])
 
     def upload_all(self, states, sources):
         """A saved state: per patch (w, p, T) and the ghosted source, then the
         source's and the state's ghost fills (p, T are taken as saved)."""
-        for k, ((w, pr, T), src) in enumerate(zip(states, sources)):
-            arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (w, pr, T, src)]
-            _lib.check(self.lib.enl_batch_copy_state(self.handle, k, 1, *[_p(a) for a in arrs]))
+        self.scatter(0xFF, [np.concatenate([w.reshape(5, *pr.shape), pr[None], T[None], s[None]])
+                            for (w, pr, T), s in zip(states, sources)])
         refl = self._reflect()
         _lib.check(self.lib.enl_batch_prepare(self.handle, refl))
         _lib.check(self.lib.enl_batch_fill(self.handle, refl))

@@ -99,7 +99,19 @@ def load_restart(path, factory=None):
 
 def plot_quantities(hier, viz_w_n=0, viz_pressure=1, viz_temperature=0, viz_source=0):
     """{name: [interior array of each patch]} in registerModelVariables'
-    order (enlightning.C:295-378); defaults are input.txt:9-12."""
+    order (enlightning.C:295-378); defaults are input.txt:9-12.  A device
+    hierarchy gathers every selected plane of every patch in one launch
+    (enl_batch_gather)."""
+    if hasattr(hier, "gather"):
+        planes = (0x1F if viz_w_n else 0) | (0x20 if viz_pressure else 0) | (0x40 if viz_temperature else 0) | (
+            0x80 if viz_source else 0)
+        out = {}
+        if planes:
+            names = [n for k, n in enumerate(hier.PLANES) if (planes >> k) & 1]
+            per = hier.gather(planes, ghosted=False)
+            for q, name in enumerate(names):
+                out[name] = [v[q] for v in per]
+        return out
     out = {}
     states = hier.states()
     g = [p.g for p in hier.h.patches]

@@ -374,6 +374,18 @@ int enl_batch_set_coarsen_rects(void *handle, int n, const enl_rect_t *rects);
 int enl_batch_transfer_rects(void *new_handle, void *old_handle, const int *reflect, int nsame, const int *s_new,
                              const int *s_old, int nrect, const enl_rect_t *rects);
 int enl_batch_destroy(void *handle);
+/* The SOLN patch data of every patch in one buffer, patch after patch in
+ * pid order: per patch the planes selected by `planes` (bit k < 5: w_n
+ * component k; bit 5: pressure; bit 6: temperature; bit 7: source), in that
+ * order, each over the interior (ghosted = 0, nx*ny values row by row: the
+ * VisIt plot quantities registerModelVariables registers,
+ * enlightning.C:295-378) or over the ghosted box (ghosted = 1: a
+ * checkpoint's patch data).  One launch; `out` is host or device memory of
+ * exactly `count` doubles (checked). */
+int enl_batch_gather(void *handle, int planes, int ghosted, double *out, long count);
+/* The inverse of enl_batch_gather over the ghosted boxes: every selected
+ * plane of every patch from `in` (a resumed checkpoint's patch data). */
+int enl_batch_scatter(void *handle, int planes, const double *in, long count);
 /* Device memory that destroyed batches gave back stays cached for the next
  * batch of a similar size (a regrid replaces the hierarchy's pool every few
  * steps); this returns the current device's cached blocks to the runtime. */
