@@ -57,13 +57,16 @@ struct Batch {
   int level_tile_begin[9] = {0};  // tiles sorted by level
   // How a level's tiles are packed into the workgroups of its stage launch
   // (ENL_PACK_*): one tile per workgroup, or a persistent grid of
-  // pack_wpc[level] workgroups per CU each working through a pack of tiles.
+  // pack_wpc[level] waves per SIMD, each workgroup working through a pack
+  // of tiles.
   // packs + level_pack_begin[level] holds the level's npack + 1 pack
   // offsets into its (reordered) tiles.
   int pack_mode[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int pack_wpc[8] = {4, 4, 4, 4, 4, 4, 4, 4};
   int ncu = 0;
   int* packs = nullptr;
+  void* tile_buf = nullptr;  // tiles, then packs, in one allocation
+  size_t tile_cap = 0;
   int level_pack_begin[9] = {0};
   int level_npack[8] = {0};
   bool tiles_dirty = true;  // tiles and packs are (re)built by ensure_tiles before the next launch
@@ -86,6 +89,7 @@ struct Batch {
   long level_cells[8] = {0};
   RhsArgs* d_args = nullptr;  // kMaxStages x npatch argument blocks
   std::vector<RhsArgs> h_args;
+  std::vector<unsigned char> args_key;  // the inputs h_args was built from
   std::vector<void*> allocations;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;

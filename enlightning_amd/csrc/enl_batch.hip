@@ -363,7 +363,7 @@ namespace {
 // that range:
 //   ENL_PACK_SINGLE   one tile per workgroup, in the range's order (the
 //                     dispatch order of xcd_tile);
-//   ENL_PACK_STRIDED  a persistent grid of wpc workgroups per CU, the j-th
+//   ENL_PACK_STRIDED  a persistent grid of wpc waves per SIMD, the j-th
 //                     workgroup of an XCD taking tiles j, j + P, j + 2P, ...
 //                     of its range (P workgroups on the XCD): neighbouring
 //                     tiles run side by side, as dispatched singly;
@@ -380,20 +380,25 @@ int make_tiles(Batch* b) {
       n = 256;
     b->ncu = n;
   }
-  std::vector<int4> t;
+  std::vector<int4> t, T;
   std::vector<int> packs;
+  std::vector<double> cost;
+  std::vector<int> cut;
+  t.reserve(b->ntiles > 0 ? (size_t)b->ntiles : 1024);
   for (int lev = 0; lev < kMaxLevels; lev++) {
     b->level_tile_begin[lev] = (int)t.size();
     b->level_pack_begin[lev] = (int)packs.size();
     const int bx = b->tile_bx[lev], by = b->tile_by[lev];
-    std::vector<int4> T;
-    std::vector<double> cost;
+    const int mode = b->pack_mode[lev];
+    T.clear();
+    cost.clear();
     for (int p = 0; p < b->npatch; p++)
       if (b->hp[p].level == lev)
         for (int j0 = 0; j0 < b->hp[p].ny; j0 += by)
           for (int i0 = 0; i0 < b->hp[p].nx; i0 += bx) {
             T.push_back(make_int4(p, i0, j0, 0));
-            cost.push_back(std::min(bx, b->hp[p].nx - i0) * std::min(by, b->hp[p].ny - j0) + 0.5 * bx * by);
+            if (mode == ENL_PACK_CHUNKED)
+              cost.push_back(std::min(bx, b->hp[p].nx - i0) * std::min(by, b->hp[p].ny - j0) + 0.5 * bx * by);
           }
     const int nt = (int)T.size();
     if (nt == 0) {
@@ -402,36 +407,48 @@ int make_tiles(Batch* b) {
     }
     const int q8 = nt / 8, r8 = nt % 8;
     auto range0 = [&](int x) { return x * q8 + std::min(x, r8); };  // XCD x: tiles [range0(x), range0(x + 1))
-    const int mode = b->pack_mode[lev];
-    const int G = mode == ENL_PACK_SINGLE ? nt : std::min(nt, std::max(8, b->ncu * b->pack_wpc[lev]));
-    std::vector<std::vector<int>> own((size_t)G);  // tile indices of each pack
-    for (int x = 0; x < 8; x++) {
-      const int a0 = range0(x), a1 = range0(x + 1);
-      std::vector<int> ws;  // the packs on XCD x
-      for (int w = x; w < G; w += 8) ws.push_back(w);
-      const int P = (int)ws.size();
-      if (a1 <= a0 || P == 0) continue;
-      if (mode == ENL_PACK_CHUNKED) {
-        double total = 0;
+    // persistent grid: pack_wpc[lev] waves of 64 lanes per SIMD, 4 SIMDs per
+    // CU, in workgroups of one tile's bx*by lanes
+    const long waves = 4L * b->ncu * b->pack_wpc[lev];
+    const int G = mode == ENL_PACK_SINGLE ? nt
+                                          : (int)std::min<long>(nt, std::max<long>(8, waves * 64 / (bx * by)));
+    auto npk = [&](int x) { return x < G ? (G - 1 - x) / 8 + 1 : 0; };  // packs w = x, x + 8, ... < G on XCD x
+    // chunked: per XCD the tile index where each of its packs starts
+    cut.assign((size_t)G + 8, 0);
+    std::vector<int> cut0(9, 0);
+    if (mode == ENL_PACK_CHUNKED) {
+      int o = 0;
+      for (int x = 0; x < 8; x++) {
+        cut0[(size_t)x] = o;
+        const int a0 = range0(x), a1 = range0(x + 1), P = npk(x);
+        double total = 0, acc = 0;
         for (int k = a0; k < a1; k++) total += cost[(size_t)k];
-        double acc = 0;
-        int j = 0;
+        int jj = 0;
+        cut[(size_t)o] = a0;
         for (int k = a0; k < a1; k++) {
           // the next pack starts when this one has its share of the cost
-          while (j + 1 < P && acc >= total * (j + 1) / P) j++;
-          own[(size_t)ws[(size_t)j]].push_back(k);
+          while (jj + 1 < P && acc >= total * (jj + 1) / P) cut[(size_t)(o + ++jj)] = k;
           acc += cost[(size_t)k];
         }
-      } else {
-        for (int k = a0; k < a1; k++) own[(size_t)ws[(size_t)((k - a0) % P)]].push_back(k);
+        while (jj + 1 < P) cut[(size_t)(o + ++jj)] = a1;
+        cut[(size_t)(o + P)] = a1;
+        o += P + 1;
       }
     }
     const int base = (int)t.size();
     int npack = 0;
     for (int w = 0; w < G; w++) {
-      if (own[(size_t)w].empty()) continue;
-      packs.push_back((int)t.size() - base);
-      for (int k : own[(size_t)w]) t.push_back(T[(size_t)k]);
+      const int x = w & 7, jw = w >> 3, P = npk(x), a0 = range0(x), a1 = range0(x + 1);
+      const int start = (int)t.size();
+      if (mode == ENL_PACK_CHUNKED) {
+        const int c0 = cut[(size_t)(cut0[(size_t)x] + jw)], c1 = cut[(size_t)(cut0[(size_t)x] + jw + 1)];
+        for (int k = c0; k < c1; k++) t.push_back(T[(size_t)k]);
+      } else {
+        // single (P = the XCD's tile count: one tile per pack) or strided
+        for (int k = a0 + jw; k < a1; k += P) t.push_back(T[(size_t)k]);
+      }
+      if ((int)t.size() == start) continue;
+      packs.push_back(start - base);
       npack++;
     }
     packs.push_back((int)t.size() - base);
@@ -439,13 +456,19 @@ int make_tiles(Batch* b) {
   }
   b->level_tile_begin[kMaxLevels] = (int)t.size();
   b->level_pack_begin[kMaxLevels] = (int)packs.size();
-  release(b, b->tiles);
-  release(b, b->packs);
-  b->tiles = nullptr;
-  b->packs = nullptr;
+  // tiles and pack offsets in one allocation and one upload, after queued
+  // launches that read the previous ones
+  (void)hipDeviceSynchronize();
+  const size_t tb = sizeof(int4) * t.size(), pb = sizeof(int) * packs.size();
+  int rc = reserve(b, &b->tile_buf, &b->tile_cap, tb + pb);
+  if (rc) return rc;
+  b->tiles = (int4*)b->tile_buf;
+  b->packs = (int*)((char*)b->tile_buf + tb);
   b->ntiles = (int)t.size();
-  int rc = upload(b, &b->tiles, t.data(), (long)t.size());
-  rc = rc ? rc : upload(b, &b->packs, packs.data(), (long)packs.size());
+  std::vector<char> h(tb + pb);
+  memcpy(h.data(), t.data(), tb);
+  memcpy(h.data() + tb, packs.data(), pb);
+  rc = check_hip(hipMemcpy(b->tile_buf, h.data(), tb + pb, hipMemcpyHostToDevice), "upload tiles");
   if (!rc) b->tiles_dirty = false;
   return rc;
 }
@@ -514,6 +537,23 @@ void patch_args(Batch* b, int p, int in, int out, int ww, int slot, RhsArgs& a) 
 // change; the kernels of stage st read b->d_args + st*npatch.
 int prepare_args(Batch* b, const enl_gas_t* gas, int order, const double* a1, const double* a2, const double* be,
                  double dt, hipStream_t s) {
+  // the table depends on the batch's fixed layout and on these inputs only
+  std::vector<unsigned char> key(sizeof *gas + sizeof order + sizeof dt + 3 * sizeof(double) * (size_t)order);
+  {
+    unsigned char* k = key.data();
+    memcpy(k, gas, sizeof *gas);
+    k += sizeof *gas;
+    memcpy(k, &order, sizeof order);
+    k += sizeof order;
+    memcpy(k, &dt, sizeof dt);
+    k += sizeof dt;
+    for (const double* v : {a1, a2, be}) {
+      memcpy(k, v, sizeof(double) * (size_t)order);
+      k += sizeof(double) * (size_t)order;
+    }
+  }
+  if (!b->h_args.empty() && key == b->args_key) return 0;
+  b->args_key = key;
   std::vector<RhsArgs> t((size_t)order * b->npatch, base_args(b, gas));
   for (int st = 0; st < order; st++) {
     for (int p = 0; p < b->npatch; p++) {
@@ -997,7 +1037,7 @@ int enl_batch_set_level_packing(void* handle, int level, int mode, int wpc) {
   if (!b) return set_error(-2, "null batch");
   if (level < 0 || level >= kMaxLevels) return set_error(-2, "bad level %d", level);
   if (mode < ENL_PACK_SINGLE || mode > ENL_PACK_CHUNKED || wpc < 1 || wpc > 32)
-    return set_error(-2, "enl_batch_set_level_packing: bad mode %d or workgroups per CU %d", mode, wpc);
+    return set_error(-2, "enl_batch_set_level_packing: bad mode %d or waves per SIMD %d", mode, wpc);
   drop_graph(b);
   b->pack_mode[level] = mode;
   b->pack_wpc[level] = wpc;

@@ -18,6 +18,7 @@
 // (i - ifirst0 + g) + Nx * (j - ifirst1 + g) inside the patch's ghosted box.
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <cstring>
 #include <functional>
@@ -69,7 +70,10 @@ inline int local(const Box& b, int g, int i, int j) { return (i - b.i0 + g) + (b
 
 // A pool of host threads kept for the process (created on first use, never
 // joined: a regrid runs several parallel loops of a millisecond or less, and
-// starting threads for each cost as much as the loop).
+// starting threads for each cost as much as the loop).  A worker that has
+// finished a loop polls for the next one for 200 us before it sleeps, since
+// a regrid's loops follow each other closely and a wake-up from sleep can
+// take as long as the loop.
 class Pool {
  public:
   static Pool& get() {
@@ -79,17 +83,18 @@ class Pool {
   int size() const { return (int)workers_.size() + 1; }
   // job(k) for k in [0, n), items handed out one at a time; the caller works too
   void run(int n, int want, const std::function<void(int)>& job) {
-    std::unique_lock<std::mutex> lk(m_);
-    job_ = &job;
-    n_ = n;
-    next_.store(0);
-    active_ = std::min(want - 1, (int)workers_.size());
-    pending_ = active_;
-    gen_++;
-    lk.unlock();
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      job_ = &job;
+      n_ = n;
+      next_.store(0);
+      active_ = std::min(want - 1, (int)workers_.size());
+      pending_ = active_;
+      gen_.fetch_add(1, std::memory_order_release);
+    }
     cv_.notify_all();
     drain();
-    lk.lock();
+    std::unique_lock<std::mutex> lk(m_);
     done_.wait(lk, [&] { return pending_ == 0; });
     job_ = nullptr;
   }
@@ -107,9 +112,13 @@ class Pool {
     in_pool() = true;
     unsigned long seen = 0;
     for (;;) {
+      const auto t0 = std::chrono::steady_clock::now();
+      while (gen_.load(std::memory_order_acquire) == seen &&
+             std::chrono::steady_clock::now() - t0 < std::chrono::microseconds(200))
+        __builtin_ia32_pause();
       std::unique_lock<std::mutex> lk(m_);
-      cv_.wait(lk, [&] { return gen_ != seen; });
-      seen = gen_;
+      cv_.wait(lk, [&] { return gen_.load(std::memory_order_acquire) != seen; });
+      seen = gen_.load(std::memory_order_acquire);
       if (id >= active_) continue;
       lk.unlock();
       drain();
@@ -130,7 +139,7 @@ class Pool {
   std::condition_variable cv_, done_;
   const std::function<void(int)>* job_ = nullptr;
   int n_ = 0, active_ = 0, pending_ = 0;
-  unsigned long gen_ = 0;
+  std::atomic<unsigned long> gen_{0};
   std::atomic<int> next_{0};
 };
 
@@ -173,6 +182,36 @@ std::vector<int> cuts(int n0, int tile0, int g) {
 
 // The patch of a level holding each cell, at the resolution of the level
 // below when every box of the level is aligned to the ratio (s = ratio).
+// Owner maps of a few MB are built at every regrid; their buffers are kept
+// for the next plan instead of being returned to the OS and faulted in again.
+struct MapPool {
+  std::mutex mu;
+  std::vector<std::vector<int>> idle;
+};
+MapPool& map_pool() {
+  static MapPool* p = new MapPool();  // never destroyed: outlives static teardown
+  return *p;
+}
+std::vector<int> take_map(size_t n) {
+  MapPool& p = map_pool();
+  {
+    std::lock_guard<std::mutex> lk(p.mu);
+    for (size_t k = 0; k < p.idle.size(); k++)
+      if (p.idle[k].size() == n) {
+        std::vector<int> v = std::move(p.idle[k]);
+        p.idle.erase(p.idle.begin() + (long)k);
+        return v;
+      }
+  }
+  return std::vector<int>(n);
+}
+void give_map(std::vector<int>&& v) {
+  if (v.size() < ((size_t)1 << 16)) return;
+  MapPool& p = map_pool();
+  std::lock_guard<std::mutex> lk(p.mu);
+  if (p.idle.size() < 4) p.idle.push_back(std::move(v));
+}
+
 struct Owner {
   std::vector<int> map;
   long n = 0, m = 0;  // level cells a side, map cells a side (n / s)
@@ -180,6 +219,13 @@ struct Owner {
   int at(long i, long j) const {
     if (sh >= 0) return map[(size_t)((j >> sh) * m + (i >> sh))];
     return map[(size_t)((j / s) * m + i / s)];
+  }
+  Owner() = default;
+  Owner(Owner&&) = default;
+  Owner& operator=(Owner&&) = default;
+  Owner(const Owner&) = delete;
+  ~Owner() {
+    if (!map.empty()) give_map(std::move(map));
   }
 };
 
@@ -201,7 +247,8 @@ Owner build_owner(const Plan& P, int level) {
   o.sh = log2_exact(o.s);
   const long m = o.n / o.s;
   o.m = m;
-  o.map.assign((size_t)(m * m), -1);
+  o.map = take_map((size_t)(m * m));
+  parallel_for((int)m, [&](int j) { std::fill(o.map.begin() + (long)j * m, o.map.begin() + (long)(j + 1) * m, -1); });
   parallel_for((int)P.boxes.size(), [&](int pid) {
     const Box& b = P.boxes[(size_t)pid];
     if (b.level != level) return;

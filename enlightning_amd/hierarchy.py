@@ -168,7 +168,7 @@ class DeviceHierarchy:
         self.h, self.gas, self.reflect = h, gas, tuple(reflect)
         self.absorb_width = 0.0
         self.tiles = {}  # level -> (bx, by) set by set_tiles / autotune; None: every level
-        self.packing = {}  # level -> (mode, workgroups per CU) set by set_packing / autotune
+        self.packing = {}  # level -> (mode, waves per SIMD) set by set_packing / autotune
         self.lib = _lib.load()
         n = len(h.patches)
         # the patch table (enl_patch_desc_t rows) built with numpy
@@ -221,18 +221,20 @@ class DeviceHierarchy:
         self.absorb_width = float(width)
 
     # how a level's tiles are packed into the stage launch's workgroups
-    # (enl_batch_set_level_packing): (mode, workgroups per CU)
-    PACKINGS = ((_lib.PACK_SINGLE, 0), (_lib.PACK_STRIDED, 2), (_lib.PACK_STRIDED, 4), (_lib.PACK_CHUNKED, 2),
-                (_lib.PACK_CHUNKED, 4))
+    # (enl_batch_set_level_packing): (mode, waves per SIMD of the persistent grid)
+    PACKINGS = ((_lib.PACK_SINGLE, 0), (_lib.PACK_STRIDED, 4), (_lib.PACK_STRIDED, 8), (_lib.PACK_CHUNKED, 4),
+                (_lib.PACK_CHUNKED, 8))
     PACK_NAMES = {_lib.PACK_SINGLE: "single", _lib.PACK_STRIDED: "strided", _lib.PACK_CHUNKED: "chunked"}
 
     def set_packing(self, level, mode, wpc=4):
+        """Packing of a level's tiles into workgroups (ENL_PACK_*; wpc: waves
+        per SIMD of the persistent grid)."""
         _lib.check(self.lib.enl_batch_set_level_packing(self.handle, level, int(mode), max(1, int(wpc))))
         self.packing[level] = (int(mode), int(wpc))
 
     def packing_name(self, level):
         mode, wpc = self.packing.get(level, (_lib.PACK_SINGLE, 0))
-        return self.PACK_NAMES[mode] + ("" if mode == _lib.PACK_SINGLE else "/%d per CU" % wpc)
+        return self.PACK_NAMES[mode] + ("" if mode == _lib.PACK_SINGLE else "/%d waves per SIMD" % wpc)
 
     def set_tiles(self, bx, by, level=None):
         if level is None:
@@ -248,7 +250,7 @@ class DeviceHierarchy:
         turn tries every tile shape with the others fixed and keeps the
         fastest, then every packing of its tiles into workgroups (one tile
         per workgroup, or persistent packs: strided or cost-balanced chunks
-        at 2 or 4 workgroups per CU) at that shape.  Reloads `states` before
+        at 4 or 8 waves per SIMD) at that shape.  Reloads `states` before
         each trial; returns {level: (bx, by)} (the packings are in
         self.packing) and the per-trial times (ms per step)."""
         import time

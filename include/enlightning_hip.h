@@ -274,9 +274,10 @@ int enl_batch_set_tiles(void *handle, int bx, int by);
 int enl_batch_set_level_tiles(void *handle, int level, int bx, int by);
 /* How the stage kernel of one level packs its tiles into workgroups:
  * ENL_PACK_SINGLE, one tile per workgroup; ENL_PACK_STRIDED, a persistent
- * grid of wpc workgroups per CU, each taking every P-th tile of its XCD's
- * range; ENL_PACK_CHUNKED, the same grid, each workgroup one contiguous run
- * of tiles of equal estimated cost.  Results do not depend on it. */
+ * grid of wpc waves per SIMD (one workgroup = one tile's lanes), each
+ * workgroup taking every P-th tile of its XCD's range; ENL_PACK_CHUNKED,
+ * the same grid, each workgroup one contiguous run of tiles of equal
+ * estimated cost.  Results do not depend on it. */
 enum { ENL_PACK_SINGLE = 0, ENL_PACK_STRIDED = 1, ENL_PACK_CHUNKED = 2 };
 int enl_batch_set_level_packing(void *handle, int level, int mode, int wpc);
 /* Device pointers of a patch's current state and ghosted source. */

@@ -66,7 +66,7 @@ def same(sa, sb, h):
 
 
 # packings of a level's tiles into workgroups (ENL_PACK_*: single, strided,
-# chunked) with workgroups per CU
+# chunked) with the persistent grid's waves per SIMD
 PACKS = ((0, 0), (1, 1), (1, 4), (2, 1), (2, 4))
 
 
