@@ -209,17 +209,49 @@ __device__ inline bool pair_smooth(T t1, T t2, T thr) {
 }
 
 // Prefilter: DRP where the sensor provably says DRP, else a candidate.  A
-// 32 x 8 tile of cells with its 2-cell halo of rho is staged in LDS (every
-// load of a thread issued before the first store); the candidates are
+// 32 x 32 tile of cells with its 2-cell halo of rho is staged in LDS (every
+// load of a thread issued before the first store); each of the 256 threads
+// tests the cells of its column in four rows 8 apart, so a workgroup's
+// staging, barriers and atomic serve 1024 cells; the candidates are
 // compacted with one atomic per workgroup.
-constexpr int kPBX = 32, kPBY = 8, kPW = kPBX + 4, kPH = kPBY + 4;
+constexpr int kPBX = 32, kPBY = 8, kPRows = 4, kPTH = kPBY * kPRows, kPW = kPBX + 4, kPH = kPTH + 4;
 constexpr int kPWaves = kPBX * kPBY / 64;
+
+// Positions of a workgroup's candidates: per wave and row r the ballot m[r];
+// a wave's candidates in row order, then lane order.
+struct PrefilterCompact {
+  unsigned wcount[kPWaves], wbase[kPWaves];
+};
+__device__ inline unsigned compact_slot(PrefilterCompact& pc, const unsigned long long (&m)[kPRows], unsigned* ncand,
+                                        int tid) {
+  const int wave = tid >> 6;
+  if ((tid & 63) == 0) {
+    unsigned c = 0;
+#pragma unroll
+    for (int r = 0; r < kPRows; r++) c += (unsigned)__popcll(m[r]);
+    pc.wcount[wave] = c;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    unsigned tot = 0;
+#pragma unroll
+    for (int w = 0; w < kPWaves; w++) tot += pc.wcount[w];
+    unsigned b = tot ? atomicAdd(ncand, tot) : 0u;
+#pragma unroll
+    for (int w = 0; w < kPWaves; w++) {
+      pc.wbase[w] = b;
+      b += pc.wcount[w];
+    }
+  }
+  __syncthreads();
+  return pc.wbase[wave];
+}
 
 __global__ void __launch_bounds__(kPBX * kPBY) prefilter_kernel(DecideArgs a) {
   __shared__ double rt[kPH * kPW];
-  __shared__ unsigned wcount[kPWaves], wbase[kPWaves];
+  __shared__ PrefilterCompact pc;
   const int nbx = (a.nx + kPBX - 1) / kPBX;
-  const int i0 = (blockIdx.x % nbx) * kPBX, j0 = (blockIdx.x / nbx) * kPBY;
+  const int i0 = (blockIdx.x % nbx) * kPBX, j0 = (blockIdx.x / nbx) * kPTH;
   const int tid = threadIdx.x;
   constexpr int kIt = (kPH * kPW + kPBX * kPBY - 1) / (kPBX * kPBY);
   double v[kIt];
@@ -235,43 +267,38 @@ __global__ void __launch_bounds__(kPBX * kPBY) prefilter_kernel(DecideArgs a) {
   for (int it = 0; it < kIt; it++)
     if (in[it]) rt[tid + it * kPBX * kPBY] = v[it];
   __syncthreads();
-  const int i = i0 + tid % kPBX, j = j0 + tid / kPBX;
-  bool cand = false;
-  long c = 0;
-  if (i < a.nx && j < a.ny) {
-    c = (long)i + (long)j * a.nx;
-    const int e = (tid / kPBX + 2) * kPW + tid % kPBX + 2;
+  const int tx = tid % kPBX, ty = tid / kPBX;
+  bool cand[kPRows];
+  unsigned long long m[kPRows];
 #pragma unroll
-    for (int q = -1; q <= 1; q++) {
-      double t1 = rt[e + q] - rt[e - 1 + q], t2 = rt[e + 1 + q] - rt[e + q];
-      if (!pair_smooth(t1, t2, a.thr)) cand = true;
-      t1 = rt[e + q * kPW] - rt[e + (q - 1) * kPW];
-      t2 = rt[e + (q + 1) * kPW] - rt[e + q * kPW];
-      if (!pair_smooth(t1, t2, a.thr)) cand = true;
+  for (int r = 0; r < kPRows; r++) {
+    const int i = i0 + tx, j = j0 + ty + kPBY * r;
+    cand[r] = false;
+    if (i < a.nx && j < a.ny) {
+      const int e = (ty + kPBY * r + 2) * kPW + tx + 2;
+#pragma unroll
+      for (int q = -1; q <= 1; q++) {
+        double t1 = rt[e + q] - rt[e - 1 + q], t2 = rt[e + 1 + q] - rt[e + q];
+        if (!pair_smooth(t1, t2, a.thr)) cand[r] = true;
+        t1 = rt[e + q * kPW] - rt[e + (q - 1) * kPW];
+        t2 = rt[e + (q + 1) * kPW] - rt[e + q * kPW];
+        if (!pair_smooth(t1, t2, a.thr)) cand[r] = true;
+      }
+      if (!cand[r]) a.decide[(long)i + (long)j * a.nx] = 0;
     }
-    if (!cand) a.decide[c] = 0;
+    m[r] = __ballot(cand[r]);
   }
-  const unsigned long long m = __ballot(cand);
-  const int wave = tid >> 6;
-  if ((tid & 63) == 0) wcount[wave] = (unsigned)__popcll(m);
-  __syncthreads();
-  if (tid == 0) {
-    unsigned tot = 0;
+  unsigned slot = compact_slot(pc, m, a.ncand, tid);
 #pragma unroll
-    for (int w = 0; w < kPWaves; w++) tot += wcount[w];
-    unsigned b = tot ? atomicAdd(a.ncand, tot) : 0u;
-#pragma unroll
-    for (int w = 0; w < kPWaves; w++) {
-      wbase[w] = b;
-      b += wcount[w];
-    }
+  for (int r = 0; r < kPRows; r++) {
+    if (cand[r])
+      a.cand[slot + __popcll(m[r] & __lanemask_lt())] = (unsigned)((long)(i0 + tx) + (long)(j0 + ty + kPBY * r) * a.nx);
+    slot += (unsigned)__popcll(m[r]);
   }
-  __syncthreads();
-  if (cand) a.cand[wbase[wave] + __popcll(m & __lanemask_lt())] = (unsigned)c;
 }
 
 // The same prefilter over the level-0 patches of an FP32-perturbation
-// hierarchy: blockIdx.y is the patch, blockIdx.x a 32 x 8 tile of it (tiles
+// hierarchy: blockIdx.y is the patch, blockIdx.x a 32 x 32 tile of it (tiles
 // past the patch's own count leave at once).  The pairs are formed as the
 // FP32 stage kernel forms them (x: differences of rho'; y: differences of
 // rho' plus the ambient's row difference), so a cell marked DRP here is one
@@ -289,13 +316,13 @@ __global__ void __launch_bounds__(kPBX * kPBY) prefilter32_kernel(const SwPatch*
                                                                  unsigned long long* cand, unsigned* ncand) {
   __shared__ float rt[kPH * kPW];
   __shared__ float dr[kPH];
-  __shared__ unsigned wcount[kPWaves], wbase[kPWaves];
+  __shared__ PrefilterCompact pc;
   const int pid = blockIdx.y;
   const SwPatch& P = pt[pid];
   const int nx = P.nx, ny = P.ny, g = P.g;
-  const int nbx = (nx + kPBX - 1) / kPBX, nby = (ny + kPBY - 1) / kPBY;
+  const int nbx = (nx + kPBX - 1) / kPBX, nby = (ny + kPTH - 1) / kPTH;
   if ((int)blockIdx.x >= nbx * nby) return;  // the same for the whole block
-  const int i0 = ((int)blockIdx.x % nbx) * kPBX, j0 = ((int)blockIdx.x / nbx) * kPBY;
+  const int i0 = ((int)blockIdx.x % nbx) * kPBX, j0 = ((int)blockIdx.x / nbx) * kPTH;
   const int tid = threadIdx.x;
   constexpr int kIt = (kPH * kPW + kPBX * kPBY - 1) / (kPBX * kPBY);
   float v[kIt];
@@ -316,48 +343,43 @@ __global__ void __launch_bounds__(kPBX * kPBY) prefilter32_kernel(const SwPatch*
   if (din) dr[tid] = dv;
   __syncthreads();
   const int tx = tid % kPBX, ty = tid / kPBX;
-  const int i = i0 + tx, j = j0 + ty;
-  bool c = false, amb_only = false;
-  if (i < nx && j < ny) {
-    const int e = (ty + 2) * kPW + tx + 2;
-    bool full = false, pert = false;
+  bool c[kPRows];
+  unsigned long long m[kPRows];
+  unsigned namb = 0;
 #pragma unroll
-    for (int q = -1; q <= 1; q++) {
-      float t1 = rt[e + q] - rt[e - 1 + q], t2 = rt[e + 1 + q] - rt[e + q];
-      if (!pair_smooth(t1, t2, thr)) full = pert = true;
-      const float p1 = rt[e + q * kPW] - rt[e + (q - 1) * kPW];
-      const float p2 = rt[e + (q + 1) * kPW] - rt[e + q * kPW];
-      if (!pair_smooth(p1 + dr[ty + q + 1], p2 + dr[ty + q + 2], thr)) full = true;
-      if (!pair_smooth(p1, p2, thr)) pert = true;
+  for (int r = 0; r < kPRows; r++) {
+    const int i = i0 + tx, j = j0 + ty + kPBY * r, tr = ty + kPBY * r;
+    c[r] = false;
+    bool amb_only = false;
+    if (i < nx && j < ny) {
+      const int e = (tr + 2) * kPW + tx + 2;
+      bool full = false, pert = false;
+#pragma unroll
+      for (int q = -1; q <= 1; q++) {
+        float t1 = rt[e + q] - rt[e - 1 + q], t2 = rt[e + 1 + q] - rt[e + q];
+        if (!pair_smooth(t1, t2, thr)) full = pert = true;
+        const float p1 = rt[e + q * kPW] - rt[e + (q - 1) * kPW];
+        const float p2 = rt[e + (q + 1) * kPW] - rt[e + q * kPW];
+        if (!pair_smooth(p1 + dr[tr + q + 1], p2 + dr[tr + q + 2], thr)) full = true;
+        if (!pair_smooth(p1, p2, thr)) pert = true;
+      }
+      const bool amb = P.amb_weno[(long)i + (long)j * nx] != 0;
+      c[r] = full && pert && !amb;
+      amb_only = full && !c[r];
+      if (!full || amb_only) P.decide[(long)i + (long)j * nx] = amb_only ? 2 : 0;
     }
-    const bool amb = P.amb_weno[(long)i + (long)j * nx] != 0;
-    c = full && pert && !amb;
-    amb_only = full && !c;
-    if (!full || amb_only) P.decide[(long)i + (long)j * nx] = amb_only ? 2 : 0;
+    m[r] = __ballot(c[r]);
+    namb += (unsigned)__popcll(__ballot(amb_only));
   }
-  const unsigned long long m = __ballot(c);
-  const unsigned long long ma = __ballot(amb_only);
-  const int wave = tid >> 6;
-  if ((tid & 63) == 0) {
-    wcount[wave] = (unsigned)__popcll(m);
-    if (ma) atomicAdd(ncand + 3, (unsigned)__popcll(ma));
-  }
-  __syncthreads();
-  if (tid == 0) {
-    unsigned tot = 0;
+  if ((tid & 63) == 0 && namb) atomicAdd(ncand + 3, namb);
+  unsigned slot = compact_slot(pc, m, ncand, tid);
 #pragma unroll
-    for (int w = 0; w < kPWaves; w++) tot += wcount[w];
-    unsigned b = tot ? atomicAdd(ncand, tot) : 0u;
-#pragma unroll
-    for (int w = 0; w < kPWaves; w++) {
-      wbase[w] = b;
-      b += wcount[w];
-    }
+  for (int r = 0; r < kPRows; r++) {
+    if (c[r])
+      cand[slot + __popcll(m[r] & __lanemask_lt())] =
+          ((unsigned long long)pid << 32) | (unsigned)((i0 + tx) + (j0 + ty + kPBY * r) * nx);
+    slot += (unsigned)__popcll(m[r]);
   }
-  __syncthreads();
-  if (c)
-    cand[wbase[wave] + __popcll(m & __lanemask_lt())] =
-        ((unsigned long long)pid << 32) | (unsigned)(i + j * nx);
 }
 
 // Inference on the matrix cores: each wave takes 16 candidates at a time
@@ -763,7 +785,7 @@ int switch_decide(Switch* sw, const Geom& G, const double* w_n, const double* pr
   src.cand = a.cand;
   src.decide = decide;
   hipLaunchKernelGGL(decide_reset_kernel, dim3(1), dim3(64), 0, s, sw->ncand);
-  const unsigned ntile = (unsigned)(((G.nx + kPBX - 1) / kPBX) * ((G.ny + kPBY - 1) / kPBY));
+  const unsigned ntile = (unsigned)(((G.nx + kPBX - 1) / kPBX) * ((G.ny + kPTH - 1) / kPTH));
   hipLaunchKernelGGL(prefilter_kernel, dim3(ntile), dim3(kPBX * kPBY), 0, s, a);
   hipLaunchKernelGGL(infer_kernel<PatchSrc>, dim3(1024), dim3(64 * kWaves), 0, s, src, sw->ncand, lo, hi, sw->wb);
   return check_hip(hipGetLastError(), "switch decisions");
@@ -778,7 +800,7 @@ int switch_decide_batch(Switch* sw, const SwPatch* patches, int npatch, int max_
   if (npatch <= 0) return 0;
   if (!sw->cand || max_side <= 0 || sw->cand_bytes < sizeof(unsigned long long) * (size_t)cells)
     return set_error(-2, "switch_decide_batch: candidate list not reserved for %ld cells", cells);
-  const int nt = ((max_side + kPBX - 1) / kPBX) * ((max_side + kPBY - 1) / kPBY);
+  const int nt = ((max_side + kPBX - 1) / kPBX) * ((max_side + kPTH - 1) / kPTH);
   Batch32Src src;
   src.pt = patches;
   src.cand = (const unsigned long long*)sw->cand;
