@@ -368,3 +368,28 @@ def test_microphones_over_the_hierarchy():
     assert [e[0] for e in got[1]].count(0) == 2
     assert got[2] == []
     d.close()
+
+
+def test_batches_after_the_memory_cache_is_trimmed():
+    """A batch built from cached device memory (blocks a destroyed batch gave
+    back) and one built after enl_batch_trim returned the cache to the
+    runtime step like a batch built from fresh memory."""
+    gas, h = make(WL128, 32)
+    dt = dt_of(h, gas)
+    out = []
+    for trim in (False, False, True):
+        d = D.Hierarchy(h, gas)
+        d.load_initial(WL128)
+        d.advance(2, dt, mode="graph")
+        out.append(d.states())
+        d.close()
+        if trim and not D.DRY:
+            from enlightning_amd import _lib
+            _lib.check(_lib.load().enl_batch_trim())
+    d = D.Hierarchy(h, gas)
+    d.load_initial(WL128)
+    d.advance(2, dt, mode="graph")
+    out.append(d.states())
+    d.close()
+    for st in out[1:]:
+        assert same(out[0], st, h)
