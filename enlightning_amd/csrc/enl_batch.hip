@@ -96,14 +96,14 @@ batch_split_kernel(const int4* __restrict__ tiles, const int* __restrict__ packs
   ENL_BATCH_LDS(BX, BY, L, false)
   if constexpr (!PACKED) {
     const int4 tl = tiles[blockIdx.x];
-    split_body<V, P, true, BX, BY>(args[tl.x], tl.y, tl.z, (unsigned)tl.x, L);
+    split_body<V, P, kStage, BX, BY>(args[tl.x], tl.y, tl.z, (unsigned)tl.x, L);
   } else {
     const int q1 = packs[blockIdx.x + 1];
 #pragma clang loop unroll(disable)
     for (int q = packs[blockIdx.x]; q < q1; q++) {
       const int4 tl = tiles[q];
       TileLds<BX, BY> Lq = rebased(L);
-      split_body<V, P, true, BX, BY>(args[tl.x], tl.y, tl.z, (unsigned)tl.x, Lq);
+      split_body<V, P, kStage, BX, BY>(args[tl.x], tl.y, tl.z, (unsigned)tl.x, Lq);
       __syncthreads();
     }
   }
@@ -118,7 +118,7 @@ batch_dense_kernel(const int4* __restrict__ tiles, const int* __restrict__ packs
   __shared__ double lds_x[10 * BX * BY];
   if constexpr (!PACKED) {
     const int4 tl = tiles[blockIdx.x];
-    dense2_body<V, P, true, BX, BY>(args[tl.x], tl.y, tl.z, L, lds_x);
+    dense2_body<V, P, kStage, BX, BY>(args[tl.x], tl.y, tl.z, L, lds_x);
   } else {
     const int q1 = packs[blockIdx.x + 1];
 #pragma clang loop unroll(disable)
@@ -127,7 +127,7 @@ batch_dense_kernel(const int4* __restrict__ tiles, const int* __restrict__ packs
       TileLds<BX, BY> Lq = rebased(L);
       int z;
       asm volatile("s_mov_b32 %0, 0" : "=s"(z));
-      dense2_body<V, P, true, BX, BY>(args[tl.x], tl.y, tl.z, Lq, lds_x + z);
+      dense2_body<V, P, kStage, BX, BY>(args[tl.x], tl.y, tl.z, Lq, lds_x + z);
       __syncthreads();
     }
   }
@@ -138,7 +138,7 @@ template <int V, bool P>
 __global__ void __launch_bounds__(256) batch_list_kernel(const RhsArgs* __restrict__ args,
                                                          const unsigned long long* __restrict__ list,
                                                          const unsigned* __restrict__ count) {
-  list_block<V, P, true>([=](int pid) -> const RhsArgs& { return args[pid]; }, list, *count);
+  list_block<V, P, kStage>([=](int pid) -> const RhsArgs& { return args[pid]; }, list, *count);
 }
 
 // Zeroes the WENO-list counters of every (stage, level) of a step.

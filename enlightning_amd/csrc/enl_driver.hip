@@ -359,6 +359,15 @@ int advance_fused(const LearnedSwitch& L, int variant, int ifirst0, int ilast0, 
   const double* dtab = nullptr;
   bool absorb_on = false;
   if (int rc = whole_domain_absorb(G, xlo, dx, reflect, absorb_width, &plan, &dtab, &absorb_on, s)) return rc;
+  // Without the sponge every stage's output is final where the boundary
+  // reads it, so the stage kernels write the next stage's physical ghosts
+  // themselves and only the first stage's input (and the source) needs the
+  // boundary launch; the sponge changes the output after the stage, so
+  // with it the boundary runs before every stage.
+  if (!absorb_on) {
+    ra.bc_fold = 1;
+    for (int k = 0; k < 4; k++) ra.bc_fold |= (reflect[k] ? 1 : 0) << (1 + k);
+  }
   hipError_t e = hipSuccess;
   long t;
   for (int step = 0; step < nsteps && e == hipSuccess; step++) {
@@ -367,14 +376,16 @@ int advance_fused(const LearnedSwitch& L, int variant, int ifirst0, int ilast0, 
       // the last stage writes A (in is B or C then), the others alternate;
       // a one-stage scheme goes through B and is copied back
       State out = (stage == order - 1 && order > 1) ? A : ((stage % 2 == 0) ? B : C);
-      bc.w = in.w;
-      bc.f[0] = in.p;
-      bc.f[1] = in.T;
-      t = g_timer.begin(ENL_KERNEL_BC, s);
-      e = launch_physical_bc(bc, s);
-      g_timer.end(t, s);
-      bc.f[2] = nullptr;  // the source is constant: its ghosts are filled once
-      if (e != hipSuccess) break;
+      if (!ra.bc_fold || (step == 0 && stage == 0)) {
+        bc.w = in.w;
+        bc.f[0] = in.p;
+        bc.f[1] = in.T;
+        t = g_timer.begin(ENL_KERNEL_BC, s);
+        e = launch_physical_bc(bc, s);
+        g_timer.end(t, s);
+        bc.f[2] = nullptr;  // the source is constant: its ghosts are filled once
+        if (e != hipSuccess) break;
+      }
       ra.w = in.w;
       ra.p = in.p;
       ra.T = in.T;

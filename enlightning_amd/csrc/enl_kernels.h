@@ -71,6 +71,11 @@ struct RhsArgs {
   // per-run operands of the source / update_state quotients admit div_nr
   double eos_k1;
   bool fast_div, eos_ok;
+  // fused stage on a patch whose four sides are physical: nonzero makes the
+  // cells next to a side also write their new state into that side's ghost
+  // layers, as launch_physical_bc would before the next stage; bit 1 + k set
+  // when side k (left, right, bottom, top) reflects
+  int bc_fold = 0;
 };
 
 struct RkArgs {

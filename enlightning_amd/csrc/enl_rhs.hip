@@ -65,7 +65,7 @@ using Sh = TileShape<kBX, kBY>;
   const int2 t_ = strip_tile((int)blockIdx.x, nbx_, nby_);              \
   const int i0 = t_.x * kBX, j0 = t_.y * kBY;
 
-template <int V, bool P, bool FUSED>
+template <int V, bool P, int FUSED>
 __global__ void __launch_bounds__(Sh::threads) __attribute__((amdgpu_waves_per_eu(5))) rhs_split_kernel(RhsArgs a) {
   ENL_TILE_LDS(L, false)
   ENL_TILE_ORIGIN(a, i0, j0)
@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(Sh::threads) __attribute__((amdgpu_waves_per_e
 // split kernels are capped at 5 waves per SIMD (<= 102 VGPRs), what the LDS
 // of a tile allows anyway (5 tiles of 29.8 KB per CU); the cap also gives
 // the better schedule (0.74 vs 0.82 ms per fused 4096^2 stage).
-template <int V, bool P, bool FUSED>
+template <int V, bool P, int FUSED>
 __global__ void __launch_bounds__(Sh::threads) __attribute__((amdgpu_waves_per_eu(5)))
 rhs_split_fused_kernel(RhsArgs a) {
   ENL_TILE_LDS(L, false)
@@ -84,14 +84,14 @@ rhs_split_fused_kernel(RhsArgs a) {
 }
 
 // WENO cells of the list, ten lanes each (list_block).
-template <int V, bool P, bool FUSED>
+template <int V, bool P, int FUSED>
 __global__ void __launch_bounds__(256) rhs_weno_list_kernel(RhsArgs a) {
-  list_block<V, P, FUSED>([&](int) -> const RhsArgs& { return a; }, a.list, *a.count);
+  list_block<V, P, FUSED>([&](int) -> const RhsArgs& { return a; }, a.list, *a.count, a.bc_fold != 0);
 }
 
 // Every cell WENO: the two directions on two thread planes (dense2_body),
 // block (kBX, kBY, 2).
-template <int V, bool P, bool FUSED>
+template <int V, bool P, int FUSED>
 __global__ void __launch_bounds__(2 * Sh::threads) __attribute__((amdgpu_waves_per_eu(4))) rhs_dense_kernel(RhsArgs a) {
   ENL_TILE_LDS(L, true)
   __shared__ double lds_x[10 * Sh::threads];
@@ -107,7 +107,7 @@ bool is_pow2(double x) {
   return std::frexp(x, &e) == 0.5;
 }
 
-template <int V, bool P, bool F>
+template <int V, bool P, int F>
 hipError_t launch_t(const RhsArgs& a, hipStream_t s) {
   using Sc = Scheme<V>;
   const Geom& G = a.geom;
@@ -141,13 +141,13 @@ hipError_t launch_t(const RhsArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int V, bool F>
+template <int V, int F>
 hipError_t launch_v(const RhsArgs& a, hipStream_t s) {
   if (is_pow2(a.dx) && is_pow2(a.dy)) return launch_t<V, true, F>(a, s);
   return launch_t<V, false, F>(a, s);
 }
 
-template <bool F>
+template <int F>
 hipError_t launch_any(int variant, const RhsArgs& a0, hipStream_t s) {
   if (a0.geom.nx <= 0 || a0.geom.ny <= 0) return hipSuccess;
   RhsArgs a = a0;
@@ -172,11 +172,11 @@ bool rhs_needs_list(int variant, int level) {
   return variant <= kHybridLow && level == 0;
 }
 
-hipError_t launch_rhs(int variant, const RhsArgs& a, hipStream_t s) { return launch_any<false>(variant, a, s); }
+hipError_t launch_rhs(int variant, const RhsArgs& a, hipStream_t s) { return launch_any<kRhs>(variant, a, s); }
 
 hipError_t launch_stage(int variant, const RhsArgs& a, hipStream_t s) {
   if (!a.ww || !a.w_out || !a.p_out || !a.T_out || a.w_out == a.w) return hipErrorInvalidValue;
-  return launch_any<true>(variant, a, s);
+  return launch_any<kStageBc>(variant, a, s);
 }
 
 }  // namespace enl

@@ -124,6 +124,13 @@ __device__ inline double tflux_g(const Tile& t, int e, int k) {
   return flux_g(t.w0[e], t.u[e], t.v[e], t.s[e], t.tn[e], t.p[e], k);
 }
 
+// What a cell's rhs becomes (the FUSED parameter of the kernel bodies):
+// kRhs stores it; kStage applies the runge_kutta update and update_state and
+// stores the new state; kStageBc does that and, when the arguments' bc_fold
+// asks for it, also writes the physical boundary ghosts (the single-patch
+// stage: the batched kernels' patches fill ghosts by schedules).
+constexpr int kRhs = 0, kStage = 1, kStageBc = 2;
+
 // Result of one cell: store rhs (plain), or apply the runge_kutta update
 // and update_state and store the new state (fused stage).
 // The fused stage's operands of the runge_kutta update at cell (i, j):
@@ -138,7 +145,42 @@ __device__ inline void rk_operands(const RhsArgs& a, int i, int j, double wa[5],
   }
 }
 
-template <bool FUSED>
+// The physical boundary fold of the fused stage (a.bc_fold nonzero): the
+// ghost cells whose boundary copy comes from interior cell (i, j), as
+// bc_edge_kernel and bc_corner_kernel fill them (enlightning.C:836-1008):
+// every layer of an edge strip is the nearest interior cell, with u negated
+// on a reflecting left/right side and v on a reflecting bottom/top side;
+// each corner block is the corner cell with both applicable negations.
+// put(d, neg_u, neg_v) receives the ghost's offset in the ghosted arrays.
+template <class Put>
+__device__ inline void bc_fold_ghosts(const RhsArgs& a, int i, int j, Put put) {
+  const Geom& G = a.geom;
+  const int f = a.bc_fold;
+  const bool lo0 = i == 0, hi0 = i == G.nx - 1, lo1 = j == 0, hi1 = j == G.ny - 1;
+  if (!(lo0 || hi0 || lo1 || hi1)) return;
+  const bool ru_lo = f & 2, ru_hi = f & 4, rv_lo = f & 8, rv_hi = f & 16;
+  const long row = (long)(j + G.g1) * G.sy, col = i + G.g0;
+  if (lo0)
+    for (int l = 0; l < G.g0; l++) put(row + G.g0 - 1 - l, ru_lo, false);
+  if (hi0)
+    for (int l = 0; l < G.g0; l++) put(row + G.g0 + G.nx + l, ru_hi, false);
+  if (lo1)
+    for (int l = 0; l < G.g1; l++) put((long)(G.g1 - 1 - l) * G.sy + col, false, rv_lo);
+  if (hi1)
+    for (int l = 0; l < G.g1; l++) put((long)(G.g1 + G.ny + l) * G.sy + col, false, rv_hi);
+  for (int cx = 0; cx < 2; cx++)
+    for (int cy = 0; cy < 2; cy++) {
+      if (!(cx ? hi0 : lo0) || !(cy ? hi1 : lo1)) continue;
+      for (int lj = 0; lj < G.g1; lj++)
+        for (int li = 0; li < G.g0; li++) {
+          const int gi = cx ? G.g0 + G.nx + li : G.g0 - 1 - li;
+          const int gj = cy ? G.g1 + G.ny + lj : G.g1 - 1 - lj;
+          put((long)gi + (long)gj * G.sy, cx ? ru_hi : ru_lo, cy ? rv_hi : rv_lo);
+        }
+    }
+}
+
+template <int FUSED>
 __device__ inline void cell_output(const RhsArgs& a, int i, int j, const double K[5]) {
   const Geom& G = a.geom;
   const long c = (long)i + (long)j * G.nx;
@@ -163,6 +205,16 @@ __device__ inline void cell_output(const RhsArgs& a, int i, int j, const double 
                T);
     a.p_out[g] = p;
     a.T_out[g] = T;
+    if (FUSED == kStageBc && a.bc_fold)
+      bc_fold_ghosts(a, i, j, [&](long d, bool neg_u, bool neg_v) {
+        a.w_out[d] = wn[0];
+        a.w_out[d + G.sk] = neg_u ? -wn[1] : wn[1];
+        a.w_out[d + 2 * G.sk] = neg_v ? -wn[2] : wn[2];
+        a.w_out[d + 3 * G.sk] = wn[3];
+        a.w_out[d + 4 * G.sk] = wn[4];
+        a.p_out[d] = p;
+        a.T_out[d] = T;
+      });
   }
 }
 
@@ -505,17 +557,17 @@ __device__ inline SplitIn split_inputs(const RhsArgs& a, int i0, int j0) {
 // reference switches the sensor off, and for rhs_weno_*) keep H in their rhs
 // slots and are appended to the list as (patch << 32 | cell): one atomic per
 // wave.  split_compute runs on a staged tile.
-template <int V, bool P, bool FUSED, int BX, int BY>
+template <int V, bool P, int FUSED, int BX, int BY>
 __device__ inline void split_compute(const RhsArgs& a, int i0, int j0, unsigned pid, TileLds<BX, BY>& L,
                                      const SplitIn& in);
-template <int V, bool P, bool FUSED, int BX, int BY>
+template <int V, bool P, int FUSED, int BX, int BY>
 __device__ inline void split_body(const RhsArgs& a, int i0, int j0, unsigned pid, TileLds<BX, BY>& L) {
   stage_tile<BX, BY, false>(a, i0, j0, L);
   const SplitIn in = split_inputs<BX, BY>(a, i0, j0);
   __syncthreads();
   split_compute<V, P, FUSED, BX, BY>(a, i0, j0, pid, L, in);
 }
-template <int V, bool P, bool FUSED, int BX, int BY>
+template <int V, bool P, int FUSED, int BX, int BY>
 __device__ inline void split_compute(const RhsArgs& a, int i0, int j0, unsigned pid, TileLds<BX, BY>& L,
                                      const SplitIn& in) {
   using Sc = Scheme<V>;
@@ -582,7 +634,7 @@ __device__ inline void split_compute(const RhsArgs& a, int i0, int j0, unsigned 
 // and the normal velocity of the 2R+1 points stay live across the five
 // components; the other quotient a component's flux needs (v, s or T_n for
 // F; u, s or T_n for G) is used by that component alone and formed there.
-template <int V, bool P, bool FUSED>
+template <int V, bool P, int FUSED>
 __device__ inline double list_dir(const RhsArgs& a, long g, int j, int dir, int k, const double* w0,
                                   const double* un, double alpha) {
 #pragma clang fp contract(fast)
@@ -615,7 +667,7 @@ __device__ inline double list_dir(const RhsArgs& a, long g, int j, int dir, int 
 // components; the other quotient a component's flux needs (v, s or T_n for
 // F; u, s or T_n for G) is used by that component alone and formed there.
 // The component loop is kept rolled to bound register use.
-template <int V, bool P, bool FUSED>
+template <int V, bool P, int FUSED>
 __device__ inline void list_cell(const RhsArgs& a, int cell) {
 #pragma clang fp contract(fast)
   using Sc = Scheme<V>;
@@ -747,8 +799,8 @@ __device__ inline double list_item(const RhsArgs& a, int cell, int dir, int k, i
 constexpr int kListCellsPerWave = 6;
 constexpr int kListCellsPerBlock = 4 * kListCellsPerWave;
 
-template <int V, bool P, bool FUSED, class ArgsOf>
-__device__ inline void list_block(ArgsOf args_of, const unsigned long long* list, unsigned n) {
+template <int V, bool P, int FUSED, class ArgsOf>
+__device__ inline void list_block(ArgsOf args_of, const unsigned long long* list, unsigned n, bool fold = false) {
   const int t = threadIdx.x;
   const int lane = t & 63, slot = lane / 10, item = lane % 10, k = item % 5;
   const int lc = (t >> 6) * kListCellsPerWave + slot;
@@ -795,10 +847,38 @@ __device__ inline void list_block(ArgsOf args_of, const unsigned long long* list
       }
     }
   }
+  if constexpr (FUSED == kStageBc) {
+    // The boundary fold (fold: the arguments' bc_fold is set), as a second
+    // pass so that it adds no live registers to the one above: each lane
+    // (0, k) revisits its cells and copies the value it stored there
+    // (component k; p and T on lane k = 0) into the ghosts.
+    if (fold && item < 5 && slot < kListCellsPerWave)
+      for (unsigned q = blockIdx.x * kListCellsPerBlock + lc; q < n; q += gridDim.x * kListCellsPerBlock) {
+        const unsigned long long e = list[q];
+        const RhsArgs& a = args_of((int)(e >> 32));
+        const Geom& G = a.geom;
+        const int cell = (int)(e & 0xffffffffu);
+        const int i = cell % G.nx, j = cell / G.nx;
+        if (i != 0 && i != G.nx - 1 && j != 0 && j != G.ny - 1) continue;
+        const long g = (long)(i + G.g0) + (long)(j + G.g1) * G.sy;
+        double* w = a.w_out + (long)k * G.sk;
+        const double v = w[g];
+        bc_fold_ghosts(a, i, j, [&](long d, bool neg_u, bool neg_v) {
+          w[d] = (k == 1 && neg_u) || (k == 2 && neg_v) ? -v : v;
+        });
+        if (k == 0) {
+          const double p = a.p_out[g], T = a.T_out[g];
+          bc_fold_ghosts(a, i, j, [&](long d, bool, bool) {
+            a.p_out[d] = p;
+            a.T_out[d] = T;
+          });
+        }
+      }
+  }
 }
 
 // Dense tile: every cell on the WENO path.
-template <int V, bool P, bool FUSED, int BX, int BY>
+template <int V, bool P, int FUSED, int BX, int BY>
 __device__ inline void dense_body(const RhsArgs& a, int i0, int j0, TileLds<BX, BY>& L) {
   using Sh = TileShape<BX, BY>;
   const Geom& G = a.geom;
@@ -829,7 +909,7 @@ __device__ inline void dense_body(const RhsArgs& a, int i0, int j0, TileLds<BX, 
 // 10 x BX*BY doubles) and the y plane forms each component as (X + Y) + H,
 // the reference's order, and writes the cell.  One direction per thread
 // needs about half the registers of both, so more waves fit a SIMD.
-template <int V, bool P, bool FUSED, int BX, int BY>
+template <int V, bool P, int FUSED, int BX, int BY>
 __device__ inline void dense2_body(const RhsArgs& a, int i0, int j0, TileLds<BX, BY>& L, double* xs) {
 #pragma clang fp contract(fast)
   using Sh = TileShape<BX, BY>;

@@ -155,7 +155,9 @@ class DevicePatch:
 
     def advance_fused(self, nsteps, dt, variant=None, level=0, rk=RK3):
         """Same steps as advance(), one fused kernel per stage (bit-identical
-        results; the ghosts of w, p, T are not refreshed on return)."""
+        interiors).  Without an absorbing envelope the returned ghosts of
+        w, p, T hold the physical boundary of the returned state; with one
+        they are not refreshed."""
         lib = _lib.load()
         use_torch_stream()
         variant = self.wl.variant if variant is None else variant

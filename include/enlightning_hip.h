@@ -226,7 +226,11 @@ size_t enl_fused_scratch_bytes(int ifirst0, int ilast0, int ifirst1, int ilast1,
  * cells), and the stages ping-pong between w_n and two scratch states, so
  * the current -> scratch copy disappears.  scratch: device memory of at
  * least enl_fused_scratch_bytes().  On return w_n, pres and temp hold the
- * state on the interior (their ghosts are not refreshed). */
+ * state on the interior.  Without an absorbing envelope (absorb_width 0, or
+ * every side REFLECT) each stage kernel also writes the physical boundary
+ * ghosts of its output, so the physical boundary runs once per call rather
+ * than before every stage, and the returned ghosts hold the boundary of the
+ * returned state; with the envelope they are not refreshed. */
 int enl_advance_patch_fused(int variant, int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw,
                             const double *xlo, const double *dx, const enl_gas_t *gas, const int *reflect,
                             int order, const double *alpha_1, const double *alpha_2, const double *beta,
