@@ -749,6 +749,7 @@ struct Fp32Form {
   SwPatch* swtab = nullptr;       // 3 x n0 entries: the level-0 patches per stage block
   int n0 = 0, max_side0 = 0;
   long cells0 = 0;
+  long fcells0 = 0;  // ghosted cells of the level-0 patches (the switch's feature map)
   double sw_thr = 0;
   float sw_inv_r = 0, sw_inv_p = 0;
 };
@@ -963,6 +964,7 @@ int setup32(Batch* b) {
   F->n0 = 0;
   F->max_side0 = 0;
   F->cells0 = 0;
+  F->fcells0 = 0;
   long offd = 0;
   for (int p = 0; p < b->npatch; p++) {
     const Patch32& q = F->hp[p];
@@ -996,13 +998,15 @@ int setup32(Batch* b) {
         S.g = q.g;
         S.rs = (int)(sizeof(RowF) / sizeof(float));
         S.sy = q.sy;
+        S.foff = F->fcells0;
       }
+      F->fcells0 += q.sk;
       q0++;
     }
     rc = check_hip(hipMemcpy(F->swtab, t.data(), sizeof(SwPatch) * t.size(), hipMemcpyHostToDevice),
                    "switch table upload");
   }
-  if (!rc && F->sw) rc = switch_reserve(F->sw, F->cells0);
+  if (!rc && F->sw) rc = switch_reserve(F->sw, F->cells0, F->fcells0);
   F->h_args.clear();
   return rc;
 }
@@ -1062,7 +1066,7 @@ int enqueue32(Batch* b, const int* reflect, int order, int nsteps, hipStream_t s
         if (lev == 0 && F->sw && (F->every_stage || st == 0)) {
           // the learned switch decides level 0's cells from this stage's input
           int rc = switch_decide_batch(F->sw, F->swtab + (size_t)in * F->n0, F->n0, F->max_side0, F->cells0,
-                                       F->sw_thr, F->sw_inv_r, F->sw_inv_p, F->lo, F->hi, s);
+                                       F->fcells0, F->sw_thr, F->sw_inv_r, F->sw_inv_p, F->lo, F->hi, s);
           if (!rc && F->totals) rc = switch_accumulate(F->sw, F->totals, s);
           if (rc) return rc;
         }
@@ -1253,6 +1257,8 @@ int enl_batch_fp32_set_switch(void* handle, void* sw, float lo, float hi, int ev
   Batch* b = (Batch*)handle;
   if (!b) return set_error(-2, "enl_batch_fp32_set_switch: no batch");
   if (sw && !(lo <= hi)) return set_error(-2, "enl_batch_fp32_set_switch: lo > hi");
+  // the features of a 5 x 5 window read one cell past it
+  if (sw && b->g < 3) return set_error(-2, "enl_batch_fp32_set_switch: the switch's feature window needs 3 ghost cells");
   if (totals && !is_device_ptr(totals)) return set_error(-4, "enl_batch_fp32_set_switch: totals must be device memory");
   Fp32Form* F = form(b);
   F->sw = (Switch*)sw;
@@ -1260,7 +1266,7 @@ int enl_batch_fp32_set_switch(void* handle, void* sw, float lo, float hi, int ev
   F->hi = sw ? hi : 1.0f;
   F->every_stage = every_stage ? 1 : 0;
   F->totals = sw ? totals : nullptr;
-  if (F->sw && F->ready) return switch_reserve(F->sw, F->cells0);
+  if (F->sw && F->ready) return switch_reserve(F->sw, F->cells0, F->fcells0);
   return 0;
 }
 

@@ -268,6 +268,8 @@ extern "C" int enl_advance_patch_learned(void* sw, float lo, float hi, int every
   if (!(variant == kHybridHighLlf || variant == kHybridHigh || variant == kHybridLowLlf || variant == kHybridLow))
     return set_error(-1, "the learned switch replaces the sensor of the hybrid variants only");
   if (!(lo <= hi)) return set_error(-2, "enl_advance_patch_learned: lo > hi");
+  // the features of a 5 x 5 window read one cell past it: 3 ghost cells
+  if (gcw < 3) return set_error(-2, "enl_advance_patch_learned: the switch's feature window needs 3 ghost cells");
   if (!is_device_ptr(decide) || (totals && !is_device_ptr(totals)))
     return set_error(-4, "enl_advance_patch_learned needs device decision and count arrays");
   LearnedSwitch L;

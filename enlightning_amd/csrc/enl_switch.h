@@ -22,6 +22,7 @@ struct SwPatch {
   const unsigned char* amb_weno;
   int nx, ny, g, rs;
   long sy;
+  long foff;  // offset of the patch's ghosted cells in the switch's feature map
 };
 
 // The decisions of every patch of the table `patches` (device, npatch
@@ -34,11 +35,13 @@ struct SwPatch {
 // inv_r = 1/sqrt(eps), inv_p = 1/(c^2 sqrt(eps)) normalise the features,
 // thr = 5e-5 eps.  Counts (candidates, WENO, undecided) of the call stay in
 // the switch for switch_accumulate.  Launches only (no allocation) once
-// switch_reserve has sized the candidate list for `cells`.
-int switch_decide_batch(Switch* sw, const SwPatch* patches, int npatch, int max_side, long cells, double thr,
-                        float inv_r, float inv_p, float lo, float hi, hipStream_t s);
-// Candidate list capacity for `cells` level-0 cells (allocates if needed).
-int switch_reserve(Switch* sw, long cells);
+// switch_reserve has sized the candidate list for `cells` and the feature
+// map for `fcells` ghosted cells (the patches' foff ranges).
+int switch_decide_batch(Switch* sw, const SwPatch* patches, int npatch, int max_side, long cells, long fcells,
+                        double thr, float inv_r, float inv_p, float lo, float hi, hipStream_t s);
+// Candidate list capacity for `cells` level-0 cells and feature map for
+// `fcells` ghosted cells (allocates if needed).
+int switch_reserve(Switch* sw, long cells, long fcells);
 // Adds the last decisions' counts to totals[3] (device), on stream s.
 int switch_accumulate(Switch* sw, unsigned long long* totals, hipStream_t s);
 

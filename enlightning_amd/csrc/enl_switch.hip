@@ -190,7 +190,15 @@ struct DecideArgs {
   unsigned char* decide;
   unsigned* cand;    // candidate cells (interior index)
   unsigned* ncand;   // [0] candidates, [1] WENO, [2] undecided
+  // per ghosted cell (row stride F.sy), the four BF16 features of that
+  // point, written for the points of the candidates' windows
+  unsigned long long* feat;
 };
+
+__device__ inline unsigned long long pack_bf16x4(const float f[4]) {
+  return (unsigned long long)f2bf(f[0]) | (unsigned long long)f2bf(f[1]) << 16 |
+         (unsigned long long)f2bf(f[2]) << 32 | (unsigned long long)f2bf(f[3]) << 48;
+}
 
 // n[0] network candidates (the list length), n[1] network WENO, n[2]
 // network undecided, n[3] cells the prefilter itself leaves to the sensor.
@@ -221,6 +229,7 @@ constexpr int kPWaves = kPBX * kPBY / 64;
 // a wave's candidates in row order, then lane order.
 struct PrefilterCompact {
   unsigned wcount[kPWaves], wbase[kPWaves];
+  unsigned total;  // the block's candidates
 };
 __device__ inline unsigned compact_slot(PrefilterCompact& pc, const unsigned long long (&m)[kPRows], unsigned* ncand,
                                         int tid) {
@@ -237,6 +246,7 @@ __device__ inline unsigned compact_slot(PrefilterCompact& pc, const unsigned lon
 #pragma unroll
     for (int w = 0; w < kPWaves; w++) tot += pc.wcount[w];
     unsigned b = tot ? atomicAdd(ncand, tot) : 0u;
+    pc.total = tot;
 #pragma unroll
     for (int w = 0; w < kPWaves; w++) {
       pc.wbase[w] = b;
@@ -245,6 +255,27 @@ __device__ inline unsigned compact_slot(PrefilterCompact& pc, const unsigned lon
   }
   __syncthreads();
   return pc.wbase[wave];
+}
+
+// The candidates' window points of a tile into `need` (the tile and its
+// 2-cell halo, row stride kPW), then the features of each marked point into
+// feat (indexed by the ghosted cell q(i, j)).  Called by the whole block,
+// and only by blocks with candidates.
+template <class FeatOf>
+__device__ inline void window_features(unsigned char* need, const bool (&cand)[kPRows], int tid, FeatOf feat_of) {
+  for (int e = tid; e < kPH * kPW; e += kPBX * kPBY) need[e] = 0;
+  __syncthreads();
+  const int tx = tid % kPBX, ty = tid / kPBX;
+#pragma unroll
+  for (int r = 0; r < kPRows; r++)
+    if (cand[r]) {
+      const int c0 = (ty + kPBY * r + 2) * kPW + tx + 2;
+      for (int dj = -2; dj <= 2; dj++)
+        for (int di = -2; di <= 2; di++) need[c0 + dj * kPW + di] = 1;
+    }
+  __syncthreads();
+  for (int e = tid; e < kPH * kPW; e += kPBX * kPBY)
+    if (need[e]) feat_of(e % kPW - 2, e / kPW - 2);
 }
 
 __global__ void __launch_bounds__(kPBX * kPBY) prefilter_kernel(DecideArgs a) {
@@ -295,6 +326,20 @@ __global__ void __launch_bounds__(kPBX * kPBY) prefilter_kernel(DecideArgs a) {
       a.cand[slot + __popcll(m[r] & __lanemask_lt())] = (unsigned)((long)(i0 + tx) + (long)(j0 + ty + kPBY * r) * a.nx);
     slot += (unsigned)__popcll(m[r]);
   }
+  // The features of the points of the candidates' 5 x 5 windows, each point
+  // once per tile (the tile and its 2-cell halo hold every window): the
+  // network reads them as 8 bytes per point instead of forming each from six
+  // loads.  Neighbouring tiles can write the same point: the same bits, from
+  // the same state.  (a.feat and pc.total are the same for the whole block.)
+  if (a.feat && pc.total) {
+    __shared__ unsigned char need[kPH * kPW];
+    window_features(need, cand, tid, [&](int di, int dj) {
+      const long q = (long)(i0 + di + a.g) + (long)(j0 + dj + a.g) * a.F.sy;
+      float f[4];
+      point_features(a.F, q, 0, 0, f);
+      a.feat[q] = pack_bf16x4(f);
+    });
+  }
 }
 
 // The same prefilter over the level-0 patches of an FP32-perturbation
@@ -312,8 +357,20 @@ __global__ void __launch_bounds__(kPBX * kPBY) prefilter_kernel(DecideArgs a) {
 // stratified ambient; the FP32 form's ambient correction holds the
 // ambient's tendency under the sensor's choice there), gets no network call
 // and is left to the sensor (2).
+// The four BF16-rounded features of point q of an FP32 perturbation state:
+// x/y differences of rho' and p', asinh-normalised.
+__device__ inline void point_features32(const float* r, const float* p, long q, long sy, float inv_r, float inv_p,
+                                        float f[4]) {
+  const float r0 = r[q], p0 = p[q];
+  f[0] = bf2f(f2bf(asinh_fast((r[q + 1] - r0) * inv_r)));
+  f[1] = bf2f(f2bf(asinh_fast((r[q + sy] - r0) * inv_r)));
+  f[2] = bf2f(f2bf(asinh_fast((p[q + 1] - p0) * inv_p)));
+  f[3] = bf2f(f2bf(asinh_fast((p[q + sy] - p0) * inv_p)));
+}
+
 __global__ void __launch_bounds__(kPBX * kPBY) prefilter32_kernel(const SwPatch* __restrict__ pt, float thr,
-                                                                 unsigned long long* cand, unsigned* ncand) {
+                                                                 unsigned long long* cand, unsigned* ncand,
+                                                                 unsigned long long* feat, float inv_r, float inv_p) {
   __shared__ float rt[kPH * kPW];
   __shared__ float dr[kPH];
   __shared__ PrefilterCompact pc;
@@ -380,6 +437,17 @@ __global__ void __launch_bounds__(kPBX * kPBY) prefilter32_kernel(const SwPatch*
           ((unsigned long long)pid << 32) | (unsigned)((i0 + tx) + (j0 + ty + kPBY * r) * nx);
     slot += (unsigned)__popcll(m[r]);
   }
+  // the features of the candidates' windows (see prefilter_kernel)
+  if (pc.total) {
+    __shared__ unsigned char need[kPH * kPW];
+    unsigned long long* f = feat + P.foff;
+    window_features(need, c, tid, [&](int di, int dj) {
+      const long q = (long)(i0 + di + g) + (long)(j0 + dj + g) * P.sy;
+      float x[4];
+      point_features32(P.rho, P.p, q, P.sy, inv_r, inv_p, x);
+      f[q] = pack_bf16x4(x);
+    });
+  }
 }
 
 // Inference on the matrix cores: each wave takes 16 candidates at a time
@@ -406,6 +474,7 @@ struct PatchSrc {
   int nx, g;
   const unsigned* cand;
   unsigned char* decide;
+  const unsigned long long* feat;  // the prefilter's window features
   struct Loc {
     long g;
   };
@@ -414,7 +483,11 @@ struct PatchSrc {
     const int i = (int)(c % (unsigned)nx), j = (int)(c / (unsigned)nx);
     return Loc{(long)(i + g) + (long)(j + g) * F.sy};
   }
-  __device__ void features(const Loc& L, int di, int dj, float f[4]) const { point_features(F, L.g, di, dj, f); }
+  __device__ void features(const Loc& L, int di, int dj, float f[4]) const {
+    const unsigned long long x = feat[L.g + di + (long)dj * F.sy];
+#pragma unroll
+    for (int k = 0; k < 4; k++) f[k] = bf2f((unsigned short)(x >> (16 * k)));
+  }
   __device__ void put(unsigned q, unsigned char d) const { decide[cand[q]] = d; }
 };
 
@@ -427,9 +500,9 @@ struct PatchSrc {
 struct Batch32Src {
   const SwPatch* pt;
   const unsigned long long* cand;
-  float inv_r, inv_p;
+  const unsigned long long* feat;  // the prefilter's window features
   struct Loc {
-    const float *r, *p;
+    const unsigned long long* f;
     long sy;
   };
   __device__ Loc locate(unsigned q) const {
@@ -438,15 +511,12 @@ struct Batch32Src {
     const unsigned cell = (unsigned)c;
     const int i = (int)(cell % (unsigned)P.nx), j = (int)(cell / (unsigned)P.nx);
     const long gx = (long)(i + P.g) + (long)(j + P.g) * P.sy;
-    return Loc{P.rho + gx, P.p + gx, P.sy};
+    return Loc{feat + P.foff + gx, P.sy};
   }
   __device__ void features(const Loc& L, int di, int dj, float f[4]) const {
-    const long q = di + (long)dj * L.sy;
-    const float r0 = L.r[q], p0 = L.p[q];
-    f[0] = bf2f(f2bf(asinh_fast((L.r[q + 1] - r0) * inv_r)));
-    f[1] = bf2f(f2bf(asinh_fast((L.r[q + L.sy] - r0) * inv_r)));
-    f[2] = bf2f(f2bf(asinh_fast((L.p[q + 1] - p0) * inv_p)));
-    f[3] = bf2f(f2bf(asinh_fast((L.p[q + L.sy] - p0) * inv_p)));
+    const unsigned long long x = L.f[di + (long)dj * L.sy];
+#pragma unroll
+    for (int k = 0; k < 4; k++) f[k] = bf2f((unsigned short)(x >> (16 * k)));
   }
   __device__ void put(unsigned q, unsigned char d) const {
     const unsigned long long c = cand[q];
@@ -720,9 +790,22 @@ struct Switch {
   void* cand = nullptr;  // candidate list of the decisions
   unsigned* ncand = nullptr;
   size_t cand_bytes = 0;
+  unsigned long long* feat = nullptr;  // window features, per ghosted cell
+  size_t feat_bytes = 0;
 };
 
 namespace {
+
+int grow_feat(Switch* s, size_t bytes) {
+  if (bytes <= s->feat_bytes) return 0;
+  if (s->feat) (void)hipFree(s->feat);
+  s->feat = nullptr;
+  s->feat_bytes = 0;
+  hipError_t e = hipMalloc((void**)&s->feat, bytes);
+  if (e != hipSuccess) return check_hip(e, "hipMalloc(switch features)");
+  s->feat_bytes = bytes;
+  return 0;
+}
 
 int grow_cand(Switch* s, size_t bytes) {
   if (bytes <= s->cand_bytes) return 0;
@@ -768,6 +851,7 @@ int switch_decide(Switch* sw, const Geom& G, const double* w_n, const double* pr
                   float hi, unsigned char* decide, hipStream_t s) {
   const long ncell = (long)G.nx * G.ny;
   int rc = grow_cand(sw, sizeof(unsigned) * (size_t)ncell);
+  if (!rc) rc = grow_feat(sw, sizeof(unsigned long long) * (size_t)G.sk);
   if (rc) return rc;
   DecideArgs a;
   a.F = make_feat(G, w_n, pres, gas);
@@ -778,12 +862,14 @@ int switch_decide(Switch* sw, const Geom& G, const double* w_n, const double* pr
   a.decide = decide;
   a.cand = (unsigned*)sw->cand;
   a.ncand = sw->ncand;
+  a.feat = sw->feat;
   PatchSrc src;
   src.F = a.F;
   src.nx = G.nx;
   src.g = G.g0;
   src.cand = a.cand;
   src.decide = decide;
+  src.feat = sw->feat;
   hipLaunchKernelGGL(decide_reset_kernel, dim3(1), dim3(64), 0, s, sw->ncand);
   const unsigned ntile = (unsigned)(((G.nx + kPBX - 1) / kPBX) * ((G.ny + kPTH - 1) / kPTH));
   hipLaunchKernelGGL(prefilter_kernel, dim3(ntile), dim3(kPBX * kPBY), 0, s, a);
@@ -791,24 +877,26 @@ int switch_decide(Switch* sw, const Geom& G, const double* w_n, const double* pr
   return check_hip(hipGetLastError(), "switch decisions");
 }
 
-int switch_reserve(Switch* sw, long cells) {
-  return grow_cand(sw, sizeof(unsigned long long) * (size_t)(cells > 0 ? cells : 1));
+int switch_reserve(Switch* sw, long cells, long fcells) {
+  int rc = grow_cand(sw, sizeof(unsigned long long) * (size_t)(cells > 0 ? cells : 1));
+  return rc ? rc : grow_feat(sw, sizeof(unsigned long long) * (size_t)(fcells > 0 ? fcells : 1));
 }
 
-int switch_decide_batch(Switch* sw, const SwPatch* patches, int npatch, int max_side, long cells, double thr,
-                        float inv_r, float inv_p, float lo, float hi, hipStream_t s) {
+int switch_decide_batch(Switch* sw, const SwPatch* patches, int npatch, int max_side, long cells, long fcells,
+                        double thr, float inv_r, float inv_p, float lo, float hi, hipStream_t s) {
   if (npatch <= 0) return 0;
   if (!sw->cand || max_side <= 0 || sw->cand_bytes < sizeof(unsigned long long) * (size_t)cells)
     return set_error(-2, "switch_decide_batch: candidate list not reserved for %ld cells", cells);
+  if (!sw->feat || sw->feat_bytes < sizeof(unsigned long long) * (size_t)fcells)
+    return set_error(-2, "switch_decide_batch: feature map not reserved for %ld cells", fcells);
   const int nt = ((max_side + kPBX - 1) / kPBX) * ((max_side + kPTH - 1) / kPTH);
   Batch32Src src;
   src.pt = patches;
   src.cand = (const unsigned long long*)sw->cand;
-  src.inv_r = inv_r;
-  src.inv_p = inv_p;
+  src.feat = sw->feat;
   hipLaunchKernelGGL(decide_reset_kernel, dim3(1), dim3(64), 0, s, sw->ncand);
   hipLaunchKernelGGL(prefilter32_kernel, dim3(nt, npatch), dim3(kPBX * kPBY), 0, s, patches, (float)thr,
-                     (unsigned long long*)sw->cand, sw->ncand);
+                     (unsigned long long*)sw->cand, sw->ncand, sw->feat, inv_r, inv_p);
   hipLaunchKernelGGL(infer_kernel<Batch32Src>, dim3(1024), dim3(64 * kWaves), 0, s, src, sw->ncand, lo, hi, sw->wb);
   return check_hip(hipGetLastError(), "batched switch decisions");
 }
@@ -851,7 +939,7 @@ int enl_switch_destroy(void* handle) {
   Switch* s = (Switch*)handle;
   if (!s) return 0;
   (void)hipDeviceSynchronize();
-  void* ptrs[] = {s->w, s->wb, s->m, s->v, s->g, s->loss, s->ncand, s->cand,
+  void* ptrs[] = {s->w, s->wb, s->m, s->v, s->g, s->loss, s->ncand, s->cand, s->feat,
                   s->t.h1, s->t.h2, s->t.dh1, s->t.dh2, s->t.dz, s->t.p};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

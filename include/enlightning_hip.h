@@ -468,7 +468,8 @@ int enl_batch_fp32_store(void *handle);
  * per decision the candidates (cells not provably DRP), the network's WENO
  * cells and the cells left to the sensor.  sw == NULL restores the sensor.  The switch must
  * outlive its use by this batch.  With lo = 0, hi = 1 and every_stage set,
- * every decision is the sensor's. */
+ * every decision is the sensor's.  The hierarchy's ghost width must be at
+ * least 3 (the features of a 5 x 5 window read one cell past it). */
 int enl_batch_fp32_set_switch(void *handle, void *sw, float lo, float hi, int every_stage,
                               unsigned long long *totals);
 /* The decision map of level-0 patch pid as the switch last wrote it
@@ -501,7 +502,10 @@ int enl_switch_gradients(void *handle, const unsigned short *X, const float *y, 
 int enl_switch_train_step(void *handle, const unsigned short *X, const float *y, const float *wt, int n, float wsum,
                           float lr, float *loss);
 /* Decisions (0 DRP, 1 WENO, 2 sensor) of a patch state into decide (nx*ny,
- * device); counts (host, optional): candidates, WENO, undecided. */
+ * device); counts (host, optional): candidates, WENO, undecided.  gcw >= 3:
+ * the features of a cell's 5 x 5 window read one cell past it.  The switch
+ * keeps a candidate list and a feature map sized for the largest patch
+ * decided so far. */
 int enl_switch_decide(void *handle, int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw, const double *w_n,
                       const double *pres, const enl_gas_t *gas, float lo, float hi, unsigned char *decide,
                       unsigned *counts);
@@ -509,7 +513,7 @@ int enl_switch_decide(void *handle, int ifirst0, int ilast0, int ifirst1, int il
  * switch deciding level 0's cells: before every RK stage (every_stage != 0)
  * or at the first stage of each step, the map kept for the step's other
  * stages (every_stage == 0); totals (device, 3 x u64, optional) accumulate
- * the decision counts. */
+ * the decision counts.  gcw >= 3, as for enl_switch_decide. */
 int enl_advance_patch_learned(void *sw, float lo, float hi, int every_stage, unsigned char *decide,
                               unsigned long long *totals,
                               int variant, int ifirst0, int ilast0, int ifirst1, int ilast1, int gcw,

@@ -96,6 +96,20 @@ def test_undecided_switch_equals_plain():
     sw.close()
 
 
+def test_learned_path_needs_three_ghost_cells():
+    """A cell's features read one cell past its 5 x 5 window: the learned
+    advance refuses a patch with two ghost cells (enough for hybrid_low_llf
+    on its own) before any launch."""
+    import dataclasses
+    gas = Gas()
+    wl = dataclasses.replace(SMALL, g=2)
+    b = S.channel_patch(wl, gas, seed=12)
+    sw = S.Switch()
+    with pytest.raises(RuntimeError, match="3 ghost cells"):
+        sw.advance(b, 1, wl.dt(gas), lo=0.0, hi=1.0, variant="hybrid_low_llf")
+    sw.close()
+
+
 @pytest.fixture(scope="module")
 def trained():
     gas = Gas()
