@@ -23,6 +23,8 @@
 // weights, BF16 copies for the forward pass.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -846,6 +848,29 @@ Feat make_feat(const Geom& G, const double* w_n, const double* pres, const enl_g
 
 }  // namespace
 
+// The inference grid: the workgroups the device holds at once.  Every
+// workgroup takes the same share of the candidates, so a larger grid would
+// leave a partly filled second round.
+template <class Src>
+unsigned infer_grid() {
+  static std::atomic<int> cached[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 1024;
+  int n = cached[dev].load(std::memory_order_relaxed);
+  if (n <= 0) {
+    int ncu = 0, per = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) ncu = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(&infer_kernel<Src>),
+                                                     64 * kWaves, 0) != hipSuccess ||
+        per <= 0)
+      per = 1;
+    (void)hipGetLastError();  // a failed query leaves no error for the launch checks
+    n = ncu * per;
+    cached[dev].store(n, std::memory_order_relaxed);
+  }
+  return (unsigned)n;
+}
+
 // The decisions of one patch state into `decide` (nx*ny, device), on stream s.
 int switch_decide(Switch* sw, const Geom& G, const double* w_n, const double* pres, const enl_gas_t* gas, float lo,
                   float hi, unsigned char* decide, hipStream_t s) {
@@ -873,7 +898,8 @@ int switch_decide(Switch* sw, const Geom& G, const double* w_n, const double* pr
   hipLaunchKernelGGL(decide_reset_kernel, dim3(1), dim3(64), 0, s, sw->ncand);
   const unsigned ntile = (unsigned)(((G.nx + kPBX - 1) / kPBX) * ((G.ny + kPTH - 1) / kPTH));
   hipLaunchKernelGGL(prefilter_kernel, dim3(ntile), dim3(kPBX * kPBY), 0, s, a);
-  hipLaunchKernelGGL(infer_kernel<PatchSrc>, dim3(1024), dim3(64 * kWaves), 0, s, src, sw->ncand, lo, hi, sw->wb);
+  hipLaunchKernelGGL(infer_kernel<PatchSrc>, dim3(infer_grid<PatchSrc>()), dim3(64 * kWaves), 0, s, src, sw->ncand, lo,
+                     hi, sw->wb);
   return check_hip(hipGetLastError(), "switch decisions");
 }
 
@@ -897,7 +923,8 @@ int switch_decide_batch(Switch* sw, const SwPatch* patches, int npatch, int max_
   hipLaunchKernelGGL(decide_reset_kernel, dim3(1), dim3(64), 0, s, sw->ncand);
   hipLaunchKernelGGL(prefilter32_kernel, dim3(nt, npatch), dim3(kPBX * kPBY), 0, s, patches, (float)thr,
                      (unsigned long long*)sw->cand, sw->ncand, sw->feat, inv_r, inv_p);
-  hipLaunchKernelGGL(infer_kernel<Batch32Src>, dim3(1024), dim3(64 * kWaves), 0, s, src, sw->ncand, lo, hi, sw->wb);
+  hipLaunchKernelGGL(infer_kernel<Batch32Src>, dim3(infer_grid<Batch32Src>()), dim3(64 * kWaves), 0, s, src,
+                     sw->ncand, lo, hi, sw->wb);
   return check_hip(hipGetLastError(), "batched switch decisions");
 }
 
