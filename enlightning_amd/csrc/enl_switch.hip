@@ -467,8 +467,8 @@ constexpr int kWaves = 4;
 constexpr int kH1Row = kH1 + 4;  // LDS row of a cell's layer-1 output (bf16)
 
 // A candidate source gives, per candidate q of its list, the location of its
-// cell (locate), the four features of window point (di, dj) from there
-// (features), and stores the decision (put).
+// cell (locate), the four packed BF16 features of window point (di, dj)
+// from there (window), and stores the decision (put).
 //
 // One FP64 patch state (enl_switch_decide, the learned fused advance).
 struct PatchSrc {
@@ -485,10 +485,11 @@ struct PatchSrc {
     const int i = (int)(c % (unsigned)nx), j = (int)(c / (unsigned)nx);
     return Loc{(long)(i + g) + (long)(j + g) * F.sy};
   }
-  __device__ void features(const Loc& L, int di, int dj, float f[4]) const {
-    const unsigned long long x = feat[L.g + di + (long)dj * F.sy];
-#pragma unroll
-    for (int k = 0; k < 4; k++) f[k] = bf2f((unsigned short)(x >> (16 * k)));
+  // the packed features of window point (di, dj), 0 where !ok (the load
+  // itself always reads a valid address, so that it can be issued early)
+  __device__ unsigned long long window(const Loc& L, bool ok, int di, int dj) const {
+    const unsigned long long x = *(ok ? feat + L.g + di + (long)dj * F.sy : feat);
+    return ok ? x : 0ull;
   }
   __device__ void put(unsigned q, unsigned char d) const { decide[cand[q]] = d; }
 };
@@ -515,10 +516,9 @@ struct Batch32Src {
     const long gx = (long)(i + P.g) + (long)(j + P.g) * P.sy;
     return Loc{feat + P.foff + gx, P.sy};
   }
-  __device__ void features(const Loc& L, int di, int dj, float f[4]) const {
-    const unsigned long long x = L.f[di + (long)dj * L.sy];
-#pragma unroll
-    for (int k = 0; k < 4; k++) f[k] = bf2f((unsigned short)(x >> (16 * k)));
+  __device__ unsigned long long window(const Loc& L, bool ok, int di, int dj) const {
+    const unsigned long long x = *(ok ? L.f + di + (long)dj * L.sy : feat);
+    return ok ? x : 0ull;
   }
   __device__ void put(unsigned q, unsigned char d) const {
     const unsigned long long c = cand[q];
@@ -561,26 +561,43 @@ __global__ void __launch_bounds__(64 * kWaves) infer_kernel(Src src, unsigned* n
 
   const unsigned n = ncand[0];
   const unsigned groups = (n + 15) / 16;
+  const unsigned stride = gridDim.x * kWaves;
   unsigned nweno = 0, nfall = 0;
-  for (unsigned grp = blockIdx.x * kWaves + wv; grp < groups; grp += gridDim.x * kWaves) {
-    const unsigned q = grp * 16 + row;
-    const bool live = q < n;
-    typename Src::Loc loc{};
-    if (live) loc = src.locate(q);
-    // layer 1: features of this lane's points straight into the A operands
+  // The wave's groups as a software pipeline: while a group multiplies, the
+  // features of the next group and the candidate cell of the one after are
+  // in flight, so a group waits on at most one memory latency, not on the
+  // cell and then its window in a row.
+  auto cell_of = [&](unsigned gp) {
+    typename Src::Loc l{};
+    if (gp < groups && gp * 16 + row < n) l = src.locate(gp * 16 + row);
+    return l;
+  };
+  auto load_window = [&](unsigned gp, const typename Src::Loc& l, unsigned long long(&x)[7]) {
+    const bool live = gp < groups && gp * 16 + row < n;
+#pragma unroll
+    for (int sk = 0; sk < 7; sk++) {
+      const int pt = 4 * sk + quad;
+      x[sk] = src.window(l, live && pt < 25, pt % 5 - 2, pt / 5 - 2);
+    }
+  };
+  unsigned grp = blockIdx.x * kWaves + wv;
+  unsigned long long x[7];
+  load_window(grp, cell_of(grp), x);
+  typename Src::Loc next = cell_of(grp + stride);
+  for (; grp < groups; grp += stride) {
+    unsigned long long xn[7];
+    load_window(grp + stride, next, xn);
+    next = cell_of(grp + 2 * stride);
+    // layer 1: the BF16 features of this lane's seven window points
+    // straight into the A operands
     f32x4 acc1[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) acc1[t] = f32x4{0, 0, 0, 0};
 #pragma unroll
     for (int sk = 0; sk < 7; sk++) {
-      const int pt = 4 * sk + quad;
-      bf16x4 av = {0, 0, 0, 0};
-      if (live && pt < 25) {
-        float f[4];
-        src.features(loc, pt % 5 - 2, pt / 5 - 2, f);
+      bf16x4 av;
 #pragma unroll
-        for (int v = 0; v < 4; v++) av[v] = (short)f2bf(f[v]);
-      }
+      for (int v = 0; v < 4; v++) av[v] = (short)(unsigned short)(x[sk] >> (16 * v));
 #pragma unroll
       for (int t = 0; t < 4; t++) acc1[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bw1[sk][t], acc1[t], 0, 0, 0);
     }
@@ -633,6 +650,8 @@ __global__ void __launch_bounds__(64 * kWaves) infer_kernel(Src src, unsigned* n
         }
       }
     }
+#pragma unroll
+    for (int sk = 0; sk < 7; sk++) x[sk] = xn[sk];
   }
   for (int off = 32; off > 0; off >>= 1) {
     nweno += __shfl_down(nweno, off);
